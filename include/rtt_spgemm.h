@@ -1,0 +1,303 @@
+/*
+ * rtt_spgemm.h -- C ABI of the MI355X-native tiled FP64 SpGEMM engine.
+ *
+ * The entry points replace the header-only C++ interface that the
+ * reference's driver (src/main.cu) calls for the C = A*A (-aat 0) and
+ * C = A*A^T (-aat 1) path.  Each declaration names the reference function it
+ * replaces.  All types are plain C: device buffers are opaque handles or raw
+ * pointers, host buffers are malloc'ed by the library and released with the
+ * matching rtt_destroy_* call.  No HIP or torch type appears here.
+ *
+ * Threading: calls on one rtt_prepared_a handle must not overlap (the handle
+ * owns the reusable C output workspace).  Distinct handles may be used from
+ * different host threads.
+ */
+#ifndef RTT_SPGEMM_H_
+#define RTT_SPGEMM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes (the reference returns bool + stderr text; see               */
+/* src/dmma_spgemm.h:1192-1199).  rtt_last_error() gives the text.           */
+/* ------------------------------------------------------------------------ */
+enum rtt_status {
+    RTT_OK = 0,
+    RTT_ERR_INVALID = 1,     /* malformed input or arguments               */
+    RTT_ERR_OOM = 2,         /* host or device allocation failed            */
+    RTT_ERR_OVERFLOW = 3,    /* a count exceeds the int32 legacy ABI        */
+    RTT_ERR_DEVICE = 4,      /* HIP runtime error or device error flag      */
+    RTT_ERR_UNSUPPORTED = 5  /* option not implemented by this build        */
+};
+
+/* Thread-local text of the last failure ("" when none). */
+const char *rtt_last_error(void);
+
+/* Version string of the library build. */
+const char *rtt_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Host matrices: the legacy SMatrix layout, field for field                 */
+/* (src/common.h:48-73).  A maintainer can cast an SMatrix* to rtt_smatrix*. */
+/* ------------------------------------------------------------------------ */
+typedef struct rtt_smatrix {
+    int m;
+    int n;
+    int nnz;
+    int isSymmetric;
+    double *value;
+    int *columnindex;
+    int *rowpointer;
+    int tilem;
+    int tilen;
+    int *tile_ptr;
+    int *tile_columnidx;
+    int *tile_rowidx;
+    int *tile_nnz;
+    int numtile;
+    double *tile_csr_Value;
+    unsigned char *tile_csr_Col;
+    unsigned char *tile_csr_Ptr;
+    unsigned short *mask;
+    int *csc_tile_ptr;
+    int *csc_tile_rowidx;
+} rtt_smatrix;
+
+/* Frees every non-NULL array of *m with free() and zeroes the struct
+ * (destroy_output_matrix / free_host_csr, src/main.cu:1059-1089). */
+void rtt_destroy_smatrix(rtt_smatrix *m);
+
+/* Matrix Market reader with mmio_allinone semantics
+ * (src/mmio_highlevel.h:593-759): 0-based CSR, symmetric/hermitian storage
+ * mirrored, rows kept in file order, duplicates kept.  When
+ * reference_values != 0 every value is replaced by (i % 10) as the
+ * reference driver's load_matrix does (src/main.cu:1091-1112). */
+int rtt_read_mtx(const char *path, int reference_values, rtt_smatrix *out);
+
+/* Writes a CSR matrix as "matrix coordinate real general", %.17g values. */
+int rtt_write_mtx(const char *path, const rtt_smatrix *csr);
+
+/* ------------------------------------------------------------------------ */
+/* Device                                                                    */
+/* ------------------------------------------------------------------------ */
+/* Selects the HIP device (cudaSetDevice, src/main.cu:1526) and creates the
+ * library stream.  Must precede every call that touches the GPU. */
+int rtt_init(int device);
+/* Blocks until all work issued by the library has completed. */
+int rtt_sync(void);
+
+/* Device tile view: DmmaDeviceTiles (src/dmma_device_tiles.h:8-28).
+ * Pointers are device addresses.  A: 8x4 tiles, row-major payload
+ * (physical = r*4+k).  B: 4x8 tiles, column-major payload
+ * (physical = c*4+k) plus a tile CSC.  A tile is dense iff its payload
+ * span value_offsets[t+1]-value_offsets[t] is 32; otherwise its values are
+ * packed in increasing mask-bit order. */
+typedef struct rtt_device_tiles {
+    int rows;
+    int cols;
+    int tile_rows;
+    int tile_cols;
+    int tile_row_count;
+    int tile_col_count;
+    int num_tiles;
+    int payload_size;
+    int dense_tiles;
+    int sparse_tiles;
+    const int *tile_row_ptr;
+    const int *tile_col_idx;
+    const int *value_offsets;
+    const uint32_t *masks;
+    const double *values;
+    const int *tile_col_ptr;
+    const int *tile_row_idx;
+    const int *csc_tile_ids;
+} rtt_device_tiles;
+
+/* Copies a device tile view to caller-provided host arrays (any may be NULL;
+ * sizes follow the view's counts). */
+int rtt_download_tiles(const rtt_device_tiles *view, int *tile_row_ptr,
+                       int *tile_col_idx, int *value_offsets,
+                       uint32_t *masks, double *values, int *tile_col_ptr,
+                       int *tile_row_idx, int *csc_tile_ids);
+
+/* ------------------------------------------------------------------------ */
+/* Offline A preparation                                                     */
+/* ------------------------------------------------------------------------ */
+typedef struct rtt_prepared_a rtt_prepared_a;   /* DmmaPreparedA */
+
+typedef struct rtt_offline_a_stats {            /* DmmaOfflineAStats */
+    double h2d_ms;
+    double validation_ms;
+    double canonicalize_ms;   /* per-row column sort when rows arrive unsorted */
+    double reorder_ms;
+    double tile_build_ms;
+    double total_ms;
+    int rows_were_sorted;     /* 1 if the input rows were already column-sorted */
+} rtt_offline_a_stats;
+
+/* gpu_prepare_identity_a (src/gpu_dmma_tiles.h:2443-2502): upload and
+ * validate the CSR, keep the identity layout with only the empty row / inner
+ * suffix trimmed, and build A's 8x4 tiles. */
+int rtt_prepare_identity_a(const rtt_smatrix *host, int dense_threshold,
+                           rtt_prepared_a **out, rtt_offline_a_stats *stats);
+
+/* gpu_prepare_external_a (src/gpu_dmma_tiles.h:2389-2441) with the order
+ * files already parsed: row_new_to_old has host->m entries and
+ * inner_new_to_old host->n entries, both bijections (new_id -> old_id, the
+ * token semantics of src/dmma_reorder.h:2048-2184). */
+int rtt_prepare_external_a(const rtt_smatrix *host,
+                           const int *row_new_to_old,
+                           const int *inner_new_to_old, int dense_threshold,
+                           rtt_prepared_a **out, rtt_offline_a_stats *stats);
+
+/* Parses one order file (exactly `expected` 0-based decimal tokens, new_id
+ * -> old_id, no duplicates or out-of-range ids: src/dmma_reorder.h:2048-2120)
+ * into out[expected]. */
+int rtt_read_order_file(const char *path, int expected, int *out);
+
+/* Views and metadata of a prepared A. */
+int rtt_prepared_a_view(const rtt_prepared_a *a, rtt_device_tiles *view);
+int rtt_prepared_a_info(const rtt_prepared_a *a, int *active_rows,
+                        int *active_inner, int *identity);
+/* Host copy of the row permutation (new -> old) for tile2csr_restore_rows;
+ * NULL for the identity layout. */
+const int *rtt_prepared_a_row_new_to_old(const rtt_prepared_a *a);
+void rtt_destroy_prepared_a(rtt_prepared_a *a);
+
+/* ------------------------------------------------------------------------ */
+/* Per-iteration B rebuild (timed in the reference: src/main.cu:2188-2191)   */
+/* ------------------------------------------------------------------------ */
+typedef struct rtt_dynamic_b rtt_dynamic_b;     /* DmmaDynamicB */
+
+typedef struct rtt_b_update_stats {             /* DmmaBUpdateStats */
+    double total_ms;
+    double csr_row_permute_ms;
+    double csr2tile_ms;
+    double csc_ms;
+    int path;            /* 0 ordered-CSR merge, 1 tile transpose, 2 CSR transpose */
+} rtt_b_update_stats;
+
+/* gpu_rebuild_dynamic_b_via_permuted_csr (aat = 0, src/gpu_dmma_tiles.h:2275)
+ * and gpu_rebuild_dynamic_b_transpose (aat = 1, :2285): B = A (rows mapped by
+ * the inner permutation) or B = A^T, as 4x8 column-major tiles with tile
+ * CSC.  *inout may hold a previous B, which is rebuilt in place. */
+int rtt_rebuild_b(const rtt_prepared_a *a, int aat, rtt_dynamic_b **inout,
+                  rtt_b_update_stats *stats);
+int rtt_dynamic_b_view(const rtt_dynamic_b *b, rtt_device_tiles *view);
+void rtt_destroy_dynamic_b(rtt_dynamic_b *b);
+
+/* gpu_compute_nnz_cub_derived (src/gpu_dmma_tiles.h:2304-2320): number of
+ * scalar products, AA: sum_e rowlen(col e); AAT: sum_e coldeg(col e). */
+int rtt_compute_nnz_cub(const rtt_prepared_a *a, int aat,
+                        unsigned long long *nnz_cub, double *elapsed_ms);
+
+/* ------------------------------------------------------------------------ */
+/* SpGEMM core: symbolic -> allocate -> numeric                              */
+/* ------------------------------------------------------------------------ */
+enum rtt_lb_mode {               /* --lb-mode (src/main.cu:132-166) */
+    RTT_LB_STATIC = 0,
+    RTT_LB_DYNAMIC = 1,
+    RTT_LB_ADAPTIVE_TAIL = 2
+};
+
+typedef struct rtt_schedule {    /* DmmaNumericScheduleConfig subset */
+    int lb_mode;                 /* enum rtt_lb_mode; default adaptive-tail */
+    int materialize_output;      /* export C tiles to the host SMatrix */
+    double split_threshold;      /* heavy admission cost (16384) */
+    double split_chunk_target;   /* cost per chunk (8192) */
+    int split_max_chunks;        /* 8 */
+    double max_heavy_fraction;   /* 0.01 */
+    int tail_record_capacity;    /* 1 << 20 */
+    int force_tail_split;        /* admit every multi-step task (tests) */
+} rtt_schedule;
+
+/* Fills *s with the reference defaults (src/main.cu:61-130). */
+void rtt_schedule_defaults(rtt_schedule *s);
+
+typedef struct rtt_stats {       /* DmmaSpGemmStats subset */
+    double symbolic_ms;          /* candidate + exact structure */
+    double allocation_ms;        /* size readback + C allocation */
+    double numeric_ms;           /* scheduler + numeric */
+    double total_ms;
+    double output_copy_ms;       /* D2H export (outside the Core) */
+    /* device-event durations of the main kernels, last call */
+    double symbolic_count_kernel_ms;
+    double symbolic_fill_kernel_ms;
+    double numeric_kernel_ms;
+    unsigned long long pair_count;     /* (A tile, B tile) products visited */
+    unsigned long long output_tiles;   /* 8x8 C tiles */
+    unsigned long long output_nnz;     /* nnz(C) */
+    unsigned long long heavy_rows;     /* tile rows on the split (tail) path */
+    unsigned long long heavy_chunks;   /* chunks those rows were split into */
+    int output_materialized;
+} rtt_stats;
+
+/* dmma_tilespgemm (src/dmma_spgemm.h:6490): C = A * B with A 8x4 and B 4x8
+ * tiles; C is produced as 8x8 tiles.  When sched->materialize_output is set,
+ * *output receives the host tile-CSR (m, n, tilem, tilen, numtile, nnz,
+ * tile_ptr, tile_columnidx, tile_nnz, tile_csr_Ptr, tile_csr_Col,
+ * tile_csr_Value), in canonical order, exactly as dmma_copy_output_to_host
+ * (src/dmma_spgemm.h:6399-6482) lays it out.  Returns RTT_ERR_OVERFLOW when
+ * the tile count or nnz(C) exceeds INT_MAX and export was requested. */
+int rtt_tilespgemm(const rtt_prepared_a *a, const rtt_dynamic_b *b,
+                   rtt_smatrix *output, rtt_stats *stats,
+                   const rtt_schedule *sched);
+
+/* Device view of the last C produced on handle `a` (valid until the next
+ * rtt_tilespgemm on `a`): 64-bit offsets, 64-bit tile masks. */
+typedef struct rtt_device_c {
+    int64_t tilem;
+    int64_t tilen;
+    int64_t num_tiles;
+    int64_t nnz;
+    const int64_t *tile_ptr;     /* tilem + 1 */
+    const int *tile_col;         /* num_tiles */
+    const uint64_t *tile_mask;   /* num_tiles */
+    const int64_t *tile_nnz;     /* num_tiles + 1 */
+    const uint8_t *tile_csr_ptr; /* 8 * num_tiles */
+    const uint8_t *tile_csr_col; /* nnz */
+    const double *tile_csr_val;  /* nnz */
+} rtt_device_c;
+int rtt_last_c_view(const rtt_prepared_a *a, rtt_device_c *view);
+
+/* Copies device memory to host (test and tool helper). */
+int rtt_memcpy_d2h(void *dst, const void *src, size_t bytes);
+
+/* ------------------------------------------------------------------------ */
+/* Output conversion (host, OpenMP)                                          */
+/* ------------------------------------------------------------------------ */
+/* tile2csr (src/tile2csr.h:232-329): fills rowpointer/columnindex/value and
+ * nnz of *m from its 8x8 tile-CSR fields. canonical != 0 skips the row sort. */
+int rtt_tile2csr(rtt_smatrix *m, int canonical);
+/* tile2csr_restore_rows (src/tile2csr.h:337-523): the same with rows mapped
+ * from the reordered space through row_new_to_old[0..permutation_size). */
+int rtt_tile2csr_restore_rows(rtt_smatrix *m, const int *row_new_to_old,
+                              int permutation_size, int canonical);
+
+/* ------------------------------------------------------------------------ */
+/* Seeded synthetic inputs (splitmix64; identical on every host)             */
+/* ------------------------------------------------------------------------ */
+/* 2D 5-point Laplacian on g x g (row = x + g*y): diagonal 4.0,
+ * off-diagonals -1.0 + U[-pert, pert]. */
+int rtt_gen_laplace2d(int g, double pert, uint64_t seed, rtt_smatrix *out);
+/* 3D 27-point stencil on g^3 (row = x + g*y + g*g*z), values U[lo, hi]. */
+int rtt_gen_stencil27(int g, double lo, double hi, uint64_t seed,
+                      rtt_smatrix *out);
+/* R-MAT with 2^scale rows, edge_factor * 2^scale draws, probabilities
+ * (a, b, c, 1-a-b-c), duplicates removed, values U[-1, 1], rows sorted. */
+int rtt_gen_rmat(int scale, int edge_factor, double a, double b, double c,
+                 uint64_t seed, rtt_smatrix *out);
+/* Random m x n matrix with `per_row` distinct sorted columns per row drawn
+ * uniformly, values U[-1, 1] (test helper). */
+int rtt_gen_random(int m, int n, int per_row, uint64_t seed, rtt_smatrix *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTT_SPGEMM_H_ */

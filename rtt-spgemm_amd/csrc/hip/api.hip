@@ -1,0 +1,503 @@
+// extern "C" entry points of the device path: A preparation, B rebuild,
+// nnzCub, the SpGEMM core and the host export of C.
+#include <algorithm>
+#include <climits>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "internal.hpp"
+
+struct rtt_prepared_a {
+    rtt::DevCsr csr;          // input CSR, rows column-sorted (B source)
+    rtt::DevCsr csr_tiled;    // reordered copy when a permutation is given
+    rtt::DevCsr csr_t;        // transposed input (reordered AAT rebuilds)
+    rtt::DevTiles tiles;      // A: 8x4 row-major hybrid tiles
+    int rows = 0, cols = 0;
+    int active_rows = 0, active_inner = 0;
+    int dense_threshold = 24;
+    bool identity = true;     // both permutations identity
+    bool row_identity = true; // row permutation identity
+    std::vector<int> row_new_to_old, inner_new_to_old;
+    rtt::DevBuf d_inner_new_to_old;
+    rtt::BuildWork build;
+    rtt::SpgemmWork core;
+    rtt::Events events;
+    rtt::DevC last_c;
+    bool has_c = false;
+};
+
+struct rtt_dynamic_b {
+    rtt::DevTiles tiles;
+    rtt::BuildWork build;
+    int aat = 0;
+};
+
+namespace {
+
+using rtt::fail;
+
+int export_c(const rtt::DevC &c, rtt_smatrix *out)
+{
+    if (c.num_tiles > INT_MAX || c.nnz > INT_MAX || c.tilem > INT_MAX)
+        return fail(RTT_ERR_OVERFLOW,
+                    "C has %lld tiles / %lld nonzeros: beyond the int32 SMatrix ABI "
+                    "(use rtt_last_c_view for the 64-bit device result)",
+                    (long long)c.num_tiles, (long long)c.nnz);
+    hipStream_t s = rtt::ctx().stream;
+    const size_t T = static_cast<size_t>(c.num_tiles);
+    const size_t Z = static_cast<size_t>(c.nnz);
+    const size_t R = static_cast<size_t>(c.tilem);
+    rtt_smatrix r;
+    std::memset(&r, 0, sizeof(r));
+    r.m = c.rows;
+    r.n = c.cols;
+    r.nnz = static_cast<int>(Z);
+    r.tilem = static_cast<int>(c.tilem);
+    r.tilen = static_cast<int>(c.tilen);
+    r.numtile = static_cast<int>(T);
+    r.tile_ptr = static_cast<int *>(std::malloc((R + 1) * sizeof(int)));
+    r.tile_columnidx = static_cast<int *>(std::malloc(std::max<size_t>(T, 1) * sizeof(int)));
+    r.tile_nnz = static_cast<int *>(std::malloc((T + 1) * sizeof(int)));
+    r.tile_csr_Ptr = static_cast<unsigned char *>(std::malloc(std::max<size_t>(T * 8, 1)));
+    r.tile_csr_Col = static_cast<unsigned char *>(std::malloc(std::max<size_t>(Z, 1)));
+    r.tile_csr_Value = static_cast<double *>(std::malloc(std::max<size_t>(Z, 1) * sizeof(double)));
+    std::vector<int64_t> tp(R + 1), tn(T + 1);
+    if (!r.tile_ptr || !r.tile_columnidx || !r.tile_nnz || !r.tile_csr_Ptr || !r.tile_csr_Col ||
+        !r.tile_csr_Value) {
+        rtt_destroy_smatrix(&r);
+        return fail(RTT_ERR_OOM, "host allocation of the C export failed");
+    }
+    auto copy = [&](void *dst, const void *src, size_t bytes) -> int {
+        if (bytes == 0)
+            return RTT_OK;
+        RTT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        return RTT_OK;
+    };
+    int rc = RTT_OK;
+    if ((rc = copy(tp.data(), c.tile_ptr.ptr, (R + 1) * sizeof(int64_t))) != RTT_OK ||
+        (rc = copy(tn.data(), c.tile_nnz.ptr, (T + 1) * sizeof(int64_t))) != RTT_OK ||
+        (rc = copy(r.tile_columnidx, c.tile_col.ptr, T * sizeof(int))) != RTT_OK ||
+        (rc = copy(r.tile_csr_Ptr, c.tile_csr_ptr.ptr, T * 8)) != RTT_OK ||
+        (rc = copy(r.tile_csr_Col, c.tile_csr_col.ptr, Z)) != RTT_OK ||
+        (rc = copy(r.tile_csr_Value, c.tile_csr_val.ptr, Z * sizeof(double))) != RTT_OK) {
+        rtt_destroy_smatrix(&r);
+        return rc;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        rtt_destroy_smatrix(&r);
+        return fail(RTT_ERR_DEVICE, "C export synchronisation failed");
+    }
+    for (size_t i = 0; i <= R; ++i)
+        r.tile_ptr[i] = static_cast<int>(tp[i]);
+    for (size_t i = 0; i <= T; ++i)
+        r.tile_nnz[i] = static_cast<int>(tn[i]);
+    rtt_destroy_smatrix(out);
+    *out = r;
+    return RTT_OK;
+}
+
+bool is_bijection(const int *p, int n)
+{
+    std::vector<char> seen(static_cast<size_t>(n), 0);
+    for (int i = 0; i < n; ++i) {
+        if (p[i] < 0 || p[i] >= n || seen[p[i]])
+            return false;
+        seen[p[i]] = 1;
+    }
+    return true;
+}
+
+bool is_identity(const int *p, int n)
+{
+    for (int i = 0; i < n; ++i)
+        if (p[i] != i)
+            return false;
+    return true;
+}
+
+} // namespace
+
+extern "C" {
+
+int rtt_init(int device)
+{
+    rtt::clear_error();
+    rtt::Context &c = rtt::ctx();
+    if (c.stream && c.device == device)
+        return RTT_OK;
+    if (c.stream)
+        return fail(RTT_ERR_INVALID, "rtt_init: already bound to device %d", c.device);
+    int count = 0;
+    RTT_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return fail(RTT_ERR_INVALID, "rtt_init: device %d not present (%d devices)", device,
+                    count);
+    RTT_HIP(hipSetDevice(device));
+    RTT_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    RTT_HIP(hipDeviceGetAttribute(&c.num_cus, hipDeviceAttributeMultiprocessorCount, device));
+    c.device = device;
+    return RTT_OK;
+}
+
+int rtt_sync(void)
+{
+    RTT_TRY(rtt::require_ctx());
+    RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
+    return RTT_OK;
+}
+
+int rtt_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+    RTT_TRY(rtt::require_ctx());
+    if (bytes == 0)
+        return RTT_OK;
+    RTT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, rtt::ctx().stream));
+    RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
+    return RTT_OK;
+}
+
+int rtt_download_tiles(const rtt_device_tiles *v, int *tile_row_ptr, int *tile_col_idx,
+                       int *value_offsets, uint32_t *masks, double *values, int *tile_col_ptr,
+                       int *tile_row_idx, int *csc_tile_ids)
+{
+    rtt::clear_error();
+    if (!v)
+        return fail(RTT_ERR_INVALID, "rtt_download_tiles: NULL view");
+    const size_t T = static_cast<size_t>(v->num_tiles);
+    struct Item {
+        void *dst;
+        const void *src;
+        size_t bytes;
+    } items[] = {
+        {tile_row_ptr, v->tile_row_ptr, (static_cast<size_t>(v->tile_row_count) + 1) * sizeof(int)},
+        {tile_col_idx, v->tile_col_idx, T * sizeof(int)},
+        {value_offsets, v->value_offsets, (T + 1) * sizeof(int)},
+        {masks, v->masks, T * sizeof(uint32_t)},
+        {values, v->values, static_cast<size_t>(v->payload_size) * sizeof(double)},
+        {tile_col_ptr, v->tile_col_ptr, (static_cast<size_t>(v->tile_col_count) + 1) * sizeof(int)},
+        {tile_row_idx, v->tile_row_idx, T * sizeof(int)},
+        {csc_tile_ids, v->csc_tile_ids, T * sizeof(int)},
+    };
+    for (const Item &it : items) {
+        if (!it.dst)
+            continue;
+        if (!it.src && it.bytes)
+            return fail(RTT_ERR_INVALID, "rtt_download_tiles: requested array is absent");
+        RTT_TRY(rtt_memcpy_d2h(it.dst, it.src, it.bytes));
+    }
+    return RTT_OK;
+}
+
+int rtt_prepare_external_a(const rtt_smatrix *host, const int *row_new_to_old,
+                           const int *inner_new_to_old, int dense_threshold,
+                           rtt_prepared_a **out, rtt_offline_a_stats *stats)
+{
+    rtt::clear_error();
+    RTT_TRY(rtt::require_ctx());
+    if (!host || !out || dense_threshold < 1 || dense_threshold > rtt::kInputElems)
+        return fail(RTT_ERR_INVALID, "rtt_prepare_*_a: bad argument");
+    if (host->m < 0 || host->n < 0 || host->nnz < 0 || !host->rowpointer ||
+        host->rowpointer[0] != 0 || host->rowpointer[host->m] != host->nnz)
+        return fail(RTT_ERR_INVALID, "rtt_prepare_*_a: malformed CSR");
+    const bool external = row_new_to_old || inner_new_to_old;
+    if (external && (!row_new_to_old || !inner_new_to_old))
+        return fail(RTT_ERR_INVALID, "external orders need both axes");
+    if (external && (!is_bijection(row_new_to_old, host->m) ||
+                     !is_bijection(inner_new_to_old, host->n)))
+        return fail(RTT_ERR_INVALID, "external order is not a bijection");
+    rtt_offline_a_stats st;
+    std::memset(&st, 0, sizeof(st));
+    auto t_total = rtt::Clock::now();
+    std::unique_ptr<rtt_prepared_a> a(new (std::nothrow) rtt_prepared_a());
+    if (!a)
+        return fail(RTT_ERR_OOM, "host allocation failed");
+    a->rows = host->m;
+    a->cols = host->n;
+    a->dense_threshold = dense_threshold;
+    std::vector<int> row_old_to_new, inner_old_to_new;
+    if (external) {
+        a->row_new_to_old.assign(row_new_to_old, row_new_to_old + host->m);
+        a->inner_new_to_old.assign(inner_new_to_old, inner_new_to_old + host->n);
+        row_old_to_new.resize(host->m);
+        inner_old_to_new.resize(host->n);
+        for (int i = 0; i < host->m; ++i)
+            row_old_to_new[row_new_to_old[i]] = i;
+        for (int i = 0; i < host->n; ++i)
+            inner_old_to_new[inner_new_to_old[i]] = i;
+        a->row_identity = is_identity(row_new_to_old, host->m);
+        a->identity = a->row_identity && is_identity(inner_new_to_old, host->n);
+    }
+    // Active prefixes: the last non-empty (new) row and the largest used
+    // (new) inner index, as gpu_prepare_identity_a / the external plan do
+    // (src/gpu_dmma_tiles.h:2458-2469, src/dmma_reorder.h:2257-2284).
+    for (int r = 0; r < host->m; ++r) {
+        const int b = host->rowpointer[r], e = host->rowpointer[r + 1];
+        if (b < 0 || e < b || e > host->nnz)
+            return fail(RTT_ERR_INVALID, "row %d has invalid bounds", r);
+        if (b != e)
+            a->active_rows = std::max(a->active_rows, (external ? row_old_to_new[r] : r) + 1);
+        for (int x = b; x < e; ++x) {
+            const int c = host->columnindex[x];
+            if (c < 0 || c >= host->n)
+                return fail(RTT_ERR_INVALID, "column %d out of range", c);
+            a->active_inner = std::max(a->active_inner, (external ? inner_old_to_new[c] : c) + 1);
+        }
+    }
+    double sort_ms = 0.0;
+    RTT_TRY(rtt::upload_csr(*host, &a->csr, &a->build, &st.h2d_ms, &st.validation_ms, &sort_ms,
+                            &st.rows_were_sorted));
+    st.canonicalize_ms = sort_ms;
+    const rtt::DevCsr *tiled = &a->csr;
+    if (external && !a->identity) {
+        auto t_re = rtt::Clock::now();
+        // Materialise the reordered CSR once (offline): rows gathered by
+        // row_new_to_old, columns mapped by inner_old_to_new.
+        rtt_smatrix p;
+        std::memset(&p, 0, sizeof(p));
+        p.m = host->m;
+        p.n = host->n;
+        p.nnz = host->nnz;
+        std::vector<int> rp(host->m + 1), ci(host->nnz);
+        std::vector<double> vv(host->nnz);
+        int e = 0;
+        for (int nr = 0; nr < host->m; ++nr) {
+            rp[nr] = e;
+            const int orow = row_new_to_old[nr];
+            for (int x = host->rowpointer[orow]; x < host->rowpointer[orow + 1]; ++x) {
+                ci[e] = inner_old_to_new[host->columnindex[x]];
+                vv[e] = host->value[x];
+                ++e;
+            }
+        }
+        rp[host->m] = e;
+        p.rowpointer = rp.data();
+        p.columnindex = ci.data();
+        p.value = vv.data();
+        double h2d = 0, val = 0, srt = 0;
+        int sorted = 0;
+        RTT_TRY(rtt::upload_csr(p, &a->csr_tiled, &a->build, &h2d, &val, &srt, &sorted));
+        st.reorder_ms = rtt::ms_since(t_re);
+        tiled = &a->csr_tiled;
+        RTT_TRY(a->d_inner_new_to_old.ensure(static_cast<size_t>(host->n) * sizeof(int)));
+        if (host->n > 0)
+            RTT_HIP(hipMemcpy(a->d_inner_new_to_old.ptr, inner_new_to_old,
+                              static_cast<size_t>(host->n) * sizeof(int),
+                              hipMemcpyHostToDevice));
+    }
+    auto t_tiles = rtt::Clock::now();
+    RTT_TRY(rtt::build_tiles(*tiled, nullptr, a->active_rows, a->active_inner, host->m, host->n,
+                             false, dense_threshold, false, &a->tiles, &a->build));
+    RTT_TRY(rtt::fill_tile_row_of(&a->tiles));
+    RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
+    st.tile_build_ms = rtt::ms_since(t_tiles);
+    st.total_ms = rtt::ms_since(t_total);
+    if (stats)
+        *stats = st;
+    *out = a.release();
+    return RTT_OK;
+}
+
+int rtt_prepare_identity_a(const rtt_smatrix *host, int dense_threshold, rtt_prepared_a **out,
+                           rtt_offline_a_stats *stats)
+{
+    return rtt_prepare_external_a(host, nullptr, nullptr, dense_threshold, out, stats);
+}
+
+int rtt_prepared_a_view(const rtt_prepared_a *a, rtt_device_tiles *view)
+{
+    if (!a || !view)
+        return fail(RTT_ERR_INVALID, "rtt_prepared_a_view: NULL argument");
+    a->tiles.view(view);
+    return RTT_OK;
+}
+
+int rtt_prepared_a_info(const rtt_prepared_a *a, int *active_rows, int *active_inner,
+                        int *identity)
+{
+    if (!a)
+        return fail(RTT_ERR_INVALID, "rtt_prepared_a_info: NULL handle");
+    if (active_rows)
+        *active_rows = a->active_rows;
+    if (active_inner)
+        *active_inner = a->active_inner;
+    if (identity)
+        *identity = a->identity ? 1 : 0;
+    return RTT_OK;
+}
+
+const int *rtt_prepared_a_row_new_to_old(const rtt_prepared_a *a)
+{
+    if (!a || a->identity || a->row_new_to_old.empty())
+        return nullptr;
+    return a->row_new_to_old.data();
+}
+
+void rtt_destroy_prepared_a(rtt_prepared_a *a)
+{
+    if (a && rtt::ctx().stream)
+        (void)hipStreamSynchronize(rtt::ctx().stream);
+    delete a;
+}
+
+int rtt_rebuild_b(const rtt_prepared_a *a_const, int aat, rtt_dynamic_b **inout,
+                  rtt_b_update_stats *stats)
+{
+    rtt::clear_error();
+    RTT_TRY(rtt::require_ctx());
+    if (!a_const || !inout || (aat != 0 && aat != 1))
+        return fail(RTT_ERR_INVALID, "rtt_rebuild_b: bad argument");
+    rtt_prepared_a *a = const_cast<rtt_prepared_a *>(a_const);
+    if (!aat && a->rows != a->cols)
+        return fail(RTT_ERR_INVALID, "C = A*A needs a square A (%d x %d)", a->rows, a->cols);
+    if (!*inout) {
+        *inout = new (std::nothrow) rtt_dynamic_b();
+        if (!*inout)
+            return fail(RTT_ERR_OOM, "host allocation failed");
+    }
+    rtt_dynamic_b *b = *inout;
+    b->aat = aat;
+    rtt_b_update_stats st;
+    std::memset(&st, 0, sizeof(st));
+    auto t0 = rtt::Clock::now();
+    const int *inner_map = a->identity ? nullptr : a->d_inner_new_to_old.as<int>();
+    if (!aat) {
+        // B = A with rows gathered through the inner permutation, columns kept.
+        st.path = 0;
+        RTT_TRY(rtt::build_tiles(a->csr, inner_map, a->active_inner, a->cols, a->cols, a->cols,
+                                 true, a->dense_threshold, true, &b->tiles, &b->build));
+    } else if (a->row_identity) {
+        // B = A^T straight from A's tiles.
+        st.path = 1;
+        RTT_TRY(rtt::transpose_tiles(a->tiles, a->rows, &b->tiles, &b->build));
+    } else {
+        st.path = 2;
+        auto tp = rtt::Clock::now();
+        RTT_TRY(rtt::transpose_csr(a->csr, &a->csr_t, &b->build));
+        RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
+        st.csr_row_permute_ms = rtt::ms_since(tp);
+        RTT_TRY(rtt::build_tiles(a->csr_t, inner_map, a->active_inner, a->rows, a->cols, a->rows,
+                                 true, a->dense_threshold, true, &b->tiles, &b->build));
+    }
+    RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
+    st.total_ms = rtt::ms_since(t0);
+    st.csr2tile_ms = st.total_ms - st.csr_row_permute_ms;
+    if (stats)
+        *stats = st;
+    return RTT_OK;
+}
+
+int rtt_dynamic_b_view(const rtt_dynamic_b *b, rtt_device_tiles *view)
+{
+    if (!b || !view)
+        return fail(RTT_ERR_INVALID, "rtt_dynamic_b_view: NULL argument");
+    b->tiles.view(view);
+    return RTT_OK;
+}
+
+void rtt_destroy_dynamic_b(rtt_dynamic_b *b)
+{
+    if (b && rtt::ctx().stream)
+        (void)hipStreamSynchronize(rtt::ctx().stream);
+    delete b;
+}
+
+int rtt_compute_nnz_cub(const rtt_prepared_a *a_const, int aat, unsigned long long *nnz_cub,
+                        double *elapsed_ms)
+{
+    rtt::clear_error();
+    RTT_TRY(rtt::require_ctx());
+    if (!a_const || !nnz_cub)
+        return fail(RTT_ERR_INVALID, "rtt_compute_nnz_cub: bad argument");
+    rtt_prepared_a *a = const_cast<rtt_prepared_a *>(a_const);
+    auto t0 = rtt::Clock::now();
+    RTT_TRY(rtt::nnz_cub(a->csr, aat != 0, nnz_cub, &a->build));
+    if (elapsed_ms)
+        *elapsed_ms = rtt::ms_since(t0);
+    return RTT_OK;
+}
+
+void rtt_schedule_defaults(rtt_schedule *s)
+{
+    if (!s)
+        return;
+    s->lb_mode = RTT_LB_ADAPTIVE_TAIL;
+    s->materialize_output = 1;
+    s->split_threshold = 16384.0;
+    s->split_chunk_target = 8192.0;
+    s->split_max_chunks = 8;
+    s->max_heavy_fraction = 0.01;
+    s->tail_record_capacity = 1 << 20;
+    s->force_tail_split = 0;
+}
+
+int rtt_tilespgemm(const rtt_prepared_a *a_const, const rtt_dynamic_b *b, rtt_smatrix *output,
+                   rtt_stats *stats, const rtt_schedule *sched)
+{
+    rtt::clear_error();
+    RTT_TRY(rtt::require_ctx());
+    if (!a_const || !b || !stats)
+        return fail(RTT_ERR_INVALID, "rtt_tilespgemm: NULL argument");
+    rtt_prepared_a *a = const_cast<rtt_prepared_a *>(a_const);
+    rtt_schedule defaults;
+    rtt_schedule_defaults(&defaults);
+    const rtt_schedule &sc = sched ? *sched : defaults;
+    if (sc.lb_mode < RTT_LB_STATIC || sc.lb_mode > RTT_LB_ADAPTIVE_TAIL)
+        return fail(RTT_ERR_INVALID, "unknown lb_mode %d", sc.lb_mode);
+    // The same shape contract as dmma_tilespgemm (src/dmma_spgemm.h:6495-6510).
+    if (a->tiles.tile_col_count != b->tiles.tile_row_count || a->tiles.tile_rows != rtt::kTileM ||
+        a->tiles.tile_cols != rtt::kTileK || b->tiles.tile_rows != rtt::kTileK ||
+        b->tiles.tile_cols != rtt::kTileN)
+        return fail(RTT_ERR_INVALID, "A and B tile shapes do not conform");
+    if (sc.materialize_output && !output)
+        return fail(RTT_ERR_INVALID, "materialize_output needs an output matrix");
+    std::memset(stats, 0, sizeof(*stats));
+    auto t0 = rtt::Clock::now();
+    rtt::CoreTimes times;
+    a->has_c = false;
+    RTT_TRY(rtt::spgemm_core(a->tiles, b->tiles, &a->last_c, &a->core, &a->events, &times));
+    a->has_c = true;
+    stats->symbolic_ms = times.symbolic_ms;
+    stats->allocation_ms = times.allocation_ms;
+    stats->numeric_ms = times.numeric_ms;
+    stats->total_ms = rtt::ms_since(t0);
+    stats->symbolic_count_kernel_ms = times.sym_count_kernel_ms;
+    stats->symbolic_fill_kernel_ms = times.sym_fill_kernel_ms;
+    stats->numeric_kernel_ms = times.numeric_kernel_ms;
+    stats->pair_count = times.pairs;
+    stats->output_tiles = static_cast<unsigned long long>(a->last_c.num_tiles);
+    stats->output_nnz = static_cast<unsigned long long>(a->last_c.nnz);
+    stats->heavy_rows = times.heavy_rows;
+    if (sc.materialize_output) {
+        auto te = rtt::Clock::now();
+        RTT_TRY(export_c(a->last_c, output));
+        stats->output_copy_ms = rtt::ms_since(te);
+        stats->output_materialized = 1;
+    }
+    return RTT_OK;
+}
+
+int rtt_last_c_view(const rtt_prepared_a *a, rtt_device_c *v)
+{
+    if (!a || !v)
+        return fail(RTT_ERR_INVALID, "rtt_last_c_view: NULL argument");
+    if (!a->has_c)
+        return fail(RTT_ERR_INVALID, "rtt_last_c_view: no C has been produced");
+    const rtt::DevC &c = a->last_c;
+    v->tilem = c.tilem;
+    v->tilen = c.tilen;
+    v->num_tiles = c.num_tiles;
+    v->nnz = c.nnz;
+    v->tile_ptr = c.tile_ptr.as<int64_t>();
+    v->tile_col = c.tile_col.as<int>();
+    v->tile_mask = c.tile_mask.as<uint64_t>();
+    v->tile_nnz = c.tile_nnz.as<int64_t>();
+    v->tile_csr_ptr = c.tile_csr_ptr.as<uint8_t>();
+    v->tile_csr_col = c.tile_csr_col.as<uint8_t>();
+    v->tile_csr_val = c.tile_csr_val.as<double>();
+    return RTT_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,742 @@
+// CSR upload/validation and the CSR -> hybrid tile builders.
+//
+// Tiles are formed without a global sort: rows are kept column-sorted (they
+// are sorted once at upload when they arrive unsorted), so the tiles of one
+// tile row are the ordered merge of its TR row streams.  A group of TR lanes
+// owns one tile row; each lane walks one scalar row.  The group repeatedly
+// takes the minimum tile column among its lanes' heads, ORs the lanes' bits
+// into the tile mask and writes the tile.  This replaces the reference's
+// 64-bit key radix sort + reduce_by_key (src/gpu_dmma_tiles.h:1117-1395) and
+// its 4-lane ordered-warp B path (src/gpu_dmma_tiles.h:313-560).
+#include <hipcub/hipcub.hpp>
+
+#include <climits>
+
+#include "device.hpp"
+#include "internal.hpp"
+
+namespace rtt {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__global__ void set_i64_kernel(int64_t *p, int64_t n, int64_t v)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+__global__ void iota_kernel(int *p, int64_t n)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = static_cast<int>(i);
+}
+
+// flags: 1 row_ptr not monotone / wrong ends, 2 column out of range,
+//        4 some row not column-sorted (duplicates allowed).
+__global__ void validate_csr_kernel(int rows, int cols, int64_t nnz,
+                                    const int *__restrict__ rp,
+                                    const int *__restrict__ ci, int *flags)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i == 0 && (rp[0] != 0 || rp[rows] != nnz))
+        atomicOr(flags, 1);
+    if (i < rows) {
+        const int b = rp[i], e = rp[i + 1];
+        if (e < b) {
+            atomicOr(flags, 1);
+        } else {
+            for (int x = b + 1; x < e; ++x)
+                if (ci[x] < ci[x - 1]) {
+                    atomicOr(flags, 4);
+                    break;
+                }
+        }
+    }
+    if (i < nnz) {
+        const int c = ci[i];
+        if (c < 0 || c >= cols)
+            atomicOr(flags, 2);
+    }
+}
+
+// One group of TR lanes per tile row.  COUNT pass: per tile row the number of
+// tiles, payload slots and dense tiles.  FILL pass: tile columns, masks,
+// value offsets and payload.  Duplicates (adjacent equal columns in a sorted
+// row) are summed in row order.
+template <int TR, int TC, bool COL_MAJOR, bool FILL>
+__global__ __launch_bounds__(kThreads) void tile_build_kernel(
+    int tile_row_count, int active_rows, int active_cols,
+    const int *__restrict__ rp, const int *__restrict__ ci,
+    const double *__restrict__ val, const int *__restrict__ row_map,
+    int dense_threshold, int64_t *__restrict__ row_tiles,
+    int64_t *__restrict__ row_payload, int64_t *__restrict__ row_dense,
+    const int64_t *__restrict__ tile_off, const int64_t *__restrict__ payload_off,
+    int *__restrict__ tile_col_idx, uint32_t *__restrict__ masks,
+    int *__restrict__ value_offsets, double *__restrict__ values)
+{
+    static_assert(TR * TC == 32, "tiles hold 32 elements");
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t tr = gid / TR;
+    const int r = static_cast<int>(gid % TR);
+    if (tr >= tile_row_count)
+        return;   // whole groups leave together
+    const int64_t row = tr * TR + r;
+    int p = 0, end = 0;
+    if (row < active_rows) {
+        const int src = row_map ? row_map[row] : static_cast<int>(row);
+        p = rp[src];
+        end = rp[src + 1];
+    }
+    // Columns at or beyond active_cols are outside the tiled extent.
+    while (end > p && ci[end - 1] >= active_cols)
+        --end;
+
+    int64_t ntiles = 0, payload = 0, ndense = 0;
+    int64_t t_out = FILL ? tile_off[tr] : 0;
+    int64_t p_out = FILL ? payload_off[tr] : 0;
+    for (;;) {
+        int head = p < end ? ci[p] / TC : INT_MAX;
+#pragma unroll
+        for (int o = TR / 2; o > 0; o >>= 1)
+            head = min(head, __shfl_xor(head, o, TR));
+        if (head == INT_MAX)
+            break;
+        uint32_t bits = 0;
+        int q = p;
+        while (q < end) {
+            const int c = ci[q];
+            if (c / TC != head)
+                break;
+            const int lc = c % TC;
+            bits |= 1u << (COL_MAJOR ? lc * TR + r : r * TC + lc);
+            ++q;
+        }
+        uint32_t mask = bits;
+#pragma unroll
+        for (int o = TR / 2; o > 0; o >>= 1)
+            mask |= __shfl_xor(mask, o, TR);
+        const int cnt = __popc(mask);
+        const bool dense = cnt >= dense_threshold;
+        if constexpr (FILL) {
+            if (r == 0) {
+                tile_col_idx[t_out] = head;
+                masks[t_out] = mask;
+                value_offsets[t_out] = static_cast<int>(p_out);
+            }
+            if (dense) {
+                // Absent slots of this lane's row are explicit zeros.
+#pragma unroll
+                for (int lc = 0; lc < TC; ++lc) {
+                    const int phys = COL_MAJOR ? lc * TR + r : r * TC + lc;
+                    if (!(bits & (1u << phys)))
+                        values[p_out + phys] = 0.0;
+                }
+            }
+            int x = p;
+            while (x < q) {
+                const int c = ci[x];
+                double s = val[x];
+                ++x;
+                while (x < q && ci[x] == c) {
+                    s += val[x];
+                    ++x;
+                }
+                const int lc = c % TC;
+                const int phys = COL_MAJOR ? lc * TR + r : r * TC + lc;
+                values[p_out + payload_slot(mask, dense ? 32 : cnt, phys)] = s;
+            }
+            ++t_out;
+            p_out += dense ? 32 : cnt;
+        }
+        ++ntiles;
+        payload += dense ? 32 : cnt;
+        ndense += dense ? 1 : 0;
+        p = q;
+    }
+    if constexpr (!FILL) {
+        if (r == 0) {
+            row_tiles[tr] = ntiles;
+            row_payload[tr] = payload;
+            row_dense[tr] = ndense;
+        }
+    }
+}
+
+__global__ void narrow_i64_kernel(const int64_t *in, int *out, int64_t n)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = static_cast<int>(in[i]);
+}
+
+__global__ void tile_row_of_kernel(int rows, const int *__restrict__ trp,
+                                   int *__restrict__ out)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < rows)
+        for (int t = trp[i]; t < trp[i + 1]; ++t)
+            out[t] = static_cast<int>(i);
+}
+
+__global__ void histogram_kernel(const int *__restrict__ keys, int64_t n,
+                                 int64_t *__restrict__ counts)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        atomicAdd(reinterpret_cast<unsigned long long *>(counts + keys[i]), 1ull);
+}
+
+// CSC of a tile matrix from its stably sorted tile ids.
+__global__ void csc_fill_kernel(int64_t n, const int *__restrict__ sorted_ids,
+                                const int *__restrict__ tile_row_of,
+                                int *__restrict__ tile_row_idx,
+                                int *__restrict__ csc_tile_ids)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int t = sorted_ids[i];
+        csc_tile_ids[i] = t;
+        tile_row_idx[i] = tile_row_of[t];
+    }
+}
+
+// B tile b (sorted by K) comes from A tile ids[b]; gather its metadata.
+__global__ void transpose_meta_kernel(int64_t n, const int *__restrict__ ids,
+                                      const int *__restrict__ a_row_of,
+                                      const uint32_t *__restrict__ a_masks,
+                                      const int *__restrict__ a_off,
+                                      int *__restrict__ b_col,
+                                      uint32_t *__restrict__ b_masks,
+                                      int64_t *__restrict__ spans,
+                                      int *__restrict__ b_csc_ids)
+{
+    const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b < n) {
+        const int a = ids[b];
+        b_col[b] = a_row_of[a];
+        b_masks[b] = a_masks[a];
+        spans[b] = a_off[a + 1] - a_off[a];
+        b_csc_ids[a] = static_cast<int>(b);   // A order == B's CSC order
+    }
+}
+
+// Payload copy, one wave per 2 tiles (32 lanes per tile).
+__global__ void transpose_payload_kernel(int64_t n, const int *__restrict__ ids,
+                                         const int *__restrict__ a_off,
+                                         const double *__restrict__ a_vals,
+                                         const int64_t *__restrict__ b_off64,
+                                         int *__restrict__ b_off,
+                                         double *__restrict__ b_vals)
+{
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t b = gid / 32;
+    const int s = static_cast<int>(gid % 32);
+    if (b >= n)
+        return;
+    const int a = ids[b];
+    const int src = a_off[a];
+    const int span = a_off[a + 1] - src;
+    const int64_t dst = b_off64[b];
+    if (s == 0)
+        b_off[b] = static_cast<int>(dst);
+    if (s < span)
+        b_vals[dst + s] = a_vals[src + s];
+}
+
+__global__ void entry_rows_kernel(int rows, const int *__restrict__ rp,
+                                  int *__restrict__ out)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < rows)
+        for (int e = rp[i]; e < rp[i + 1]; ++e)
+            out[e] = static_cast<int>(i);
+}
+
+__global__ void gather_transposed_kernel(int64_t nnz, const int *__restrict__ order,
+                                         const int *__restrict__ entry_row,
+                                         const double *__restrict__ val,
+                                         int *__restrict__ t_col,
+                                         double *__restrict__ t_val)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < nnz) {
+        const int e = order[i];
+        t_col[i] = entry_row[e];
+        t_val[i] = val[e];
+    }
+}
+
+__global__ void nnz_cub_kernel(int64_t nnz, const int *__restrict__ ci,
+                               const int *__restrict__ rp,
+                               const int64_t *__restrict__ coldeg,
+                               unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long part[kThreads / kWave];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    unsigned long long v = 0;
+    if (i < nnz) {
+        const int c = ci[i];
+        v = coldeg ? static_cast<unsigned long long>(coldeg[c])
+                   : static_cast<unsigned long long>(rp[c + 1] - rp[c]);
+    }
+    v = wave_sum(v);
+    if (lane_id() == 0)
+        part[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0;
+        for (int w = 0; w < kThreads / kWave; ++w)
+            s += part[w];
+        atomicAdd(out, s);
+    }
+}
+
+int bits_for(int64_t max_value)
+{
+    int bits = 1;
+    while (bits < 62 && (int64_t(1) << bits) <= max_value)
+        ++bits;
+    return bits;
+}
+
+// Stable sort of int keys with int payload (hipcub LSD radix sort).
+int sort_pairs(const int *k_in, int *k_out, const int *v_in, int *v_out, int64_t n,
+               int bits, DevBuf *tmp, hipStream_t s)
+{
+    size_t bytes = 0;
+    RTT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k_in, k_out, v_in, v_out,
+                                               static_cast<int>(n), 0, bits, s));
+    RTT_TRY(tmp->ensure(bytes));
+    RTT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp->ptr, bytes, k_in, k_out, v_in, v_out,
+                                               static_cast<int>(n), 0, bits, s));
+    return RTT_OK;
+}
+
+int read_i64(const int64_t *d, int64_t *h, hipStream_t s)
+{
+    RTT_HIP(hipMemcpyAsync(h, d, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    RTT_HIP(hipStreamSynchronize(s));
+    return RTT_OK;
+}
+
+// Builds B's tile CSC (tile_col_ptr, tile_row_idx, csc_tile_ids) from its
+// tile CSR: tiles of each tile column in increasing tile-row order
+// (src/gpu_dmma_tiles.h:1003-1088 semantics).
+int build_csc(DevTiles *t, BuildWork *w)
+{
+    hipStream_t s = ctx().stream;
+    const int64_t n = t->num_tiles;
+    const int ncols = t->tile_col_count;
+    RTT_TRY(t->tile_col_ptr.ensure((ncols + 1) * sizeof(int)));
+    RTT_TRY(t->tile_row_idx.ensure(n * sizeof(int)));
+    RTT_TRY(t->csc_tile_ids.ensure(n * sizeof(int)));
+    RTT_TRY(fill_tile_row_of(t));
+    RTT_TRY(w->spans.ensure((ncols + 1) * sizeof(int64_t)));
+    RTT_TRY(w->counters.ensure((ncols + 1) * sizeof(int64_t)));
+    int64_t *cnt = w->spans.as<int64_t>();
+    hipLaunchKernelGGL(set_i64_kernel, dim3(blocks_for(ncols + 1, kThreads)), dim3(kThreads),
+                       0, s, cnt, static_cast<int64_t>(ncols) + 1, int64_t(0));
+    if (n > 0) {
+        hipLaunchKernelGGL(histogram_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0,
+                           s, t->tile_col_idx.as<int>(), n, cnt);
+        RTT_TRY(w->ids_in.ensure(n * sizeof(int)));
+        RTT_TRY(w->ids_out.ensure(n * sizeof(int)));
+        RTT_TRY(w->keys_out.ensure(n * sizeof(int)));
+        hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
+                           w->ids_in.as<int>(), n);
+        RTT_TRY(sort_pairs(t->tile_col_idx.as<int>(), w->keys_out.as<int>(),
+                           w->ids_in.as<int>(), w->ids_out.as<int>(), n, bits_for(ncols),
+                           &w->scan_tmp, s));
+        hipLaunchKernelGGL(csc_fill_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0,
+                           s, n, w->ids_out.as<int>(), t->tile_row_of.as<int>(),
+                           t->tile_row_idx.as<int>(), t->csc_tile_ids.as<int>());
+    }
+    int64_t *ptr64 = w->counters.as<int64_t>();
+    RTT_TRY(exclusive_scan_i64(cnt, ptr64, ncols, &w->scan_tmp, s));
+    hipLaunchKernelGGL(narrow_i64_kernel, dim3(blocks_for(ncols + 1, kThreads)),
+                       dim3(kThreads), 0, s, ptr64, t->tile_col_ptr.as<int>(),
+                       static_cast<int64_t>(ncols) + 1);
+    RTT_HIP(hipGetLastError());
+    t->has_csc = true;
+    return RTT_OK;
+}
+
+template <int TR, int TC, bool CM>
+int run_tile_build(const DevCsr &src, const int *row_map, int active_rows,
+                   int active_cols, int dense_threshold, DevTiles *out,
+                   BuildWork *w)
+{
+    hipStream_t s = ctx().stream;
+    const int R = out->tile_row_count;
+    RTT_TRY(w->row_tiles.ensure((R + 1) * sizeof(int64_t)));
+    RTT_TRY(w->row_payload.ensure((R + 1) * sizeof(int64_t)));
+    RTT_TRY(w->row_dense.ensure((R + 1) * sizeof(int64_t)));
+    // counters: [0, R] tile prefix, [R+1, 2R+1] payload prefix, then dense sum
+    RTT_TRY(w->counters.ensure((2 * (R + 1) + 1) * sizeof(int64_t)));
+    int64_t *tile_pre = w->counters.as<int64_t>();
+    int64_t *pay_pre = tile_pre + (R + 1);
+    int64_t *dense_pre = w->spans.ensure((R + 1) * sizeof(int64_t)) == RTT_OK
+                             ? w->spans.as<int64_t>()
+                             : nullptr;
+    if (!dense_pre)
+        return RTT_ERR_OOM;
+    hipLaunchKernelGGL(set_i64_kernel, dim3(1), dim3(1), 0, s, w->row_tiles.as<int64_t>() + R,
+                       int64_t(1), int64_t(0));
+    hipLaunchKernelGGL(set_i64_kernel, dim3(1), dim3(1), 0, s,
+                       w->row_payload.as<int64_t>() + R, int64_t(1), int64_t(0));
+    hipLaunchKernelGGL(set_i64_kernel, dim3(1), dim3(1), 0, s, w->row_dense.as<int64_t>() + R,
+                       int64_t(1), int64_t(0));
+    const int64_t threads_total = static_cast<int64_t>(R) * TR;
+    if (R > 0)
+        hipLaunchKernelGGL((tile_build_kernel<TR, TC, CM, false>),
+                           dim3(blocks_for(threads_total, kThreads)), dim3(kThreads), 0, s, R,
+                           active_rows, active_cols, src.row_ptr.as<int>(),
+                           src.col_idx.as<int>(), src.values.as<double>(), row_map,
+                           dense_threshold, w->row_tiles.as<int64_t>(),
+                           w->row_payload.as<int64_t>(), w->row_dense.as<int64_t>(),
+                           nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    RTT_HIP(hipGetLastError());
+    RTT_TRY(exclusive_scan_i64(w->row_tiles.as<int64_t>(), tile_pre, R, &w->scan_tmp, s));
+    RTT_TRY(exclusive_scan_i64(w->row_payload.as<int64_t>(), pay_pre, R, &w->scan_tmp, s));
+    RTT_TRY(exclusive_scan_i64(w->row_dense.as<int64_t>(), dense_pre, R, &w->scan_tmp, s));
+    int64_t totals[3] = {0, 0, 0};
+    RTT_TRY(read_i64(tile_pre + R, &totals[0], s));
+    RTT_TRY(read_i64(pay_pre + R, &totals[1], s));
+    RTT_TRY(read_i64(dense_pre + R, &totals[2], s));
+    if (totals[0] > INT_MAX || totals[1] > INT_MAX)
+        return fail(RTT_ERR_OVERFLOW,
+                    "tile count %lld or payload %lld exceeds the int32 tile ABI",
+                    (long long)totals[0], (long long)totals[1]);
+    out->num_tiles = totals[0];
+    out->payload = totals[1];
+    out->dense = totals[2];
+    out->sparse = totals[0] - totals[2];
+    RTT_TRY(out->tile_row_ptr.ensure((R + 1) * sizeof(int)));
+    RTT_TRY(out->tile_col_idx.ensure(out->num_tiles * sizeof(int)));
+    RTT_TRY(out->masks.ensure(out->num_tiles * sizeof(uint32_t)));
+    RTT_TRY(out->value_offsets.ensure((out->num_tiles + 1) * sizeof(int)));
+    RTT_TRY(out->values.ensure(out->payload * sizeof(double)));
+    hipLaunchKernelGGL(narrow_i64_kernel, dim3(blocks_for(R + 1, kThreads)), dim3(kThreads), 0,
+                       s, tile_pre, out->tile_row_ptr.as<int>(), static_cast<int64_t>(R) + 1);
+    // value_offsets[num_tiles] = payload total.
+    hipLaunchKernelGGL(narrow_i64_kernel, dim3(1), dim3(1), 0, s, pay_pre + R,
+                       out->value_offsets.as<int>() + out->num_tiles, int64_t(1));
+    if (R > 0 && out->num_tiles > 0)
+        hipLaunchKernelGGL((tile_build_kernel<TR, TC, CM, true>),
+                           dim3(blocks_for(threads_total, kThreads)), dim3(kThreads), 0, s, R,
+                           active_rows, active_cols, src.row_ptr.as<int>(),
+                           src.col_idx.as<int>(), src.values.as<double>(), row_map,
+                           dense_threshold, nullptr, nullptr, nullptr, tile_pre, pay_pre,
+                           out->tile_col_idx.as<int>(), out->masks.as<uint32_t>(),
+                           out->value_offsets.as<int>(), out->values.as<double>());
+    RTT_HIP(hipGetLastError());
+    return RTT_OK;
+}
+
+} // namespace
+
+void DevTiles::view(rtt_device_tiles *v) const
+{
+    v->rows = rows;
+    v->cols = cols;
+    v->tile_rows = tile_rows;
+    v->tile_cols = tile_cols;
+    v->tile_row_count = tile_row_count;
+    v->tile_col_count = tile_col_count;
+    v->num_tiles = static_cast<int>(num_tiles);
+    v->payload_size = static_cast<int>(payload);
+    v->dense_tiles = static_cast<int>(dense);
+    v->sparse_tiles = static_cast<int>(sparse);
+    v->tile_row_ptr = tile_row_ptr.as<int>();
+    v->tile_col_idx = tile_col_idx.as<int>();
+    v->value_offsets = value_offsets.as<int>();
+    v->masks = masks.as<uint32_t>();
+    v->values = values.as<double>();
+    v->tile_col_ptr = has_csc ? tile_col_ptr.as<int>() : nullptr;
+    v->tile_row_idx = has_csc ? tile_row_idx.as<int>() : nullptr;
+    v->csc_tile_ids = has_csc ? csc_tile_ids.as<int>() : nullptr;
+}
+
+int fill_tile_row_of(DevTiles *t)
+{
+    RTT_TRY(t->tile_row_of.ensure(t->num_tiles * sizeof(int)));
+    if (t->tile_row_count > 0)
+        hipLaunchKernelGGL(tile_row_of_kernel, dim3(blocks_for(t->tile_row_count, kThreads)),
+                           dim3(kThreads), 0, ctx().stream, t->tile_row_count,
+                           t->tile_row_ptr.as<int>(), t->tile_row_of.as<int>());
+    RTT_HIP(hipGetLastError());
+    return RTT_OK;
+}
+
+int upload_csr(const rtt_smatrix &host, DevCsr *out, BuildWork *w, double *h2d_ms,
+               double *validate_ms, double *sort_ms, int *rows_were_sorted)
+{
+    RTT_TRY(require_ctx());
+    hipStream_t s = ctx().stream;
+    if (host.m < 0 || host.n < 0 || host.nnz < 0 || (host.m > 0 && !host.rowpointer) ||
+        (host.nnz > 0 && (!host.columnindex || !host.value)))
+        return fail(RTT_ERR_INVALID, "malformed host CSR");
+    auto t0 = Clock::now();
+    out->rows = host.m;
+    out->cols = host.n;
+    out->nnz = host.nnz;
+    RTT_TRY(out->row_ptr.ensure((static_cast<size_t>(host.m) + 1) * sizeof(int)));
+    RTT_TRY(out->col_idx.ensure(static_cast<size_t>(host.nnz) * sizeof(int)));
+    RTT_TRY(out->values.ensure(static_cast<size_t>(host.nnz) * sizeof(double)));
+    if (host.m > 0)
+        RTT_HIP(hipMemcpyAsync(out->row_ptr.ptr, host.rowpointer,
+                               (static_cast<size_t>(host.m) + 1) * sizeof(int),
+                               hipMemcpyHostToDevice, s));
+    else {
+        int zero = 0;
+        RTT_HIP(hipMemcpyAsync(out->row_ptr.ptr, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+    }
+    if (host.nnz > 0) {
+        RTT_HIP(hipMemcpyAsync(out->col_idx.ptr, host.columnindex,
+                               static_cast<size_t>(host.nnz) * sizeof(int),
+                               hipMemcpyHostToDevice, s));
+        RTT_HIP(hipMemcpyAsync(out->values.ptr, host.value,
+                               static_cast<size_t>(host.nnz) * sizeof(double),
+                               hipMemcpyHostToDevice, s));
+    }
+    RTT_HIP(hipStreamSynchronize(s));
+    *h2d_ms = ms_since(t0);
+
+    t0 = Clock::now();
+    RTT_TRY(w->counters.ensure(sizeof(int64_t)));
+    int *flags = w->counters.as<int>();
+    RTT_HIP(hipMemsetAsync(flags, 0, sizeof(int), s));
+    const int64_t work = std::max<int64_t>(host.m, host.nnz) + 1;
+    hipLaunchKernelGGL(validate_csr_kernel, dim3(blocks_for(work, kThreads)), dim3(kThreads), 0,
+                       s, host.m, host.n, static_cast<int64_t>(host.nnz),
+                       out->row_ptr.as<int>(), out->col_idx.as<int>(), flags);
+    RTT_HIP(hipGetLastError());
+    int hflags = 0;
+    RTT_HIP(hipMemcpyAsync(&hflags, flags, sizeof(int), hipMemcpyDeviceToHost, s));
+    RTT_HIP(hipStreamSynchronize(s));
+    *validate_ms = ms_since(t0);
+    if (hflags & 3)
+        return fail(RTT_ERR_INVALID, "CSR validation failed (flags %d)", hflags);
+    *rows_were_sorted = (hflags & 4) ? 0 : 1;
+
+    t0 = Clock::now();
+    if (hflags & 4) {
+        // Stable per-row sort by column; duplicates keep their input order.
+        DevBuf keys, vals;
+        RTT_TRY(keys.ensure(static_cast<size_t>(host.nnz) * sizeof(int)));
+        RTT_TRY(vals.ensure(static_cast<size_t>(host.nnz) * sizeof(double)));
+        size_t bytes = 0;
+        const int *rp = out->row_ptr.as<int>();
+        RTT_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
+            nullptr, bytes, out->col_idx.as<int>(), keys.as<int>(), out->values.as<double>(),
+            vals.as<double>(), host.nnz, host.m, rp, rp + 1, 0, bits_for(host.n), s));
+        RTT_TRY(w->scan_tmp.ensure(bytes));
+        RTT_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
+            w->scan_tmp.ptr, bytes, out->col_idx.as<int>(), keys.as<int>(),
+            out->values.as<double>(), vals.as<double>(), host.nnz, host.m, rp, rp + 1, 0,
+            bits_for(host.n), s));
+        RTT_HIP(hipMemcpyAsync(out->col_idx.ptr, keys.ptr,
+                               static_cast<size_t>(host.nnz) * sizeof(int),
+                               hipMemcpyDeviceToDevice, s));
+        RTT_HIP(hipMemcpyAsync(out->values.ptr, vals.ptr,
+                               static_cast<size_t>(host.nnz) * sizeof(double),
+                               hipMemcpyDeviceToDevice, s));
+        RTT_HIP(hipStreamSynchronize(s));
+    }
+    *sort_ms = ms_since(t0);
+    return RTT_OK;
+}
+
+int build_tiles(const DevCsr &src, const int *row_map, int active_rows, int active_cols,
+                int logical_rows, int logical_cols, bool col_major, int dense_threshold,
+                bool want_csc, DevTiles *out, BuildWork *w)
+{
+    RTT_TRY(require_ctx());
+    const int TR = col_major ? kTileK : kTileM;
+    const int TC = col_major ? kTileN : kTileK;
+    out->rows = logical_rows;
+    out->cols = logical_cols;
+    out->tile_rows = TR;
+    out->tile_cols = TC;
+    out->tile_row_count = (active_rows + TR - 1) / TR;
+    out->tile_col_count = (active_cols + TC - 1) / TC;
+    out->has_csc = false;
+    if (col_major)
+        RTT_TRY((run_tile_build<kTileK, kTileN, true>(src, row_map, active_rows, active_cols,
+                                                      dense_threshold, out, w)));
+    else
+        RTT_TRY((run_tile_build<kTileM, kTileK, false>(src, row_map, active_rows, active_cols,
+                                                       dense_threshold, out, w)));
+    if (want_csc)
+        RTT_TRY(build_csc(out, w));
+    return RTT_OK;
+}
+
+int transpose_tiles(const DevTiles &a, int b_cols, DevTiles *b, BuildWork *w)
+{
+    RTT_TRY(require_ctx());
+    hipStream_t s = ctx().stream;
+    const int64_t n = a.num_tiles;
+    b->rows = a.cols;
+    b->cols = b_cols;
+    b->tile_rows = kTileK;
+    b->tile_cols = kTileN;
+    b->tile_row_count = a.tile_col_count;
+    b->tile_col_count = (b_cols + kTileN - 1) / kTileN;
+    b->num_tiles = n;
+    b->payload = a.payload;
+    b->dense = a.dense;
+    b->sparse = a.sparse;
+    const int R = b->tile_row_count;
+    const int C = b->tile_col_count;
+    RTT_TRY(b->tile_row_ptr.ensure((R + 1) * sizeof(int)));
+    RTT_TRY(b->tile_col_idx.ensure(n * sizeof(int)));
+    RTT_TRY(b->masks.ensure(n * sizeof(uint32_t)));
+    RTT_TRY(b->value_offsets.ensure((n + 1) * sizeof(int)));
+    RTT_TRY(b->values.ensure(b->payload * sizeof(double)));
+    RTT_TRY(b->tile_col_ptr.ensure((C + 1) * sizeof(int)));
+    RTT_TRY(b->tile_row_idx.ensure(n * sizeof(int)));
+    RTT_TRY(b->csc_tile_ids.ensure(n * sizeof(int)));
+    RTT_TRY(w->spans.ensure((std::max<int64_t>(n, R) + 1) * sizeof(int64_t)));
+    RTT_TRY(w->counters.ensure((std::max<int64_t>(n, R) + 1) * sizeof(int64_t)));
+    RTT_TRY(w->row_tiles.ensure((R + 1) * sizeof(int64_t)));
+    int64_t *cnt = w->row_tiles.as<int64_t>();
+    hipLaunchKernelGGL(set_i64_kernel, dim3(blocks_for(R + 1, kThreads)), dim3(kThreads), 0, s,
+                       cnt, static_cast<int64_t>(R) + 1, int64_t(0));
+    if (n > 0) {
+        RTT_TRY(w->ids_in.ensure(n * sizeof(int)));
+        RTT_TRY(w->ids_out.ensure(n * sizeof(int)));
+        RTT_TRY(w->keys_out.ensure(n * sizeof(int)));
+        hipLaunchKernelGGL(histogram_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0,
+                           s, a.tile_col_idx.as<int>(), n, cnt);
+        hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
+                           w->ids_in.as<int>(), n);
+        RTT_TRY(sort_pairs(a.tile_col_idx.as<int>(), w->keys_out.as<int>(), w->ids_in.as<int>(),
+                           w->ids_out.as<int>(), n, bits_for(R), &w->scan_tmp, s));
+        int64_t *spans = w->spans.as<int64_t>();
+        hipLaunchKernelGGL(set_i64_kernel, dim3(1), dim3(1), 0, s, spans + n, int64_t(1),
+                           int64_t(0));
+        hipLaunchKernelGGL(transpose_meta_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads),
+                           0, s, n, w->ids_out.as<int>(), a.tile_row_of.as<int>(),
+                           a.masks.as<uint32_t>(), a.value_offsets.as<int>(),
+                           b->tile_col_idx.as<int>(), b->masks.as<uint32_t>(), spans,
+                           b->csc_tile_ids.as<int>());
+        int64_t *off64 = w->counters.as<int64_t>();
+        RTT_TRY(exclusive_scan_i64(spans, off64, n, &w->scan_tmp, s));
+        hipLaunchKernelGGL(transpose_payload_kernel, dim3(blocks_for(n * 32, kThreads)),
+                           dim3(kThreads), 0, s, n, w->ids_out.as<int>(),
+                           a.value_offsets.as<int>(), a.values.as<double>(), off64,
+                           b->value_offsets.as<int>(), b->values.as<double>());
+        hipLaunchKernelGGL(narrow_i64_kernel, dim3(1), dim3(1), 0, s, off64 + n,
+                           b->value_offsets.as<int>() + n, int64_t(1));
+        // CSC of B = CSR of A: tile_col_ptr = A.tile_row_ptr (padded to C + 1),
+        // tile_row_idx = A.tile_col_idx.
+        RTT_HIP(hipMemcpyAsync(b->tile_row_idx.ptr, a.tile_col_idx.ptr, n * sizeof(int),
+                               hipMemcpyDeviceToDevice, s));
+    } else {
+        int zero = 0;
+        RTT_HIP(hipMemcpyAsync(b->value_offsets.ptr, &zero, sizeof(int), hipMemcpyHostToDevice,
+                               s));
+    }
+    int64_t *ptr64 = w->row_payload.ensure((R + 1) * sizeof(int64_t)) == RTT_OK
+                         ? w->row_payload.as<int64_t>()
+                         : nullptr;
+    if (!ptr64)
+        return RTT_ERR_OOM;
+    RTT_TRY(exclusive_scan_i64(cnt, ptr64, R, &w->scan_tmp, s));
+    hipLaunchKernelGGL(narrow_i64_kernel, dim3(blocks_for(R + 1, kThreads)), dim3(kThreads), 0,
+                       s, ptr64, b->tile_row_ptr.as<int>(), static_cast<int64_t>(R) + 1);
+    // tile_col_ptr: A's tile_row_ptr for A's tile rows, then the total for
+    // the empty trailing tile columns.
+    const int ar = a.tile_row_count;
+    RTT_HIP(hipMemcpyAsync(b->tile_col_ptr.ptr, a.tile_row_ptr.ptr, (ar + 1) * sizeof(int),
+                           hipMemcpyDeviceToDevice, s));
+    if (C > ar) {
+        std::vector<int> tail(C - ar, static_cast<int>(n));
+        RTT_HIP(hipMemcpyAsync(b->tile_col_ptr.as<int>() + ar + 1, tail.data(),
+                               tail.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        RTT_HIP(hipStreamSynchronize(s));
+    }
+    RTT_HIP(hipGetLastError());
+    b->has_csc = true;
+    return RTT_OK;
+}
+
+int transpose_csr(const DevCsr &src, DevCsr *out, BuildWork *w)
+{
+    RTT_TRY(require_ctx());
+    hipStream_t s = ctx().stream;
+    const int64_t nnz = src.nnz;
+    out->rows = src.cols;
+    out->cols = src.rows;
+    out->nnz = nnz;
+    RTT_TRY(out->row_ptr.ensure((static_cast<size_t>(src.cols) + 1) * sizeof(int)));
+    RTT_TRY(out->col_idx.ensure(nnz * sizeof(int)));
+    RTT_TRY(out->values.ensure(nnz * sizeof(double)));
+    RTT_TRY(w->row_tiles.ensure((static_cast<size_t>(src.cols) + 1) * sizeof(int64_t)));
+    RTT_TRY(w->row_payload.ensure((static_cast<size_t>(src.cols) + 1) * sizeof(int64_t)));
+    int64_t *cnt = w->row_tiles.as<int64_t>();
+    hipLaunchKernelGGL(set_i64_kernel, dim3(blocks_for(src.cols + 1, kThreads)), dim3(kThreads),
+                       0, s, cnt, static_cast<int64_t>(src.cols) + 1, int64_t(0));
+    if (nnz > 0) {
+        RTT_TRY(w->ids_in.ensure(nnz * sizeof(int)));
+        RTT_TRY(w->ids_out.ensure(nnz * sizeof(int)));
+        RTT_TRY(w->keys_out.ensure(nnz * sizeof(int)));
+        RTT_TRY(w->spans.ensure(nnz * sizeof(int)));
+        hipLaunchKernelGGL(histogram_kernel, dim3(blocks_for(nnz, kThreads)), dim3(kThreads), 0,
+                           s, src.col_idx.as<int>(), nnz, cnt);
+        hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(nnz, kThreads)), dim3(kThreads), 0, s,
+                           w->ids_in.as<int>(), nnz);
+        RTT_TRY(sort_pairs(src.col_idx.as<int>(), w->keys_out.as<int>(), w->ids_in.as<int>(),
+                           w->ids_out.as<int>(), nnz, bits_for(src.cols), &w->scan_tmp, s));
+        int *entry_row = w->spans.as<int>();
+        hipLaunchKernelGGL(entry_rows_kernel, dim3(blocks_for(src.rows, kThreads)),
+                           dim3(kThreads), 0, s, src.rows, src.row_ptr.as<int>(), entry_row);
+        hipLaunchKernelGGL(gather_transposed_kernel, dim3(blocks_for(nnz, kThreads)),
+                           dim3(kThreads), 0, s, nnz, w->ids_out.as<int>(), entry_row,
+                           src.values.as<double>(), out->col_idx.as<int>(),
+                           out->values.as<double>());
+    }
+    int64_t *ptr64 = w->row_payload.as<int64_t>();
+    RTT_TRY(exclusive_scan_i64(cnt, ptr64, src.cols, &w->scan_tmp, s));
+    hipLaunchKernelGGL(narrow_i64_kernel, dim3(blocks_for(src.cols + 1, kThreads)),
+                       dim3(kThreads), 0, s, ptr64, out->row_ptr.as<int>(),
+                       static_cast<int64_t>(src.cols) + 1);
+    RTT_HIP(hipGetLastError());
+    return RTT_OK;
+}
+
+int nnz_cub(const DevCsr &csr, bool aat, unsigned long long *result, BuildWork *w)
+{
+    RTT_TRY(require_ctx());
+    hipStream_t s = ctx().stream;
+    RTT_TRY(w->counters.ensure(sizeof(unsigned long long)));
+    unsigned long long *acc = w->counters.as<unsigned long long>();
+    RTT_HIP(hipMemsetAsync(acc, 0, sizeof(unsigned long long), s));
+    const int64_t *deg = nullptr;
+    if (aat) {
+        RTT_TRY(w->row_tiles.ensure((static_cast<size_t>(csr.cols) + 1) * sizeof(int64_t)));
+        int64_t *d = w->row_tiles.as<int64_t>();
+        hipLaunchKernelGGL(set_i64_kernel, dim3(blocks_for(csr.cols + 1, kThreads)),
+                           dim3(kThreads), 0, s, d, static_cast<int64_t>(csr.cols) + 1,
+                           int64_t(0));
+        if (csr.nnz > 0)
+            hipLaunchKernelGGL(histogram_kernel, dim3(blocks_for(csr.nnz, kThreads)),
+                               dim3(kThreads), 0, s, csr.col_idx.as<int>(), csr.nnz, d);
+        deg = d;
+    }
+    if (csr.nnz > 0)
+        hipLaunchKernelGGL(nnz_cub_kernel, dim3(blocks_for(csr.nnz, kThreads)), dim3(kThreads),
+                           0, s, csr.nnz, csr.col_idx.as<int>(), csr.row_ptr.as<int>(), deg,
+                           acc);
+    RTT_HIP(hipGetLastError());
+    RTT_HIP(hipMemcpyAsync(result, acc, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    RTT_HIP(hipStreamSynchronize(s));
+    return RTT_OK;
+}
+
+} // namespace rtt

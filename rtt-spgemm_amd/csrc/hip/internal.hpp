@@ -1,0 +1,144 @@
+// Internal device-side data structures and launcher declarations.  Only the
+// library's own .hip files include this header.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../rtt_common.hpp"
+
+namespace rtt {
+
+constexpr int kTileM = 8;   // A tile rows / C tile rows
+constexpr int kTileK = 4;   // A tile cols / B tile rows
+constexpr int kTileN = 8;   // B tile cols / C tile cols
+constexpr int kInputElems = 32;
+
+// Checks a HIP call; on failure records the error and returns RTT_ERR_DEVICE.
+#define RTT_HIP(call)                                                            \
+    do {                                                                         \
+        hipError_t rtt_err_ = (call);                                            \
+        if (rtt_err_ != hipSuccess)                                              \
+            return ::rtt::fail(RTT_ERR_DEVICE, "%s failed: %s (%s:%d)", #call,   \
+                               hipGetErrorString(rtt_err_), __FILE__, __LINE__); \
+    } while (0)
+
+#define RTT_TRY(expr)                  \
+    do {                               \
+        int rtt_rc_ = (expr);          \
+        if (rtt_rc_ != RTT_OK)         \
+            return rtt_rc_;            \
+    } while (0)
+
+// Grow-only device buffer.  ensure() reallocates only when the requested
+// capacity exceeds what is held, so per-iteration rebuilds reuse memory.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+    template <typename T>
+    T *as() const { return static_cast<T *>(ptr); }
+    ~DevBuf() { release(); }
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+};
+
+// Device CSR with rows sorted by column (duplicates adjacent).
+struct DevCsr {
+    int rows = 0, cols = 0;
+    int64_t nnz = 0;
+    DevBuf row_ptr, col_idx, values;
+};
+
+// Hybrid tile matrix on the device (the DmmaDeviceTiles fields).
+struct DevTiles {
+    int rows = 0, cols = 0, tile_rows = 0, tile_cols = 0;
+    int tile_row_count = 0, tile_col_count = 0;
+    int64_t num_tiles = 0, payload = 0, dense = 0, sparse = 0;
+    bool has_csc = false;
+    DevBuf tile_row_ptr, tile_col_idx, value_offsets, masks, values;
+    DevBuf tile_col_ptr, tile_row_idx, csc_tile_ids;
+    DevBuf tile_row_of;   // tile -> tile row (A only; used by the B transpose)
+    void view(rtt_device_tiles *v) const;
+};
+
+// Last C produced on a prepared-A handle (64-bit offsets).
+struct DevC {
+    int64_t tilem = 0, tilen = 0, num_tiles = 0, nnz = 0;
+    int rows = 0, cols = 0;
+    DevBuf tile_ptr, tile_col, tile_mask, tile_nnz, tile_csr_ptr, tile_csr_col,
+        tile_csr_val;
+};
+
+// Scratch of the SpGEMM core, reused across calls.
+struct SpgemmWork {
+    DevBuf row_tiles, row_nnz, row_level, row_pairs;   // per A tile row
+    DevBuf scan_tmp, counters, l3_rows, num_rows, slab;
+    DevBuf c_row_nnz_off;                               // per tile row, int64
+};
+
+// Scratch of the tile builders, reused across calls.
+struct BuildWork {
+    DevBuf row_tiles, row_payload, row_dense, scan_tmp, counters;
+    DevBuf keys_in, keys_out, ids_in, ids_out, spans;
+    DevBuf tcsr_row_ptr, tcsr_col, tcsr_val;   // transposed CSR (reordered AAT)
+};
+
+struct Events {
+    hipEvent_t ev[16] = {};
+    bool ready = false;
+    int create();
+    ~Events();
+};
+
+// Library-wide device context (one device per process).
+struct Context {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int num_cus = 0;
+};
+Context &ctx();
+int require_ctx();
+
+// ---- CSR / tile builders (build.hip) ------------------------------------
+// Uploads and validates a host CSR; sorts rows by column when needed.
+int upload_csr(const rtt_smatrix &host, DevCsr *out, BuildWork *work,
+               double *h2d_ms, double *validate_ms, double *sort_ms,
+               int *rows_were_sorted);
+// Builds hybrid tiles from a row-sorted CSR.  row_map (device, may be null)
+// maps tiled row -> source row for rows < active_rows.
+int build_tiles(const DevCsr &src, const int *row_map, int active_rows,
+                int active_cols, int logical_rows, int logical_cols,
+                bool col_major, int dense_threshold, bool build_csc,
+                DevTiles *out, BuildWork *work);
+// B = A^T at tile level (A tile (J,K) row-major == B tile (K,J) col-major).
+int transpose_tiles(const DevTiles &a, int b_cols, DevTiles *b, BuildWork *work);
+// Transposed CSR of `src` (stable: rows ascending within each column).
+int transpose_csr(const DevCsr &src, DevCsr *out, BuildWork *work);
+int nnz_cub(const DevCsr &csr, bool aat, unsigned long long *out, BuildWork *work);
+int fill_tile_row_of(DevTiles *t);
+
+// ---- SpGEMM core (spgemm.hip) ---------------------------------------------
+struct CoreTimes {
+    double symbolic_ms = 0, allocation_ms = 0, numeric_ms = 0;
+    double sym_count_kernel_ms = 0, sym_fill_kernel_ms = 0, numeric_kernel_ms = 0;
+    unsigned long long pairs = 0, heavy_rows = 0;
+};
+int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *work,
+                Events *events, CoreTimes *times);
+
+// ---- scans (scan.hip) -------------------------------------------------------
+// Exclusive sum of n int64 values into out[0..n] (out[n] = total).
+int exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, DevBuf *tmp,
+                       hipStream_t s);
+
+inline unsigned int blocks_for(int64_t n, int threads)
+{
+    return static_cast<unsigned int>((n + threads - 1) / threads);
+}
+
+} // namespace rtt

@@ -1,0 +1,253 @@
+// rtt_spgemm: drop-in driver for the C = A*A / C = A*A^T benchmark path.
+//
+// Mirrors the reference driver's flags (src/main.cu:168-309), its timed
+// boundary (B rebuild + symbolic + allocation + numeric, median over the
+// iterations, A preparation untimed: src/main.cu:2171-2318) and its
+// RTT_RESULT summary line (src/main.cu:2804-2854).
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "rtt_spgemm.h"
+
+namespace {
+
+struct Options {
+    int device = 0;
+    int aat = 0;
+    int dense_threshold = 24;
+    int warmup = 0;
+    int iterations = 5;
+    int lb_mode = RTT_LB_ADAPTIVE_TAIL;
+    const char *lb_name = "adaptive-tail";
+    bool no_reorder = false;
+    bool reference_values = false;
+    bool prepare_only = false;
+    const char *row_order = nullptr;
+    const char *inner_order = nullptr;
+    const char *reorder_name = nullptr;
+    const char *a_file = nullptr;
+};
+
+void usage(const char *prog)
+{
+    std::printf(
+        "Usage: %s [options] A.mtx\n"
+        "  -d GPU                         HIP device (default: 0)\n"
+        "  -aat 0|1                       C = A*A (0) or C = A*A^T (1)\n"
+        "  --lb-mode static|dynamic|adaptive-tail (default: adaptive-tail)\n"
+        "  --warmup-iterations N          warmups (default: 0)\n"
+        "  --iterations N                 repeats (default: 5)\n"
+        "  --no-reorder                   identity layout\n"
+        "  --row-order FILE --inner-order FILE --reorder-name NAME\n"
+        "  --reference-values             replace values by i %% 10 as the\n"
+        "                                 reference driver does\n"
+        "  --prepare-only\n",
+        prog);
+}
+
+bool parse(int argc, char **argv, Options *o)
+{
+    for (int i = 1; i < argc; ++i) {
+        const char *a = argv[i];
+        auto next = [&](void) -> const char * { return i + 1 < argc ? argv[++i] : nullptr; };
+        if (!std::strcmp(a, "-d")) {
+            const char *v = next();
+            if (!v)
+                return false;
+            o->device = std::atoi(v);
+        } else if (!std::strcmp(a, "-aat")) {
+            const char *v = next();
+            if (!v)
+                return false;
+            o->aat = std::atoi(v);
+        } else if (!std::strcmp(a, "--iterations")) {
+            const char *v = next();
+            if (!v)
+                return false;
+            o->iterations = std::atoi(v);
+        } else if (!std::strcmp(a, "--warmup-iterations")) {
+            const char *v = next();
+            if (!v)
+                return false;
+            o->warmup = std::atoi(v);
+        } else if (!std::strcmp(a, "--lb-mode")) {
+            const char *v = next();
+            if (!v)
+                return false;
+            if (!std::strcmp(v, "static"))
+                o->lb_mode = RTT_LB_STATIC, o->lb_name = "static";
+            else if (!std::strcmp(v, "dynamic"))
+                o->lb_mode = RTT_LB_DYNAMIC, o->lb_name = "dynamic";
+            else if (!std::strcmp(v, "adaptive-tail") || !std::strcmp(v, "tail"))
+                o->lb_mode = RTT_LB_ADAPTIVE_TAIL, o->lb_name = "adaptive-tail";
+            else
+                return false;
+        } else if (!std::strcmp(a, "--no-reorder")) {
+            o->no_reorder = true;
+        } else if (!std::strcmp(a, "--row-order")) {
+            if (!(o->row_order = next()))
+                return false;
+        } else if (!std::strcmp(a, "--inner-order")) {
+            if (!(o->inner_order = next()))
+                return false;
+        } else if (!std::strcmp(a, "--reorder-name")) {
+            if (!(o->reorder_name = next()))
+                return false;
+        } else if (!std::strcmp(a, "--reference-values")) {
+            o->reference_values = true;
+        } else if (!std::strcmp(a, "--prepare-only")) {
+            o->prepare_only = true;
+        } else if (a[0] != '-' && !o->a_file) {
+            o->a_file = a;
+        } else {
+            std::fprintf(stderr, "unsupported option '%s'\n", a);
+            return false;
+        }
+    }
+    const bool any_ext = o->row_order || o->inner_order || o->reorder_name;
+    const bool all_ext = o->row_order && o->inner_order && o->reorder_name;
+    return o->a_file && (o->aat == 0 || o->aat == 1) && o->warmup >= 0 && o->iterations > 0 &&
+           (!any_ext || all_ext) && !(o->no_reorder && any_ext);
+}
+
+double median_of(const std::vector<double> &v, size_t *selected)
+{
+    std::vector<size_t> order(v.size());
+    for (size_t i = 0; i < order.size(); ++i)
+        order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return v[x] < v[y]; });
+    *selected = order[order.size() / 2];
+    return v[*selected];
+}
+
+int check(int rc, const char *what)
+{
+    if (rc != RTT_OK)
+        std::fprintf(stderr, "%s failed: %s\n", what, rtt_last_error());
+    return rc;
+}
+
+} // namespace
+
+int main(int argc, char **argv)
+{
+    Options o;
+    if (!parse(argc, argv, &o)) {
+        usage(argv[0]);
+        return EXIT_FAILURE;
+    }
+    rtt_smatrix A;
+    std::memset(&A, 0, sizeof(A));
+    if (check(rtt_read_mtx(o.a_file, o.reference_values ? 1 : 0, &A), "read"))
+        return EXIT_FAILURE;
+    if (!o.aat && A.m != A.n) {
+        std::fprintf(stderr, "C = A*A needs a square matrix (got %d x %d).\n", A.m, A.n);
+        return EXIT_FAILURE;
+    }
+    if (check(rtt_init(o.device), "rtt_init"))
+        return EXIT_FAILURE;
+    std::printf("A: %d x %d, nnz %d, aat %d, lb-mode %s\n", A.m, A.n, A.nnz, o.aat, o.lb_name);
+
+    rtt_prepared_a *pa = nullptr;
+    rtt_offline_a_stats ast;
+    std::vector<int> row_order, inner_order;
+    const char *layout = "identity";
+    int rc;
+    if (o.row_order) {
+        row_order.resize(A.m);
+        inner_order.resize(A.n);
+        if (check(rtt_read_order_file(o.row_order, A.m, row_order.data()), "row order") ||
+            check(rtt_read_order_file(o.inner_order, A.n, inner_order.data()), "inner order"))
+            return EXIT_FAILURE;
+        rc = rtt_prepare_external_a(&A, row_order.data(), inner_order.data(), o.dense_threshold,
+                                    &pa, &ast);
+        layout = "external";
+    } else {
+        // The learned BGRF layout is not part of this build; the default is
+        // the identity layout with the empty suffix trimmed (--no-reorder).
+        rc = rtt_prepare_identity_a(&A, o.dense_threshold, &pa, &ast);
+    }
+    if (check(rc, "prepare A"))
+        return EXIT_FAILURE;
+    std::printf("A prepared (%s): upload %.3f ms, validate %.3f ms, row sort %.3f ms, "
+                "tiles %.3f ms\n",
+                layout, ast.h2d_ms, ast.validation_ms, ast.canonicalize_ms, ast.tile_build_ms);
+    if (o.prepare_only) {
+        rtt_destroy_prepared_a(pa);
+        rtt_destroy_smatrix(&A);
+        return EXIT_SUCCESS;
+    }
+    unsigned long long nnz_cub = 0;
+    double cub_ms = 0;
+    if (check(rtt_compute_nnz_cub(pa, o.aat, &nnz_cub, &cub_ms), "nnzCub"))
+        return EXIT_FAILURE;
+    std::printf("SpGEMM nnzCub = %llu; GPU nnzCub time = %.3f ms\n", nnz_cub, cub_ms);
+
+    rtt_schedule sched;
+    rtt_schedule_defaults(&sched);
+    sched.lb_mode = o.lb_mode;
+    rtt_dynamic_b *pb = nullptr;
+    rtt_smatrix C;
+    std::memset(&C, 0, sizeof(C));
+    std::vector<double> b_ms, b_perm_ms, b_tile_ms, sym_ms, alloc_ms, num_ms, total_ms;
+    for (int it = 0; it < o.warmup + o.iterations; ++it) {
+        const bool timed = it >= o.warmup;
+        const bool last = it + 1 == o.warmup + o.iterations;
+        sched.materialize_output = last ? 1 : 0;
+        rtt_b_update_stats bst;
+        rtt_stats st;
+        if (check(rtt_sync(), "sync") || check(rtt_rebuild_b(pa, o.aat, &pb, &bst), "B rebuild") ||
+            check(rtt_tilespgemm(pa, pb, &C, &st, &sched), "SpGEMM"))
+            return EXIT_FAILURE;
+        if (!timed)
+            continue;
+        b_ms.push_back(bst.total_ms);
+        b_perm_ms.push_back(bst.csr_row_permute_ms);
+        b_tile_ms.push_back(bst.csr2tile_ms);
+        sym_ms.push_back(st.symbolic_ms);
+        alloc_ms.push_back(st.allocation_ms);
+        num_ms.push_back(st.numeric_ms);
+        total_ms.push_back(bst.total_ms + st.symbolic_ms + st.allocation_ms + st.numeric_ms);
+        std::printf("iteration %d: b_update %.3f ms symbolic %.3f ms malloc %.3f ms "
+                    "numeric %.3f ms tiles %llu nnzC %llu\n",
+                    it - o.warmup + 1, bst.total_ms, st.symbolic_ms, st.allocation_ms,
+                    st.numeric_ms, st.output_tiles, st.output_nnz);
+    }
+    size_t sel = 0;
+    const double runtime = median_of(total_ms, &sel);
+    const double gflops = runtime > 0 ? 2.0 * static_cast<double>(nnz_cub) / (runtime * 1e6) : 0.0;
+    std::printf("RTT-SpGEMM (%s), repeats = %d, statistic = median\n", o.lb_name, o.iterations);
+    std::printf("B online preparation runtime is %.3f ms\n", b_ms[sel]);
+    std::printf("  CSR row permutation is %.3f ms; CSR2Tile is %.3f ms\n", b_perm_ms[sel],
+                b_tile_ms[sel]);
+    std::printf("Symbolic runtime is %.3f ms\n", sym_ms[sel]);
+    std::printf("Malloc runtime is %.3f ms\n", alloc_ms[sel]);
+    std::printf("Numeric runtime is %.3f ms\n", num_ms[sel]);
+    std::printf("HIP   RTT-SpGEMM runtime is %.3f ms, gflops = %.2f\n", runtime, gflops);
+    std::printf("RTT_RESULT matrix=%s mode=%s repeats=%d statistic=median "
+                "selected_iteration=%zu b_permutation_ms=%.6f "
+                "b_csr_row_permute_ms=%.6f b_csr2tile_ms=%.6f "
+                "symbolic_ms=%.6f malloc_ms=%.6f "
+                "numeric_ms=%.6f runtime_ms=%.6f gflops=%.6f "
+                "input_h2d_included=0 output_d2h_included=0 "
+                "validation_included=0 capacity_workspace_reuse=1\n",
+                o.a_file, o.lb_name, o.iterations, sel + 1, b_ms[sel], b_perm_ms[sel],
+                b_tile_ms[sel], sym_ms[sel], alloc_ms[sel], num_ms[sel], runtime, gflops);
+    // Convert the exported tiles to CSR (untimed, as in the reference).
+    const int *perm = rtt_prepared_a_row_new_to_old(pa);
+    int active_rows = 0;
+    rtt_prepared_a_info(pa, &active_rows, nullptr, nullptr);
+    rc = perm ? rtt_tile2csr_restore_rows(&C, perm, active_rows, 1) : rtt_tile2csr(&C, 1);
+    if (check(rc, "tile2csr"))
+        return EXIT_FAILURE;
+    std::printf("C: %d x %d, nnz %d, tiles %d\n", C.m, C.n, C.nnz, C.numtile);
+    rtt_destroy_smatrix(&C);
+    rtt_destroy_dynamic_b(pb);
+    rtt_destroy_prepared_a(pa);
+    rtt_destroy_smatrix(&A);
+    std::printf("---------------------------------------------------------------\n");
+    return EXIT_SUCCESS;
+}

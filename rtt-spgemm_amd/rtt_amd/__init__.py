@@ -1,0 +1,522 @@
+"""Python host-side mirror of the tiled SpGEMM driver interface.
+
+Thin ctypes layer over ``lib/librtt_spgemm.so`` (the C ABI declared in
+``include/rtt_spgemm.h``).  Names follow the reference's host API:
+
+=====================================  =========================================
+reference (file:line)                  here
+=====================================  =========================================
+mmio_allinone  src/mmio_highlevel.h:593 read_mtx
+gpu_prepare_identity_a                  prepare_identity_a
+  src/gpu_dmma_tiles.h:2443
+gpu_prepare_external_a  :2389           prepare_external_a
+gpu_rebuild_dynamic_b_via_permuted_csr  rebuild_b(a, aat=0)
+  / _transpose  :2275 / :2285           rebuild_b(a, aat=1)
+gpu_compute_nnz_cub_derived  :2304      nnz_cub
+dmma_tilespgemm src/dmma_spgemm.h:6490  tilespgemm
+tile2csr / tile2csr_restore_rows        tile2csr / tile2csr_restore_rows
+  src/tile2csr.h:232 / :337
+=====================================  =========================================
+
+Every compute call runs the HIP library; there is no CPU fallback.  Importing
+works without a GPU; ``init()`` fails loudly when no device or library is
+present.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "librtt_spgemm.so")
+HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "rtt_spgemm.h")
+
+RTT_OK = 0
+STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "OOM", 3: "OVERFLOW", 4: "DEVICE", 5: "UNSUPPORTED"}
+LB_MODES = {"static": 0, "dynamic": 1, "adaptive-tail": 2}
+
+
+class RttError(RuntimeError):
+    def __init__(self, code: int, where: str, text: str):
+        super().__init__(f"{where}: {STATUS_NAMES.get(code, code)}: {text}")
+        self.code = code
+
+
+class SMatrix(C.Structure):
+    """The legacy SMatrix layout (src/common.h:48-73)."""
+
+    _fields_ = [
+        ("m", C.c_int), ("n", C.c_int), ("nnz", C.c_int), ("isSymmetric", C.c_int),
+        ("value", C.POINTER(C.c_double)), ("columnindex", C.POINTER(C.c_int)),
+        ("rowpointer", C.POINTER(C.c_int)), ("tilem", C.c_int), ("tilen", C.c_int),
+        ("tile_ptr", C.POINTER(C.c_int)), ("tile_columnidx", C.POINTER(C.c_int)),
+        ("tile_rowidx", C.POINTER(C.c_int)), ("tile_nnz", C.POINTER(C.c_int)),
+        ("numtile", C.c_int), ("tile_csr_Value", C.POINTER(C.c_double)),
+        ("tile_csr_Col", C.POINTER(C.c_ubyte)), ("tile_csr_Ptr", C.POINTER(C.c_ubyte)),
+        ("mask", C.POINTER(C.c_ushort)), ("csc_tile_ptr", C.POINTER(C.c_int)),
+        ("csc_tile_rowidx", C.POINTER(C.c_int)),
+    ]
+
+
+class DeviceTiles(C.Structure):
+    _fields_ = [(n, C.c_int) for n in (
+        "rows", "cols", "tile_rows", "tile_cols", "tile_row_count", "tile_col_count",
+        "num_tiles", "payload_size", "dense_tiles", "sparse_tiles")] + [
+        ("tile_row_ptr", C.c_void_p), ("tile_col_idx", C.c_void_p),
+        ("value_offsets", C.c_void_p), ("masks", C.c_void_p), ("values", C.c_void_p),
+        ("tile_col_ptr", C.c_void_p), ("tile_row_idx", C.c_void_p),
+        ("csc_tile_ids", C.c_void_p)]
+
+
+class OfflineAStats(C.Structure):
+    _fields_ = [("h2d_ms", C.c_double), ("validation_ms", C.c_double),
+                ("canonicalize_ms", C.c_double), ("reorder_ms", C.c_double),
+                ("tile_build_ms", C.c_double), ("total_ms", C.c_double),
+                ("rows_were_sorted", C.c_int)]
+
+
+class BUpdateStats(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("csr_row_permute_ms", C.c_double),
+                ("csr2tile_ms", C.c_double), ("csc_ms", C.c_double), ("path", C.c_int)]
+
+
+class Schedule(C.Structure):
+    _fields_ = [("lb_mode", C.c_int), ("materialize_output", C.c_int),
+                ("split_threshold", C.c_double), ("split_chunk_target", C.c_double),
+                ("split_max_chunks", C.c_int), ("max_heavy_fraction", C.c_double),
+                ("tail_record_capacity", C.c_int), ("force_tail_split", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("symbolic_ms", C.c_double), ("allocation_ms", C.c_double),
+                ("numeric_ms", C.c_double), ("total_ms", C.c_double),
+                ("output_copy_ms", C.c_double), ("symbolic_count_kernel_ms", C.c_double),
+                ("symbolic_fill_kernel_ms", C.c_double), ("numeric_kernel_ms", C.c_double),
+                ("pair_count", C.c_ulonglong), ("output_tiles", C.c_ulonglong),
+                ("output_nnz", C.c_ulonglong), ("heavy_rows", C.c_ulonglong),
+                ("heavy_chunks", C.c_ulonglong), ("output_materialized", C.c_int)]
+
+
+class DeviceC(C.Structure):
+    _fields_ = [("tilem", C.c_int64), ("tilen", C.c_int64), ("num_tiles", C.c_int64),
+                ("nnz", C.c_int64), ("tile_ptr", C.c_void_p), ("tile_col", C.c_void_p),
+                ("tile_mask", C.c_void_p), ("tile_nnz", C.c_void_p),
+                ("tile_csr_ptr", C.c_void_p), ("tile_csr_col", C.c_void_p),
+                ("tile_csr_val", C.c_void_p)]
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    sig = {
+        "rtt_last_error": (C.c_char_p, []),
+        "rtt_version": (C.c_char_p, []),
+        "rtt_destroy_smatrix": (None, [P(SMatrix)]),
+        "rtt_read_mtx": (C.c_int, [C.c_char_p, C.c_int, P(SMatrix)]),
+        "rtt_write_mtx": (C.c_int, [C.c_char_p, P(SMatrix)]),
+        "rtt_init": (C.c_int, [C.c_int]),
+        "rtt_sync": (C.c_int, []),
+        "rtt_download_tiles": (C.c_int, [P(DeviceTiles)] + [vp] * 8),
+        "rtt_prepare_identity_a": (C.c_int, [P(SMatrix), C.c_int, P(vp), P(OfflineAStats)]),
+        "rtt_prepare_external_a": (C.c_int, [P(SMatrix), vp, vp, C.c_int, P(vp),
+                                             P(OfflineAStats)]),
+        "rtt_read_order_file": (C.c_int, [C.c_char_p, C.c_int, vp]),
+        "rtt_prepared_a_view": (C.c_int, [vp, P(DeviceTiles)]),
+        "rtt_prepared_a_info": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_int)]),
+        "rtt_prepared_a_row_new_to_old": (vp, [vp]),
+        "rtt_destroy_prepared_a": (None, [vp]),
+        "rtt_rebuild_b": (C.c_int, [vp, C.c_int, P(vp), P(BUpdateStats)]),
+        "rtt_dynamic_b_view": (C.c_int, [vp, P(DeviceTiles)]),
+        "rtt_destroy_dynamic_b": (None, [vp]),
+        "rtt_compute_nnz_cub": (C.c_int, [vp, C.c_int, P(C.c_ulonglong), P(C.c_double)]),
+        "rtt_schedule_defaults": (None, [P(Schedule)]),
+        "rtt_tilespgemm": (C.c_int, [vp, vp, P(SMatrix), P(Stats), P(Schedule)]),
+        "rtt_last_c_view": (C.c_int, [vp, P(DeviceC)]),
+        "rtt_memcpy_d2h": (C.c_int, [vp, vp, C.c_size_t]),
+        "rtt_tile2csr": (C.c_int, [P(SMatrix), C.c_int]),
+        "rtt_tile2csr_restore_rows": (C.c_int, [P(SMatrix), vp, C.c_int, C.c_int]),
+        "rtt_gen_laplace2d": (C.c_int, [C.c_int, C.c_double, C.c_uint64, P(SMatrix)]),
+        "rtt_gen_stencil27": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_uint64,
+                                        P(SMatrix)]),
+        "rtt_gen_rmat": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                   C.c_uint64, P(SMatrix)]),
+        "rtt_gen_random": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, P(SMatrix)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int, where: str) -> None:
+    if rc != RTT_OK:
+        raise RttError(rc, where, load().rtt_last_error().decode())
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+@dataclass
+class CsrMatrix:
+    """Host CSR (int32 row pointer / columns, float64 values)."""
+    m: int
+    n: int
+    rowptr: np.ndarray
+    col: np.ndarray
+    val: np.ndarray
+    is_symmetric: int = 0
+
+    @property
+    def nnz(self) -> int:
+        return int(self.rowptr[self.m]) if self.m > 0 else 0
+
+    def as_smatrix(self) -> SMatrix:
+        self.rowptr = np.ascontiguousarray(self.rowptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(self.col, dtype=np.int32)
+        self.val = np.ascontiguousarray(self.val, dtype=np.float64)
+        s = SMatrix()
+        s.m, s.n, s.nnz, s.isSymmetric = self.m, self.n, self.nnz, self.is_symmetric
+        s.rowpointer = self.rowptr.ctypes.data_as(C.POINTER(C.c_int))
+        s.columnindex = self.col.ctypes.data_as(C.POINTER(C.c_int))
+        s.value = self.val.ctypes.data_as(C.POINTER(C.c_double))
+        return s
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.val, self.col, self.rowptr), shape=(self.m, self.n))
+
+
+def _np_copy(ptr, count: int, dtype) -> np.ndarray:
+    if count <= 0 or not ptr:
+        return np.zeros(0, dtype=dtype)
+    ct = np.ctypeslib.as_ctypes_type(np.dtype(dtype))
+    addr = C.cast(ptr, C.c_void_p).value
+    buf = (ct * count).from_address(addr)
+    return np.array(np.ctypeslib.as_array(buf), dtype=dtype, copy=True)
+
+
+def _csr_from_smatrix(s: SMatrix) -> CsrMatrix:
+    out = CsrMatrix(s.m, s.n, _np_copy(s.rowpointer, s.m + 1, np.int32),
+                    _np_copy(s.columnindex, s.nnz, np.int32),
+                    _np_copy(s.value, s.nnz, np.float64), s.isSymmetric)
+    if s.m + 1 > 0 and out.rowptr.size == 0:
+        out.rowptr = np.zeros(s.m + 1, np.int32)
+    return out
+
+
+def _consume_csr(s: SMatrix) -> CsrMatrix:
+    try:
+        return _csr_from_smatrix(s)
+    finally:
+        load().rtt_destroy_smatrix(C.byref(s))
+
+
+# ---- host utilities ----------------------------------------------------------
+
+def read_mtx(path: str, reference_values: bool = False) -> CsrMatrix:
+    s = SMatrix()
+    _check(load().rtt_read_mtx(path.encode(), int(reference_values), C.byref(s)), "read_mtx")
+    return _consume_csr(s)
+
+
+def write_mtx(path: str, a: CsrMatrix) -> None:
+    s = a.as_smatrix()
+    _check(load().rtt_write_mtx(path.encode(), C.byref(s)), "write_mtx")
+
+
+def read_order_file(path: str, expected: int) -> np.ndarray:
+    out = np.zeros(max(expected, 1), np.int32)
+    _check(load().rtt_read_order_file(path.encode(), expected, _ptr(out)), "read_order_file")
+    return out[:expected]
+
+
+def gen_laplace2d(g: int, pert: float = 0.1, seed: int = 1) -> CsrMatrix:
+    s = SMatrix()
+    _check(load().rtt_gen_laplace2d(g, pert, seed, C.byref(s)), "gen_laplace2d")
+    return _consume_csr(s)
+
+
+def gen_stencil27(g: int, lo: float = 0.5, hi: float = 1.5, seed: int = 1) -> CsrMatrix:
+    s = SMatrix()
+    _check(load().rtt_gen_stencil27(g, lo, hi, seed, C.byref(s)), "gen_stencil27")
+    return _consume_csr(s)
+
+
+def gen_rmat(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19,
+             c: float = 0.19, seed: int = 20260101) -> CsrMatrix:
+    s = SMatrix()
+    _check(load().rtt_gen_rmat(scale, edge_factor, a, b, c, seed, C.byref(s)), "gen_rmat")
+    return _consume_csr(s)
+
+
+def gen_random(m: int, n: int, per_row: int, seed: int = 1) -> CsrMatrix:
+    s = SMatrix()
+    _check(load().rtt_gen_random(m, n, per_row, seed, C.byref(s)), "gen_random")
+    return _consume_csr(s)
+
+
+# ---- tile matrices -------------------------------------------------------------
+
+@dataclass
+class TileC:
+    """Host tile-CSR of C (the SMatrix fields dmma_copy_output_to_host fills)."""
+    m: int
+    n: int
+    tilem: int
+    tilen: int
+    numtile: int
+    nnz: int
+    tile_ptr: np.ndarray
+    tile_columnidx: np.ndarray
+    tile_nnz: np.ndarray
+    tile_csr_Ptr: np.ndarray
+    tile_csr_Col: np.ndarray
+    tile_csr_Value: np.ndarray
+
+    def as_smatrix(self) -> SMatrix:
+        s = SMatrix()
+        s.m, s.n, s.nnz, s.tilem, s.tilen, s.numtile = (
+            self.m, self.n, self.nnz, self.tilem, self.tilen, self.numtile)
+        for name, ct in (("tile_ptr", C.c_int), ("tile_columnidx", C.c_int),
+                         ("tile_nnz", C.c_int), ("tile_csr_Ptr", C.c_ubyte),
+                         ("tile_csr_Col", C.c_ubyte), ("tile_csr_Value", C.c_double)):
+            arr = getattr(self, name)
+            setattr(s, name, arr.ctypes.data_as(C.POINTER(ct)))
+        return s
+
+
+def _tilec_from_smatrix(s: SMatrix) -> TileC:
+    return TileC(s.m, s.n, s.tilem, s.tilen, s.numtile, s.nnz,
+                 _np_copy(s.tile_ptr, s.tilem + 1, np.int32),
+                 _np_copy(s.tile_columnidx, s.numtile, np.int32),
+                 _np_copy(s.tile_nnz, s.numtile + 1, np.int32),
+                 _np_copy(s.tile_csr_Ptr, 8 * s.numtile, np.uint8),
+                 _np_copy(s.tile_csr_Col, s.nnz, np.uint8),
+                 _np_copy(s.tile_csr_Value, s.nnz, np.float64))
+
+
+def _download_tiles(view: DeviceTiles) -> dict:
+    T = view.num_tiles
+    out = {
+        "rows": view.rows, "cols": view.cols, "tile_rows": view.tile_rows,
+        "tile_cols": view.tile_cols, "tile_row_count": view.tile_row_count,
+        "tile_col_count": view.tile_col_count, "num_tiles": T,
+        "payload_size": view.payload_size, "dense_tiles": view.dense_tiles,
+        "sparse_tiles": view.sparse_tiles,
+        "tile_row_ptr": np.zeros(view.tile_row_count + 1, np.int32),
+        "tile_col_idx": np.zeros(max(T, 1), np.int32),
+        "value_offsets": np.zeros(T + 1, np.int32),
+        "masks": np.zeros(max(T, 1), np.uint32),
+        "values": np.zeros(max(view.payload_size, 1), np.float64),
+    }
+    has_csc = bool(view.tile_col_ptr)
+    if has_csc:
+        out["tile_col_ptr"] = np.zeros(view.tile_col_count + 1, np.int32)
+        out["tile_row_idx"] = np.zeros(max(T, 1), np.int32)
+        out["csc_tile_ids"] = np.zeros(max(T, 1), np.int32)
+    names = ["tile_row_ptr", "tile_col_idx", "value_offsets", "masks", "values",
+             "tile_col_ptr", "tile_row_idx", "csc_tile_ids"]
+    ptrs = [_ptr(out[n]) if n in out else None for n in names]
+    _check(load().rtt_download_tiles(C.byref(view), *ptrs), "download_tiles")
+    for n in ("tile_col_idx", "masks", "tile_row_idx", "csc_tile_ids"):
+        if n in out:
+            out[n] = out[n][:T]
+    out["values"] = out["values"][:view.payload_size]
+    return out
+
+
+def init(device: int = 0) -> None:
+    _check(load().rtt_init(device), "rtt_init")
+
+
+def sync() -> None:
+    _check(load().rtt_sync(), "rtt_sync")
+
+
+class PreparedA:
+    """Owns an rtt_prepared_a handle (A in 8x4 hybrid tiles on the GPU)."""
+
+    def __init__(self, handle: C.c_void_p, stats: OfflineAStats, csr: CsrMatrix):
+        self.handle = handle
+        self.stats = {f: getattr(stats, f) for f, _ in OfflineAStats._fields_}
+        self.m, self.n = csr.m, csr.n
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.rtt_destroy_prepared_a(self.handle)
+            self.handle = None
+
+    def tiles(self) -> dict:
+        v = DeviceTiles()
+        _check(load().rtt_prepared_a_view(self.handle, C.byref(v)), "prepared_a_view")
+        return _download_tiles(v)
+
+    def info(self) -> tuple[int, int, bool]:
+        ar, ai, ident = C.c_int(), C.c_int(), C.c_int()
+        _check(load().rtt_prepared_a_info(self.handle, C.byref(ar), C.byref(ai),
+                                          C.byref(ident)), "prepared_a_info")
+        return ar.value, ai.value, bool(ident.value)
+
+    def row_new_to_old(self):
+        p = load().rtt_prepared_a_row_new_to_old(self.handle)
+        return None if not p else _np_copy(p, self.m, np.int32)
+
+
+def prepare_identity_a(a: CsrMatrix, dense_threshold: int = 24) -> PreparedA:
+    s = a.as_smatrix()
+    h = C.c_void_p()
+    st = OfflineAStats()
+    _check(load().rtt_prepare_identity_a(C.byref(s), dense_threshold, C.byref(h),
+                                         C.byref(st)), "prepare_identity_a")
+    return PreparedA(h, st, a)
+
+
+def prepare_external_a(a: CsrMatrix, row_new_to_old, inner_new_to_old,
+                       dense_threshold: int = 24) -> PreparedA:
+    s = a.as_smatrix()
+    r = np.ascontiguousarray(row_new_to_old, dtype=np.int32)
+    i = np.ascontiguousarray(inner_new_to_old, dtype=np.int32)
+    h = C.c_void_p()
+    st = OfflineAStats()
+    _check(load().rtt_prepare_external_a(C.byref(s), _ptr(r), _ptr(i), dense_threshold,
+                                         C.byref(h), C.byref(st)), "prepare_external_a")
+    return PreparedA(h, st, a)
+
+
+class DynamicB:
+    """Owns an rtt_dynamic_b handle (B in 4x8 hybrid tiles + tile CSC)."""
+
+    def __init__(self):
+        self.handle = C.c_void_p()
+        self.last = {}
+
+    def __del__(self):
+        if getattr(self, "handle", None) and self.handle.value and _lib is not None:
+            _lib.rtt_destroy_dynamic_b(self.handle)
+            self.handle = C.c_void_p()
+
+    def tiles(self) -> dict:
+        v = DeviceTiles()
+        _check(load().rtt_dynamic_b_view(self.handle, C.byref(v)), "dynamic_b_view")
+        return _download_tiles(v)
+
+
+def rebuild_b(a: PreparedA, aat: int, b: DynamicB | None = None) -> DynamicB:
+    b = b or DynamicB()
+    st = BUpdateStats()
+    _check(load().rtt_rebuild_b(a.handle, int(aat), C.byref(b.handle), C.byref(st)),
+           "rebuild_b")
+    b.last = {f: getattr(st, f) for f, _ in BUpdateStats._fields_}
+    return b
+
+
+def nnz_cub(a: PreparedA, aat: int) -> int:
+    v = C.c_ulonglong()
+    ms = C.c_double()
+    _check(load().rtt_compute_nnz_cub(a.handle, int(aat), C.byref(v), C.byref(ms)), "nnz_cub")
+    return int(v.value)
+
+
+def schedule(lb_mode: str = "adaptive-tail", materialize_output: bool = True, **kw) -> Schedule:
+    s = Schedule()
+    load().rtt_schedule_defaults(C.byref(s))
+    s.lb_mode = LB_MODES[lb_mode]
+    s.materialize_output = int(materialize_output)
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+def tilespgemm(a: PreparedA, b: DynamicB, sched: Schedule | None = None):
+    """Runs C = A*B; returns (TileC or None, stats dict)."""
+    sched = sched or schedule()
+    out = SMatrix()
+    st = Stats()
+    _check(load().rtt_tilespgemm(a.handle, b.handle, C.byref(out), C.byref(st),
+                                 C.byref(sched)), "tilespgemm")
+    stats = {f: getattr(st, f) for f, _ in Stats._fields_}
+    tc = None
+    if sched.materialize_output:
+        try:
+            tc = _tilec_from_smatrix(out)
+        finally:
+            load().rtt_destroy_smatrix(C.byref(out))
+    return tc, stats
+
+
+def last_c(a: PreparedA, with_values: bool = True) -> dict:
+    """Downloads the 64-bit device C of the last tilespgemm on `a`."""
+    v = DeviceC()
+    _check(load().rtt_last_c_view(a.handle, C.byref(v)), "last_c_view")
+    T, Z, R = v.num_tiles, v.nnz, v.tilem
+    out = {"tilem": R, "tilen": v.tilen, "num_tiles": T, "nnz": Z}
+    items = [("tile_ptr", v.tile_ptr, R + 1, np.int64), ("tile_col", v.tile_col, T, np.int32),
+             ("tile_mask", v.tile_mask, T, np.uint64), ("tile_nnz", v.tile_nnz, T + 1, np.int64),
+             ("tile_csr_ptr", v.tile_csr_ptr, 8 * T, np.uint8),
+             ("tile_csr_col", v.tile_csr_col, Z, np.uint8)]
+    if with_values:
+        items.append(("tile_csr_val", v.tile_csr_val, Z, np.float64))
+    for name, p, count, dt in items:
+        arr = np.zeros(count, dt)
+        if count:
+            _check(load().rtt_memcpy_d2h(_ptr(arr), C.c_void_p(p), arr.nbytes), "memcpy_d2h")
+        out[name] = arr
+    return out
+
+
+def tile2csr(tc: TileC, canonical: bool = True) -> CsrMatrix:
+    s = tc.as_smatrix()
+    _check(load().rtt_tile2csr(C.byref(s), int(canonical)), "tile2csr")
+    try:
+        return CsrMatrix(s.m, s.n, _np_copy(s.rowpointer, s.m + 1, np.int32),
+                         _np_copy(s.columnindex, s.nnz, np.int32),
+                         _np_copy(s.value, s.nnz, np.float64))
+    finally:
+        _free_csr_part(s)
+
+
+def tile2csr_restore_rows(tc: TileC, row_new_to_old, permutation_size: int,
+                          canonical: bool = True) -> CsrMatrix:
+    s = tc.as_smatrix()
+    perm = np.ascontiguousarray(row_new_to_old, dtype=np.int32)
+    _check(load().rtt_tile2csr_restore_rows(C.byref(s), _ptr(perm), permutation_size,
+                                            int(canonical)), "tile2csr_restore_rows")
+    try:
+        return CsrMatrix(s.m, s.n, _np_copy(s.rowpointer, s.m + 1, np.int32),
+                         _np_copy(s.columnindex, s.nnz, np.int32),
+                         _np_copy(s.value, s.nnz, np.float64))
+    finally:
+        _free_csr_part(s)
+
+
+def _free_csr_part(s: SMatrix) -> None:
+    # Only the CSR arrays belong to the library here; the tile arrays are
+    # numpy-owned, so free the CSR part through a separate struct.
+    t = SMatrix()
+    t.rowpointer, t.columnindex, t.value = s.rowpointer, s.columnindex, s.value
+    load().rtt_destroy_smatrix(C.byref(t))
+
+
+def abi_symbols(header: str = HEADER_PATH) -> list[str]:
+    """Function names declared in the public header."""
+    import re
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(rtt_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))

@@ -1,0 +1,165 @@
+"""CPU tests of the shipped library's host side: the C ABI loads and exports
+every declared symbol, and the host functions (Matrix Market reader,
+tile2csr, order files, generators) agree with the oracle.  No GPU call."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import rtt_amd
+
+
+def test_library_exports_every_header_symbol():
+    lib = rtt_amd.load()
+    names = rtt_amd.abi_symbols()
+    assert len(names) >= 29
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", rtt_amd.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    assert set(names) <= exported
+
+
+def test_smatrix_layout_matches_header(tmp_path):
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stddef.h>\n#include <stdio.h>\n#include "rtt_spgemm.h"\n'
+        'int main(void){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(rtt_smatrix),'
+        ' offsetof(rtt_smatrix, value), offsetof(rtt_smatrix, tile_ptr),'
+        ' offsetof(rtt_smatrix, numtile), offsetof(rtt_smatrix, csc_tile_rowidx),'
+        ' sizeof(rtt_stats));return 0;}\n')
+    exe = tmp_path / "layout"
+    inc = os.path.dirname(rtt_amd.HEADER_PATH)
+    subprocess.run(["gcc", "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    S = rtt_amd.SMatrix
+    assert got == [C.sizeof(S), S.value.offset, S.tile_ptr.offset, S.numtile.offset,
+                   S.csc_tile_rowidx.offset, C.sizeof(rtt_amd.Stats)]
+
+
+def write(path, text):
+    with open(path, "w") as f:
+        f.write(text)
+
+
+@pytest.mark.parametrize("kind", ["general", "symmetric", "pattern"])
+def test_read_mtx_matches_oracle(tmp_path, kind):
+    rng = np.random.default_rng(4)
+    p = str(tmp_path / "a.mtx")
+    ent = []
+    for _ in range(200):
+        i, j = int(rng.integers(40)), int(rng.integers(40))
+        if kind == "symmetric" and j > i:
+            i, j = j, i
+        ent.append((i, j, rng.uniform(-1, 1)))
+    ent += ent[:7]
+    field = "pattern" if kind == "pattern" else "real"
+    sym = "symmetric" if kind == "symmetric" else "general"
+    lines = [f"%%MatrixMarket matrix coordinate {field} {sym}", "% c", f"40 40 {len(ent)}"]
+    lines += [f"{i + 1} {j + 1}" if field == "pattern" else f"{i + 1} {j + 1} {v!r}"
+              for i, j, v in ent]
+    write(p, "\n".join(lines) + "\n")
+    a = rtt_amd.read_mtx(p)
+    m, n, rp, ci, v = oracle.mmio_read(p)
+    assert (a.m, a.n) == (m, n)
+    assert np.array_equal(a.rowptr, rp) and np.array_equal(a.col, ci)
+    assert np.array_equal(a.val, v)
+    r = rtt_amd.read_mtx(p, reference_values=True)
+    assert np.array_equal(r.val, np.arange(r.nnz) % 10)
+
+
+def test_write_then_read_roundtrip(tmp_path):
+    a = rtt_amd.gen_laplace2d(16, 0.1, 3)
+    p = str(tmp_path / "l.mtx")
+    rtt_amd.write_mtx(p, a)
+    b = rtt_amd.read_mtx(p)
+    assert np.array_equal(a.rowptr, b.rowptr) and np.array_equal(a.col, b.col)
+    assert np.array_equal(a.val, b.val)
+
+
+def test_generators_closed_forms_and_determinism():
+    a = rtt_amd.gen_laplace2d(64, 0.1, 20260101)
+    assert a.nnz == 20224 and a.m == 4096
+    diag = a.val[np.repeat(np.arange(a.m), np.diff(a.rowptr)) == a.col]
+    assert np.all(diag == 4.0)
+    off = a.val[np.repeat(np.arange(a.m), np.diff(a.rowptr)) != a.col]
+    assert off.min() >= -1.1 and off.max() <= -0.9
+    s = rtt_amd.gen_stencil27(16, 0.5, 1.5, 1)
+    assert s.nnz == 46 ** 3 and s.val.min() >= 0.5 and s.val.max() < 1.5
+    for x in (a, s):
+        for i in range(x.m):
+            assert np.all(np.diff(x.col[x.rowptr[i]:x.rowptr[i + 1]]) > 0)
+    s2 = rtt_amd.gen_stencil27(16, 0.5, 1.5, 1)
+    assert np.array_equal(s.val, s2.val)
+    r = rtt_amd.gen_rmat(10, 16, seed=5)
+    r2 = rtt_amd.gen_rmat(10, 16, seed=5)
+    assert r.m == 1024 and 0 < r.nnz <= 16 * 1024
+    assert np.array_equal(r.col, r2.col) and np.array_equal(r.val, r2.val)
+    deg = np.diff(r.rowptr)
+    assert deg.max() > 8 * deg.mean()   # power-law skew
+
+
+def test_tile2csr_matches_oracle_including_malformed_bounds():
+    rng = np.random.default_rng(8)
+    m, n = 50, 61
+    tilem = (m + 7) // 8
+    tiles_per_row = rng.integers(0, 5, tilem)
+    tile_ptr = np.concatenate([[0], np.cumsum(tiles_per_row)]).astype(np.int32)
+    T = int(tile_ptr[-1])
+    cols, nnzs, ptrs, lcols = [], [0], [], []
+    for tr in range(tilem):
+        picks = np.sort(rng.choice((n + 7) // 8, size=tiles_per_row[tr], replace=False))
+        for J in picks:
+            cols.append(int(J))
+            k = int(rng.integers(0, 20))
+            nnzs.append(nnzs[-1] + k)
+            ptrs += list(np.sort(rng.integers(0, k + 3, 8)))   # may exceed the tile
+            lcols += list(rng.integers(0, 9, k))               # 8 is out of range
+    tc = rtt_amd.TileC(m, n, tilem, (n + 7) // 8, T, nnzs[-1], tile_ptr,
+                       np.array(cols, np.int32), np.array(nnzs, np.int32),
+                       np.array(ptrs, np.uint8), np.array(lcols, np.uint8),
+                       rng.uniform(-1, 1, nnzs[-1]))
+    for canonical in (True, False):
+        got = rtt_amd.tile2csr(tc, canonical)
+        want = oracle.tile2csr(m, n, tilem, tc.tile_ptr, tc.tile_columnidx, tc.tile_nnz,
+                               tc.tile_csr_Ptr, tc.tile_csr_Col, tc.tile_csr_Value,
+                               canonical=canonical)
+        assert np.array_equal(got.rowptr.astype(np.int64), want[0])
+        if canonical:
+            assert np.array_equal(got.col, want[1]) and np.array_equal(got.val, want[2])
+        else:
+            for i in range(m):
+                s, e = got.rowptr[i], got.rowptr[i + 1]
+                assert np.array_equal(got.col[s:e], want[1][s:e])
+    perm = rng.permutation(m).astype(np.int32)
+    got = rtt_amd.tile2csr_restore_rows(tc, perm, m)
+    want = oracle.tile2csr(m, n, tilem, tc.tile_ptr, tc.tile_columnidx, tc.tile_nnz,
+                           tc.tile_csr_Ptr, tc.tile_csr_Col, tc.tile_csr_Value,
+                           row_new_to_old=perm, permutation_size=m)
+    assert np.array_equal(got.rowptr.astype(np.int64), want[0])
+    assert np.array_equal(got.col, want[1])
+    bad = perm.copy()
+    bad[3] = bad[4]
+    with pytest.raises(rtt_amd.RttError):
+        rtt_amd.tile2csr_restore_rows(tc, bad, m)
+
+
+def test_order_file_parser(tmp_path):
+    p = str(tmp_path / "o.txt")
+    write(p, "2 0\n1 3\n")
+    assert list(rtt_amd.read_order_file(p, 4)) == [2, 0, 1, 3]
+    for text in ("2 0 1", "2 0 1 3 4", "2 0 1 1", "2 0 1 7", "2 0 x 1", "-1 0 1 2"):
+        write(p, text)
+        with pytest.raises(rtt_amd.RttError):
+            rtt_amd.read_order_file(p, 4)
+
+
+def test_gpu_entry_points_fail_loudly_without_init():
+    a = rtt_amd.gen_laplace2d(8, 0.1, 1)
+    with pytest.raises(rtt_amd.RttError):
+        rtt_amd.prepare_identity_a(a)
