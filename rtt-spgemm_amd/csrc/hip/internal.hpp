@@ -16,6 +16,15 @@ constexpr int kTileK = 4;   // A tile cols / B tile rows
 constexpr int kTileN = 8;   // B tile cols / C tile cols
 constexpr int kInputElems = 32;
 
+// Per-B-tile record read by the numeric kernel: the packed meta and the
+// first kRecVals payload values, so that one (A tile, B tile) pair costs one
+// contiguous 80-byte load.
+constexpr int kRecVals = 8;
+struct alignas(16) BRecord {
+    int4 meta;
+    double v[kRecVals];
+};
+
 // Checks a HIP call; on failure records the error and returns RTT_ERR_DEVICE.
 #define RTT_HIP(call)                                                            \
     do {                                                                         \
@@ -54,15 +63,29 @@ struct DevCsr {
     DevBuf row_ptr, col_idx, values;
 };
 
-// Hybrid tile matrix on the device (the DmmaDeviceTiles fields).
+// Hybrid tile matrix on the device (the DmmaDeviceTiles fields) plus the
+// packed per-tile record the SpGEMM kernels read: meta[t] = {tile column,
+// mask, per-k byte sets (byte k = stride-4 bits k, k+4, ..., k+28 of the
+// mask), payload offset | dense bit 31}.
 struct DevTiles {
     int rows = 0, cols = 0, tile_rows = 0, tile_cols = 0;
     int tile_row_count = 0, tile_col_count = 0;
     int64_t num_tiles = 0, payload = 0, dense = 0, sparse = 0;
-    bool has_csc = false;
+    bool has_csc = false;       // tile_col_ptr / tile_row_idx / csc_tile_ids valid
+    bool arrays_ready = true;   // tile_col_idx / masks / value_offsets / values valid
     DevBuf tile_row_ptr, tile_col_idx, value_offsets, masks, values;
     DevBuf tile_col_ptr, tile_row_idx, csc_tile_ids;
-    DevBuf tile_row_of;   // tile -> tile row (A only; used by the B transpose)
+    DevBuf tile_row_of;   // tile -> tile row (A: used by the B transpose)
+    DevBuf meta;          // int4 per tile
+    DevBuf records;       // BRecord per tile (B only)
+    // B built as A^T at tile level: meta offsets index A's payload, and
+    // sorted_ids[b] is the A tile behind B tile b.
+    const DevTiles *transpose_of = nullptr;
+    DevBuf sorted_ids;
+    const double *payload_values() const
+    {
+        return transpose_of ? transpose_of->values.as<double>() : values.as<double>();
+    }
     void view(rtt_device_tiles *v) const;
 };
 
@@ -76,9 +99,10 @@ struct DevC {
 
 // Scratch of the SpGEMM core, reused across calls.
 struct SpgemmWork {
-    DevBuf row_tiles, row_nnz, row_level, row_pairs;   // per A tile row
-    DevBuf scan_tmp, counters, l3_rows, num_rows, slab;
-    DevBuf c_row_nnz_off;                               // per tile row, int64
+    DevBuf row_tiles, row_nnz, row_level;   // per A tile row
+    DevBuf scan_tmp, counters, l3_rows, lds_rows, slab;
+    DevBuf stage_j, stage_m;                // staged sorted tile lists
+    DevBuf c_row_nnz_off;                   // per tile row, int64
 };
 
 // Scratch of the tile builders, reused across calls.
@@ -116,11 +140,17 @@ int build_tiles(const DevCsr &src, const int *row_map, int active_rows,
                 bool col_major, int dense_threshold, bool build_csc,
                 DevTiles *out, BuildWork *work);
 // B = A^T at tile level (A tile (J,K) row-major == B tile (K,J) col-major).
+// Builds B's tile row pointer and meta only; materialize_tiles() fills the
+// remaining DmmaDeviceTiles arrays when a caller asks for them.
 int transpose_tiles(const DevTiles &a, int b_cols, DevTiles *b, BuildWork *work);
+// Completes the standard arrays (and the tile CSC) of a B built lazily.
+int materialize_tiles(DevTiles *t, BuildWork *work);
 // Transposed CSR of `src` (stable: rows ascending within each column).
 int transpose_csr(const DevCsr &src, DevCsr *out, BuildWork *work);
 int nnz_cub(const DevCsr &csr, bool aat, unsigned long long *out, BuildWork *work);
 int fill_tile_row_of(DevTiles *t);
+// Fills t->records from t->meta and the payload the meta offsets index.
+int build_records(DevTiles *t);
 
 // ---- SpGEMM core (spgemm.hip) ---------------------------------------------
 struct CoreTimes {

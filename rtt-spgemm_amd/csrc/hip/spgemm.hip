@@ -1,21 +1,25 @@
 // SpGEMM core: symbolic -> allocate -> numeric for C = A * B with A in 8x4
 // row-major tiles, B in 4x8 column-major tiles and C in 8x8 tiles.
 //
-// Work unit: one A tile row I (8 scalar rows).  Its (A tile, B tile) pairs
-// are the products A(I,K) x B(K,J) for every A tile in row I and every B tile
-// in B's tile row K -- a tile-level Gustavson expansion, so no K-list merge
-// or candidate sort is needed (the reference's Super16 candidate/exact steps,
-// src/super16_symbolic.cuh:2406-2774, and its K-merge numeric,
-// src/dmma_spgemm.h:4307-4410, do the equivalent through merges).
+// Work unit: one A tile row I (8 scalar rows), one 256-thread workgroup.
+// Its (A tile, B tile) pairs are A(I,K) x B(K,J) for every A tile of row I
+// and every B tile of B's tile row K -- a tile-level Gustavson expansion, so
+// no K-list merge or candidate sort is needed (the reference reaches the same
+// pairs through Super16 candidates + exact intersection,
+// src/super16_symbolic.cuh:2406-2774, and a K-merge per C tile,
+// src/dmma_spgemm.h:4307-4410).
 //
-//  * symbolic: the 8x8 masks of all C tiles of row I are ORed into a hash
-//    table in LDS (one wave per row); rows whose tile set outgrows the table
-//    are redone by a workgroup over a direct-mapped mask slab in HBM.
-//  * numeric: the exact C layout of row I (sorted tile columns, masks,
-//    value offsets) is loaded into LDS; every scalar product is added into an
-//    LDS accumulator laid out exactly like the row's output segment, which is
-//    then streamed to HBM.  Rows whose segment does not fit go to a
-//    workgroup path that accumulates with FP64 atomics directly in HBM.
+//  * symbolic: the 8x8 masks of the row's C tiles are ORed into an LDS hash
+//    table, compacted, bitonic-sorted by tile column and staged; after the
+//    size scan a light pass writes the final structure.  Rows whose tile set
+//    outgrows the table use a direct-mapped mask slab in HBM instead.
+//  * numeric: the row's exact C layout (tile columns, masks, value offsets)
+//    and its A payload are loaded into LDS.  Each thread takes pairs in
+//    batches, loading every pair's 80-byte B record (packed meta + first
+//    eight values) before using any, and adds each scalar product into an LDS
+//    FP64 accumulator laid out exactly like the row's output segment, which
+//    is finally streamed to HBM.  Rows that exceed the LDS budgets fall back
+//    to a workgroup path with FP64 atomics in HBM.
 #include <climits>
 
 #include "device.hpp"
@@ -28,30 +32,32 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / kWave;
 
-constexpr int kSymSlots = 512;     // LDS hash slots per wave (power of 2)
-constexpr int kSymSlotBits = 9;
-constexpr int kSymLimit = 384;     // distinct C tiles before the slab path
-constexpr int kSymPerLane = kSymLimit / kWave;   // 6
+constexpr int kSymSlots = 1024;    // LDS hash slots (power of 2)
+constexpr int kSymSlotBits = 10;
+constexpr int kSymPerThread = kSymSlots / kThreads;
+constexpr int kSymBatch = 4;       // pairs whose loads a thread issues together
 
-constexpr int kNumSlots = 512;     // numeric J -> local tile index hash
-constexpr int kNumSlotBits = 9;
-constexpr int kNumAcc = 1024;      // FP64 accumulator slots per wave
+constexpr int kNumSlots = 1024;    // numeric J -> local tile hash
+constexpr int kNumSlotBits = 10;
+constexpr int kNumTiles = 512;     // C tiles per row on the LDS path
+constexpr int kChunk = 128;        // A tiles loaded per step
+constexpr int kNumBatch = 2;       // pairs whose loads a thread issues together
+constexpr int kAccSmall = 1024, kAStageSmall = 256;   // LDS budgets, common rows
+constexpr int kAccLarge = 4096, kAStageLarge = 2048;  // LDS budgets, large rows
 
 constexpr int kEmpty = -1;
+constexpr uint32_t kDenseBit = 0x80000000u;
 
-enum RowLevel : int { kLevelEmpty = 0, kLevelWave = 1, kLevelSlab = 3 };
+enum RowLevel : int { kLevelEmpty = 0, kLevelLds = 1, kLevelSlab = 3 };
 
 struct Operands {
     int a_tile_rows;
     const int *__restrict__ a_trp;
-    const int *__restrict__ a_tci;
-    const uint32_t *__restrict__ a_masks;
+    const int4 *__restrict__ a_meta;
     const int *__restrict__ a_off;
     const double *__restrict__ a_vals;
     const int *__restrict__ b_trp;
-    const int *__restrict__ b_tci;
-    const uint32_t *__restrict__ b_masks;
-    const int *__restrict__ b_off;
+    const int4 *__restrict__ b_meta;
     const double *__restrict__ b_vals;
     int b_tile_cols;
 };
@@ -67,13 +73,40 @@ struct COut {
     double *__restrict__ val;
 };
 
+struct Stage {
+    int *__restrict__ j;
+    unsigned long long *__restrict__ m;
+    int stride;
+};
+
 __device__ __forceinline__ uint32_t slot_of(int j, int bits)
 {
     return hash_tile_col(j) >> (32 - bits);
 }
 
-// Eight local row offsets of an 8x8 tile mask packed into one 64-bit word
-// (byte r = number of entries in rows < r).
+__device__ __forceinline__ uint32_t byte_of(uint32_t v, int k) { return (v >> (8 * k)) & 0xFFu; }
+
+// Structural product from the per-k row/column byte sets of the A tile
+// (rows present in column k) and the B tile (columns present in row k).
+__device__ __forceinline__ uint64_t bytes_product(uint32_t a_bytes, uint32_t b_bytes)
+{
+    uint64_t out = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        out |= rows_to_bytes(byte_of(a_bytes, k)) &
+               (static_cast<uint64_t>(byte_of(b_bytes, k)) * 0x0101010101010101ull);
+    return out;
+}
+
+__device__ __forceinline__ int product_count(uint32_t a_bytes, uint32_t b_bytes)
+{
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        n += __popc(byte_of(a_bytes, k)) * __popc(byte_of(b_bytes, k));
+    return n;
+}
+
 __device__ __forceinline__ uint64_t packed_row_offsets(uint64_t m)
 {
     uint64_t out = 0;
@@ -83,8 +116,6 @@ __device__ __forceinline__ uint64_t packed_row_offsets(uint64_t m)
     return out;
 }
 
-// Writes the structural part of one C tile: column, mask, value offset, row
-// offsets and the local column of every entry.
 __device__ __forceinline__ void write_tile_structure(const COut &c, int64_t t, int j,
                                                      uint64_t m, int64_t off)
 {
@@ -100,190 +131,7 @@ __device__ __forceinline__ void write_tile_structure(const COut &c, int64_t t, i
     }
 }
 
-// Loads up to 64 A tiles of one row into LDS with their B-row extents and an
-// exclusive prefix of pair counts; returns the chunk's pair count.
-__device__ __forceinline__ int load_a_chunk(const Operands &g, int c0, int n, uint32_t *s_am,
-                                            int *s_b0, int *s_pre)
-{
-    const int lane = lane_id();
-    int nb = 0;
-    if (lane < n) {
-        const int a = c0 + lane;
-        const int k = g.a_tci[a];
-        const int b0 = g.b_trp[k];
-        nb = g.b_trp[k + 1] - b0;
-        s_am[lane] = g.a_masks[a];
-        s_b0[lane] = b0;
-    }
-    const int incl = wave_inclusive_scan(nb);
-    if (lane < n)
-        s_pre[lane] = incl - nb;
-    return __shfl(incl, kWave - 1, kWave);
-}
-
-// ---------------------------------------------------------------------------
-// Symbolic, one wave per A tile row, LDS hash of (tile column -> 8x8 mask).
-// ---------------------------------------------------------------------------
-template <bool FILL>
-__global__ __launch_bounds__(kThreads) void symbolic_wave_kernel(
-    Operands g, int64_t *__restrict__ row_tiles, int64_t *__restrict__ row_nnz,
-    int *__restrict__ row_level, unsigned long long *__restrict__ pair_counter, COut c)
-{
-    __shared__ int s_key[kWavesPerBlock][kSymSlots];
-    __shared__ unsigned long long s_msk[kWavesPerBlock][kSymSlots];
-    __shared__ uint32_t s_am[kWavesPerBlock][kWave];
-    __shared__ int s_b0[kWavesPerBlock][kWave];
-    __shared__ int s_pre[kWavesPerBlock][kWave];
-    __shared__ int s_cnt[kWavesPerBlock];
-    __shared__ int s_ovf[kWavesPerBlock];
-
-    const int w = threadIdx.x / kWave;
-    const int lane = lane_id();
-    const int row = blockIdx.x * kWavesPerBlock + w;
-    if (row >= g.a_tile_rows)
-        return;
-    if (FILL && row_level[row] != kLevelWave)
-        return;
-    int *key = s_key[w];
-    unsigned long long *msk = s_msk[w];
-    for (int i = lane; i < kSymSlots; i += kWave) {
-        key[i] = kEmpty;
-        msk[i] = 0;
-    }
-    if (lane == 0) {
-        s_cnt[w] = 0;
-        s_ovf[w] = 0;
-    }
-    wave_sync();
-
-    const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
-    unsigned long long pairs = 0;
-    for (int c0 = a0; c0 < a1; c0 += kWave) {
-        const int n = min(kWave, a1 - c0);
-        const int total = load_a_chunk(g, c0, n, s_am[w], s_b0[w], s_pre[w]);
-        wave_sync();
-        for (int p = lane; p < total; p += kWave) {
-            const int ai = upper_index(s_pre[w], n, p);
-            const int b = s_b0[w][ai] + (p - s_pre[w][ai]);
-            const int j = g.b_tci[b];
-            const uint64_t pm = tile_bool_product(s_am[w][ai], g.b_masks[b]);
-            if (!pm)
-                continue;
-            uint32_t h = slot_of(j, kSymSlotBits);
-            for (int probe = 0; probe < kSymSlots; ++probe) {
-                int k = key[h];
-                if (k == kEmpty) {
-                    k = atomicCAS(&key[h], kEmpty, j);
-                    if (k == kEmpty) {
-                        atomicOr(&msk[h], static_cast<unsigned long long>(pm));
-                        if (atomicAdd(&s_cnt[w], 1) >= kSymLimit)
-                            s_ovf[w] = 1;
-                        break;
-                    }
-                }
-                if (k == j) {
-                    atomicOr(&msk[h], static_cast<unsigned long long>(pm));
-                    break;
-                }
-                h = (h + 1) & (kSymSlots - 1);
-                if (probe == kSymSlots - 1)
-                    s_ovf[w] = 1;
-            }
-        }
-        pairs += total;
-        wave_sync();
-        if (s_ovf[w])
-            break;
-    }
-    wave_sync();
-    const bool overflow = s_ovf[w] != 0;
-    const int nt = s_cnt[w];
-
-    if constexpr (!FILL) {
-        if (overflow) {
-            if (lane == 0) {
-                row_level[row] = kLevelSlab;
-                row_tiles[row] = 0;
-                row_nnz[row] = 0;
-            }
-            return;
-        }
-        long long nz = 0;
-        for (int i = lane; i < kSymSlots; i += kWave)
-            nz += __popcll(msk[i]);
-        nz = wave_sum(nz);
-        if (lane == 0) {
-            row_level[row] = nt > 0 ? kLevelWave : kLevelEmpty;
-            row_tiles[row] = nt;
-            row_nnz[row] = nz;
-            atomicAdd(pair_counter, pairs);
-        }
-    } else {
-        // Compact the occupied slots (slot order), then rank by tile column.
-        int rk[kSymSlots / kWave];
-        unsigned long long rm[kSymSlots / kWave];
-#pragma unroll
-        for (int i = 0; i < kSymSlots / kWave; ++i) {
-            rk[i] = key[lane + i * kWave];
-            rm[i] = msk[lane + i * kWave];
-        }
-        wave_sync();
-        const uint64_t lt = (1ull << lane) - 1ull;
-        int base = 0;
-#pragma unroll
-        for (int i = 0; i < kSymSlots / kWave; ++i) {
-            const bool occ = rk[i] != kEmpty;
-            const uint64_t bal = __ballot(occ);
-            if (occ) {
-                const int pos = base + __popcll(bal & lt);
-                key[pos] = rk[i];
-                msk[pos] = rm[i];
-            }
-            base += __popcll(bal);
-        }
-        wave_sync();
-        int ej[kSymPerLane], er[kSymPerLane];
-        unsigned long long em[kSymPerLane];
-#pragma unroll
-        for (int i = 0; i < kSymPerLane; ++i) {
-            const int e = lane + i * kWave;
-            ej[i] = e < nt ? key[e] : INT_MAX;
-            em[i] = e < nt ? msk[e] : 0ull;
-            er[i] = 0;
-        }
-        for (int f = 0; f < nt; ++f) {
-            const int jf = key[f];
-#pragma unroll
-            for (int i = 0; i < kSymPerLane; ++i)
-                er[i] += jf < ej[i] ? 1 : 0;
-        }
-        wave_sync();
-#pragma unroll
-        for (int i = 0; i < kSymPerLane; ++i)
-            if (lane + i * kWave < nt) {
-                key[er[i]] = ej[i];
-                msk[er[i]] = em[i];
-            }
-        wave_sync();
-        const int64_t tb = c.tile_ptr[row];
-        const int64_t zb = c.row_nnz_off[row];
-        int64_t carry = 0;
-        for (int q0 = 0; q0 < nt; q0 += kWave) {
-            const int q = q0 + lane;
-            const uint64_t m = q < nt ? msk[q] : 0ull;
-            const int pc = __popcll(m);
-            const int incl = wave_inclusive_scan(pc);
-            if (q < nt)
-                write_tile_structure(c, tb + q, key[q], m, zb + carry + (incl - pc));
-            carry += __shfl(incl, kWave - 1, kWave);
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Symbolic for wide rows: one workgroup per row, direct-mapped mask slab in
-// HBM (one 64-bit mask per C tile column), read back in column order.
-// ---------------------------------------------------------------------------
+// Exclusive scan over the workgroup; *total receives the sum.
 __device__ __forceinline__ int block_exclusive_scan(int v, int *s_tmp, int *total)
 {
     const int w = threadIdx.x / kWave;
@@ -303,6 +151,220 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *s_tmp, int *tota
     return before + incl - v;
 }
 
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *s_tmp)
+{
+    v = wave_sum(v);
+    if (lane_id() == 0)
+        s_tmp[threadIdx.x / kWave] = v;
+    __syncthreads();
+    T all = 0;
+#pragma unroll
+    for (int i = 0; i < kWavesPerBlock; ++i)
+        all += s_tmp[i];
+    __syncthreads();
+    return all;
+}
+
+// Loads up to 256 A tiles of a row (their column-byte sets and B-row
+// extents) and an exclusive prefix of their pair counts.
+__device__ __forceinline__ int load_a_chunk(const Operands &g, int c0, int n, uint32_t *s_ab,
+                                            int *s_b0, int *s_pre, int *s_tmp)
+{
+    const int t = threadIdx.x;
+    int nb = 0;
+    if (t < n) {
+        const int4 m = g.a_meta[c0 + t];
+        const int b0 = g.b_trp[m.x];
+        nb = g.b_trp[m.x + 1] - b0;
+        s_ab[t] = static_cast<uint32_t>(m.z);
+        s_b0[t] = b0;
+    }
+    int total = 0;
+    const int excl = block_exclusive_scan(nb, s_tmp, &total);
+    if (t < n)
+        s_pre[t] = excl;
+    __syncthreads();
+    return total;
+}
+
+// ---------------------------------------------------------------------------
+// Symbolic: per-row hash of C tile masks, compaction, bitonic sort, staging.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
+    Operands g, int stage_limit, Stage st, int64_t *__restrict__ row_tiles,
+    int64_t *__restrict__ row_nnz, int *__restrict__ row_level,
+    unsigned long long *__restrict__ pair_counter)
+{
+    __shared__ int s_key[kSymSlots];
+    __shared__ unsigned long long s_msk[kSymSlots];
+    __shared__ uint32_t s_ab[kThreads];
+    __shared__ int s_b0[kThreads];
+    __shared__ int s_pre[kThreads];
+    __shared__ int s_tmp[kWavesPerBlock];
+    __shared__ long long s_tmp64[kWavesPerBlock];
+    __shared__ int s_cnt, s_ovf;
+
+    const int row = blockIdx.x;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kSymSlots; i += kThreads) {
+        s_key[i] = kEmpty;
+        s_msk[i] = 0ull;
+    }
+    if (tid == 0) {
+        s_cnt = 0;
+        s_ovf = 0;
+    }
+    __syncthreads();
+    const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
+    long long pairs = 0;
+    for (int c0 = a0; c0 < a1; c0 += kThreads) {
+        const int n = min(kThreads, a1 - c0);
+        const int total = load_a_chunk(g, c0, n, s_ab, s_b0, s_pre, s_tmp);
+        // Each thread takes kSymBatch pairs at a time and issues their
+        // B-record loads together before using any of them.
+        for (int p0 = tid; p0 < total; p0 += kSymBatch * kThreads) {
+            int4 bm[kSymBatch];
+            int ai[kSymBatch];
+#pragma unroll
+            for (int u = 0; u < kSymBatch; ++u) {
+                const int p = p0 + u * kThreads;
+                ai[u] = 0;
+                bm[u] = make_int4(0, 0, 0, 0);
+                if (p < total) {
+                    ai[u] = upper_index(s_pre, n, p);
+                    bm[u] = g.b_meta[s_b0[ai[u]] + (p - s_pre[ai[u]])];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kSymBatch; ++u) {
+                const uint64_t pm = bytes_product(s_ab[ai[u]], static_cast<uint32_t>(bm[u].z));
+                if (!pm)
+                    continue;
+                const int j = bm[u].x;
+                uint32_t h = slot_of(j, kSymSlotBits);
+                for (int probe = 0; probe < kSymSlots; ++probe) {
+                    int k = s_key[h];
+                    if (k == kEmpty) {
+                        k = atomicCAS(&s_key[h], kEmpty, j);
+                        if (k == kEmpty) {
+                            atomicOr(&s_msk[h], static_cast<unsigned long long>(pm));
+                            if (atomicAdd(&s_cnt, 1) >= stage_limit)
+                                s_ovf = 1;
+                            break;
+                        }
+                    }
+                    if (k == j) {
+                        atomicOr(&s_msk[h], static_cast<unsigned long long>(pm));
+                        break;
+                    }
+                    h = (h + 1) & (kSymSlots - 1);
+                    if (probe == kSymSlots - 1)
+                        s_ovf = 1;
+                }
+            }
+        }
+        pairs += total;
+        __syncthreads();
+        if (s_ovf)
+            break;
+    }
+    if (s_ovf) {
+        if (tid == 0) {
+            row_level[row] = kLevelSlab;
+            row_tiles[row] = 0;
+            row_nnz[row] = 0;
+        }
+        return;
+    }
+    const int nt = s_cnt;
+    // Compact occupied slots to the front.
+    int rk[kSymPerThread];
+    unsigned long long rm[kSymPerThread];
+    int mine = 0;
+#pragma unroll
+    for (int i = 0; i < kSymPerThread; ++i) {
+        rk[i] = s_key[tid * kSymPerThread + i];
+        rm[i] = s_msk[tid * kSymPerThread + i];
+        mine += rk[i] != kEmpty ? 1 : 0;
+    }
+    int dummy = 0;
+    int pos = block_exclusive_scan(mine, s_tmp, &dummy);
+    long long nz = 0;
+#pragma unroll
+    for (int i = 0; i < kSymPerThread; ++i)
+        if (rk[i] != kEmpty) {
+            s_key[pos] = rk[i];
+            s_msk[pos] = rm[i];
+            nz += __popcll(rm[i]);
+            ++pos;
+        }
+    int npow = 1;
+    while (npow < nt)
+        npow <<= 1;
+    for (int i = nt + tid; i < npow; i += kThreads)
+        s_key[i] = INT_MAX;
+    __syncthreads();
+    // Bitonic sort of the nt (padded to npow) entries by tile column.
+    for (int k = 2; k <= npow; k <<= 1) {
+        for (int jj = k >> 1; jj > 0; jj >>= 1) {
+            for (int i = tid; i < npow; i += kThreads) {
+                const int ixj = i ^ jj;
+                if (ixj > i) {
+                    const int ki = s_key[i], kx = s_key[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((ki > kx) == up) {
+                        s_key[i] = kx;
+                        s_key[ixj] = ki;
+                        const unsigned long long t = s_msk[i];
+                        s_msk[i] = s_msk[ixj];
+                        s_msk[ixj] = t;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const int64_t sb = static_cast<int64_t>(row) * st.stride;
+    for (int i = tid; i < nt; i += kThreads) {
+        st.j[sb + i] = s_key[i];
+        st.m[sb + i] = s_msk[i];
+    }
+    const long long row_nz = block_sum(nz, s_tmp64);
+    if (tid == 0) {
+        row_level[row] = nt > 0 ? kLevelLds : kLevelEmpty;
+        row_tiles[row] = nt;
+        row_nnz[row] = row_nz;
+        atomicAdd(pair_counter, static_cast<unsigned long long>(pairs));   // same on all threads
+    }
+}
+
+// Staged (column, mask) lists -> final C structure; one wave per row.
+__global__ __launch_bounds__(kThreads) void symbolic_write_kernel(
+    int rows, Stage st, const int *__restrict__ row_level, COut c)
+{
+    const int row = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    const int lane = lane_id();
+    if (row >= rows || row_level[row] != kLevelLds)
+        return;
+    const int64_t tb = c.tile_ptr[row];
+    const int nt = static_cast<int>(c.tile_ptr[row + 1] - tb);
+    const int64_t zb = c.row_nnz_off[row];
+    const int64_t sb = static_cast<int64_t>(row) * st.stride;
+    int64_t carry = 0;
+    for (int q0 = 0; q0 < nt; q0 += kWave) {
+        const int q = q0 + lane;
+        const uint64_t m = q < nt ? st.m[sb + q] : 0ull;
+        const int pc = __popcll(m);
+        const int incl = wave_inclusive_scan(pc);
+        if (q < nt)
+            write_tile_structure(c, tb + q, st.j[sb + q], m, zb + carry + (incl - pc));
+        carry += __shfl(incl, kWave - 1, kWave);
+    }
+}
+
+// Rows whose tile set outgrows the LDS table: one workgroup per row, one
+// 64-bit mask per C tile column in an HBM slab, read back in column order.
 __device__ __forceinline__ unsigned long long load_coherent(const unsigned long long *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -314,7 +376,7 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
     int64_t *__restrict__ row_tiles, int64_t *__restrict__ row_nnz,
     unsigned long long *__restrict__ pair_counter, COut c)
 {
-    __shared__ uint32_t s_am[kThreads];
+    __shared__ uint32_t s_ab[kThreads];
     __shared__ int s_b0[kThreads];
     __shared__ int s_pre[kThreads];
     __shared__ int s_tmp[kWavesPerBlock];
@@ -332,30 +394,21 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
         unsigned long long pairs = 0;
         for (int c0 = a0; c0 < a1; c0 += kThreads) {
             const int n = min(kThreads, a1 - c0);
-            int nb = 0;
             if (tid < n) {
-                const int a = c0 + tid;
-                const int k = g.a_tci[a];
-                const int b0 = g.b_trp[k];
-                nb = g.b_trp[k + 1] - b0;
-                s_am[tid] = g.a_masks[a];
-                s_b0[tid] = b0;
-                if (nb > 0) {
-                    atomicMin(&s_lo, g.b_tci[b0]);
-                    atomicMax(&s_hi, g.b_tci[b0 + nb - 1]);
+                const int4 m = g.a_meta[c0 + tid];
+                const int b0 = g.b_trp[m.x], b1 = g.b_trp[m.x + 1];
+                if (b1 > b0) {
+                    atomicMin(&s_lo, g.b_meta[b0].x);
+                    atomicMax(&s_hi, g.b_meta[b1 - 1].x);
                 }
             }
-            int total = 0;
-            const int excl = block_exclusive_scan(nb, s_tmp, &total);
-            if (tid < n)
-                s_pre[tid] = excl;
-            __syncthreads();
+            const int total = load_a_chunk(g, c0, n, s_ab, s_b0, s_pre, s_tmp);
             for (int p = tid; p < total; p += kThreads) {
                 const int ai = upper_index(s_pre, n, p);
-                const int b = s_b0[ai] + (p - s_pre[ai]);
-                const uint64_t pm = tile_bool_product(s_am[ai], g.b_masks[b]);
+                const int4 bm = g.b_meta[s_b0[ai] + (p - s_pre[ai])];
+                const uint64_t pm = bytes_product(s_ab[ai], static_cast<uint32_t>(bm.z));
                 if (pm)
-                    atomicOr(&slab[g.b_tci[b]], static_cast<unsigned long long>(pm));
+                    atomicOr(&slab[bm.x], static_cast<unsigned long long>(pm));
             }
             pairs += total;
             __syncthreads();
@@ -393,10 +446,8 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
             if (tid == 0) {
                 row_tiles[row] = tiles;
                 row_nnz[row] = nz;
-            }
-            pairs = tid == 0 ? pairs : 0;
-            if (tid == 0)
                 atomicAdd(pair_counter, pairs);
+            }
         }
         __threadfence();
         __syncthreads();
@@ -404,119 +455,189 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Numeric, one wave per A tile row with an LDS accumulator.
+// Numeric, LDS path: one workgroup per row, LDS accumulator.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void numeric_wave_kernel(
-    Operands g, const int *__restrict__ row_level, COut c)
+// Adds the products of one (A tile, B tile) pair into the row accumulator.
+// B's nonzeros are walked in payload order (the first kRecVals come from the
+// record in registers); for each, the rows of A's matching column.
+__device__ __forceinline__ void accumulate_pair(
+    const Operands &g, double *s_acc, const double *s_aval, uint32_t ab, uint32_t am, uint32_t ao,
+    const int4 &meta, const double (&v)[kRecVals], uint64_t cm, int cb)
 {
-    __shared__ int s_hk[kWavesPerBlock][kNumSlots];
-    __shared__ int s_hq[kWavesPerBlock][kNumSlots];
-    __shared__ double s_acc[kWavesPerBlock][kNumAcc];
-    __shared__ uint32_t s_am[kWavesPerBlock][kWave];
-    __shared__ int s_b0[kWavesPerBlock][kWave];
-    __shared__ int s_pre[kWavesPerBlock][kWave];
-    __shared__ int s_ao[kWavesPerBlock][kWave];
+    const bool adense = (ao & kDenseBit) != 0;
+    const uint32_t aoff = ao & ~kDenseBit;
+    const uint32_t bmask = static_cast<uint32_t>(meta.y);
+    const uint32_t bw = static_cast<uint32_t>(meta.w);
+    const bool bdense = (bw & kDenseBit) != 0;
+    const uint32_t boff = bw & ~kDenseBit;
+    uint32_t mm = bmask;
+    auto column = [&](int pos, double vb) {
+        const int k = pos & 3, cl = pos >> 2;
+        uint32_t rws = byte_of(ab, k);
+        while (rws) {
+            const int r = __builtin_ctz(rws);
+            rws &= rws - 1;
+            const int pa = r * 4 + k;
+            const double va = s_aval[aoff + (adense ? pa : __popc(am & ((1u << pa) - 1u)))];
+            const int pos_c = r * 8 + cl;
+            atomicAdd(&s_acc[cb + __popcll(cm & ((1ull << pos_c) - 1ull))], va * vb);
+        }
+    };
+    if (!bdense) {
+#pragma unroll
+        for (int j = 0; j < kRecVals; ++j) {
+            if (mm) {
+                const int pos = __builtin_ctz(mm);
+                mm &= mm - 1;
+                column(pos, v[j]);
+            }
+        }
+        int j = kRecVals;
+        while (mm) {
+            const int pos = __builtin_ctz(mm);
+            mm &= mm - 1;
+            column(pos, g.b_vals[boff + j]);
+            ++j;
+        }
+    } else {
+        while (mm) {
+            const int pos = __builtin_ctz(mm);
+            mm &= mm - 1;
+            column(pos, g.b_vals[boff + pos]);
+        }
+    }
+}
 
-    const int w = threadIdx.x / kWave;
-    const int lane = lane_id();
-    const int row = blockIdx.x * kWavesPerBlock + w;
-    if (row >= g.a_tile_rows || row_level[row] != kLevelWave)
+// A rows must fit the LDS budgets (tiles, accumulator, staged A values); the
+// instance launched over every row skips the others, which the host routes
+// to the larger instance or to the slab path.
+template <int ACC, int AST>
+__global__ __launch_bounds__(kThreads) void numeric_row_kernel(
+    Operands g, const BRecord *__restrict__ recs, const int *__restrict__ rows, int nrows,
+    const int *__restrict__ row_level, COut c)
+{
+    __shared__ double s_acc[ACC];
+    __shared__ double s_aval[AST];
+    __shared__ unsigned long long s_cmask[kNumTiles];
+    __shared__ unsigned short s_cbase[kNumTiles];
+    __shared__ int s_hk[kNumSlots];
+    __shared__ unsigned short s_hq[kNumSlots];
+    __shared__ uint32_t s_ab[kChunk];
+    __shared__ uint32_t s_am[kChunk];
+    __shared__ uint32_t s_ao[kChunk];
+    __shared__ int s_b0[kChunk];
+    __shared__ int s_pre[kChunk];
+    __shared__ int s_tmp[kWavesPerBlock];
+
+    const int tid = threadIdx.x;
+    const int row = rows ? rows[blockIdx.x] : static_cast<int>(blockIdx.x);
+    if (row_level[row] != kLevelLds)
         return;
     const int64_t t0 = c.tile_ptr[row];
     const int nt = static_cast<int>(c.tile_ptr[row + 1] - t0);
     const int64_t zb = c.row_nnz_off[row];
-    const int nnz_row = static_cast<int>(c.row_nnz_off[row + 1] - zb);
-    if (nt > kSymLimit || nnz_row > kNumAcc)
-        return;   // the workgroup path takes this row
-    int *hk = s_hk[w];
-    int *hq = s_hq[w];
-    double *acc = s_acc[w];
-    for (int i = lane; i < kNumSlots; i += kWave)
-        hk[i] = kEmpty;
-    for (int i = lane; i < nnz_row; i += kWave)
-        acc[i] = 0.0;
-    wave_sync();
-    for (int q = lane; q < nt; q += kWave) {
-        const int j = c.col[t0 + q];
-        uint32_t h = slot_of(j, kNumSlotBits);
-        while (atomicCAS(&hk[h], kEmpty, j) != kEmpty)
-            h = (h + 1) & (kNumSlots - 1);
-        hq[h] = q;
-    }
-    wave_sync();
-
+    const int64_t nnz_row = c.row_nnz_off[row + 1] - zb;
     const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
-    for (int c0 = a0; c0 < a1; c0 += kWave) {
-        const int n = min(kWave, a1 - c0);
-        const int total = load_a_chunk(g, c0, n, s_am[w], s_b0[w], s_pre[w]);
-        if (lane < n)
-            s_ao[w][lane] = c0 + lane;
-        wave_sync();
-        for (int p = lane; p < total; p += kWave) {
-            const int ai = upper_index(s_pre[w], n, p);
-            const int b = s_b0[w][ai] + (p - s_pre[w][ai]);
-            const uint32_t am = s_am[w][ai];
-            const uint32_t bm = g.b_masks[b];
-            const int j = g.b_tci[b];
-            uint32_t h = slot_of(j, kNumSlotBits);
-            int q = -1;
-            for (;;) {
-                const int k = hk[h];
-                if (k == j) {
-                    q = hq[h];
-                    break;
-                }
-                if (k == kEmpty)
-                    break;
-                h = (h + 1) & (kNumSlots - 1);
-            }
-            if (q < 0)
-                continue;   // structurally empty product
-            const int64_t tq = t0 + q;
-            const uint64_t cm = c.mask[tq];
-            const int cb = static_cast<int>(c.tile_nnz[tq] - zb);
-            const int at = s_ao[w][ai];
-            const int aoff = g.a_off[at];
-            const int aspan = g.a_off[at + 1] - aoff;
-            const int boff = g.b_off[b];
-            const int bspan = g.b_off[b + 1] - boff;
+    const int a_base = g.a_off[a0];
+    const int a_len = g.a_off[a1] - a_base;
+    if (nt > kNumTiles || nnz_row > ACC || a_len > AST)
+        return;
+    for (int i = tid; i < kNumSlots; i += kThreads)
+        s_hk[i] = kEmpty;
+    for (int i = tid; i < nnz_row; i += kThreads)
+        s_acc[i] = 0.0;
+    for (int i = tid; i < a_len; i += kThreads)
+        s_aval[i] = g.a_vals[a_base + i];
+    __syncthreads();
+    for (int q = tid; q < nt; q += kThreads) {
+        const int j = c.col[t0 + q];
+        s_cmask[q] = c.mask[t0 + q];
+        s_cbase[q] = static_cast<unsigned short>(c.tile_nnz[t0 + q] - zb);
+        uint32_t h = slot_of(j, kNumSlotBits);
+        while (atomicCAS(&s_hk[h], kEmpty, j) != kEmpty)
+            h = (h + 1) & (kNumSlots - 1);
+        s_hq[h] = static_cast<unsigned short>(q);
+    }
+    for (int c0 = a0; c0 < a1; c0 += kChunk) {
+        const int n = min(kChunk, a1 - c0);
+        int nb = 0;
+        if (tid < n) {
+            const int4 m = g.a_meta[c0 + tid];
+            const int b0 = g.b_trp[m.x];
+            nb = g.b_trp[m.x + 1] - b0;
+            s_ab[tid] = static_cast<uint32_t>(m.z);
+            s_am[tid] = static_cast<uint32_t>(m.y);
+            const uint32_t off = static_cast<uint32_t>(m.w);
+            s_ao[tid] = ((off & ~kDenseBit) - static_cast<uint32_t>(a_base)) | (off & kDenseBit);
+            s_b0[tid] = b0;
+        }
+        int total = 0;
+        const int excl = block_exclusive_scan(nb, s_tmp, &total);
+        if (tid < n)
+            s_pre[tid] = excl;
+        __syncthreads();
+        for (int p0 = tid; p0 < total; p0 += kNumBatch * kThreads) {
+            int ai[kNumBatch];
+            int4 meta[kNumBatch];
+            double v[kNumBatch][kRecVals];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                uint32_t rows = stride4_bits(am, k);
-                const uint32_t cols = stride4_bits(bm, k);
-                if (!cols)
-                    continue;
-                while (rows) {
-                    const int r = __builtin_ctz(rows);
-                    rows &= rows - 1;
-                    const double va = g.a_vals[aoff + payload_slot(am, aspan, r * 4 + k)];
-                    uint32_t cc = cols;
-                    while (cc) {
-                        const int cl = __builtin_ctz(cc);
-                        cc &= cc - 1;
-                        const double vb = g.b_vals[boff + payload_slot(bm, bspan, cl * 4 + k)];
-                        const int pos = r * 8 + cl;
-                        const int slot = cb + __popcll(cm & ((1ull << pos) - 1ull));
-                        atomicAdd(&acc[slot], va * vb);
-                    }
+            for (int u = 0; u < kNumBatch; ++u) {
+                const int p = p0 + u * kThreads;
+                ai[u] = -1;
+                meta[u] = make_int4(0, 0, 0, 0);
+#pragma unroll
+                for (int j = 0; j < kRecVals; ++j)
+                    v[u][j] = 0.0;
+                if (p < total) {
+                    ai[u] = upper_index(s_pre, n, p);
+                    const BRecord *rp = recs + s_b0[ai[u]] + (p - s_pre[ai[u]]);
+                    meta[u] = rp->meta;
+#pragma unroll
+                    for (int j = 0; j < kRecVals; ++j)
+                        v[u][j] = rp->v[j];
                 }
+            }
+#pragma unroll
+            for (int u = 0; u < kNumBatch; ++u) {
+                if (ai[u] < 0)
+                    continue;
+                const uint32_t ab = s_ab[ai[u]];
+                if (!product_count(ab, static_cast<uint32_t>(meta[u].z)))
+                    continue;
+                // Every tile of the row is in the table and at least half of
+                // its slots are empty, so the probe ends.
+                uint32_t h = slot_of(meta[u].x, kNumSlotBits);
+                int q = -1;
+                for (;;) {
+                    const int k = s_hk[h];
+                    if (k == meta[u].x) {
+                        q = s_hq[h];
+                        break;
+                    }
+                    if (k == kEmpty)
+                        break;
+                    h = (h + 1) & (kNumSlots - 1);
+                }
+                if (q < 0)
+                    continue;
+                accumulate_pair(g, s_acc, s_aval, ab, s_am[ai[u]], s_ao[ai[u]], meta[u], v[u],
+                                s_cmask[q], s_cbase[q]);
             }
         }
-        wave_sync();
+        __syncthreads();
     }
-    wave_sync();
-    for (int i = lane; i < nnz_row; i += kWave)
-        c.val[zb + i] = acc[i];
+    for (int64_t i = tid; i < nnz_row; i += kThreads)
+        c.val[zb + i] = s_acc[i];
 }
 
 // ---------------------------------------------------------------------------
-// Numeric for rows that do not fit the wave path: one workgroup per row,
+// Numeric for rows beyond the LDS budgets: one workgroup per row,
 // direct-mapped (tile column -> local tile) slab, FP64 atomics in HBM.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
     Operands g, const int *__restrict__ rows, int nrows, int *__restrict__ slabs, COut c)
 {
-    __shared__ uint32_t s_am[kThreads];
+    __shared__ uint32_t s_ab[kThreads];
     __shared__ int s_b0[kThreads];
     __shared__ int s_pre[kThreads];
     __shared__ int s_tmp[kWavesPerBlock];
@@ -537,53 +658,46 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
         const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
         for (int c0 = a0; c0 < a1; c0 += kThreads) {
             const int n = min(kThreads, a1 - c0);
-            int nb = 0;
-            if (tid < n) {
-                const int a = c0 + tid;
-                const int k = g.a_tci[a];
-                const int b0 = g.b_trp[k];
-                nb = g.b_trp[k + 1] - b0;
-                s_am[tid] = g.a_masks[a];
-                s_b0[tid] = b0;
-            }
-            int total = 0;
-            const int excl = block_exclusive_scan(nb, s_tmp, &total);
-            if (tid < n)
-                s_pre[tid] = excl;
-            __syncthreads();
+            const int total = load_a_chunk(g, c0, n, s_ab, s_b0, s_pre, s_tmp);
             for (int p = tid; p < total; p += kThreads) {
                 const int ai = upper_index(s_pre, n, p);
-                const int b = s_b0[ai] + (p - s_pre[ai]);
-                const uint32_t am = s_am[ai];
-                const uint32_t bm = g.b_masks[b];
-                if (!tile_bool_product(am, bm))
+                const int4 bm = g.b_meta[s_b0[ai] + (p - s_pre[ai])];
+                const uint32_t ab = s_ab[ai], bb = static_cast<uint32_t>(bm.z);
+                if (!product_count(ab, bb))
                     continue;
-                const int q = __hip_atomic_load(&slab[g.b_tci[b]], __ATOMIC_RELAXED,
+                const int4 am4 = g.a_meta[c0 + ai];
+                const uint32_t am = static_cast<uint32_t>(am4.y);
+                const uint32_t aw = static_cast<uint32_t>(am4.w);
+                const bool adense = (aw & kDenseBit) != 0;
+                const uint32_t aoff = aw & ~kDenseBit;
+                const uint32_t bmask = static_cast<uint32_t>(bm.y);
+                const uint32_t bw = static_cast<uint32_t>(bm.w);
+                const bool bdense = (bw & kDenseBit) != 0;
+                const uint32_t boff = bw & ~kDenseBit;
+                const int q = __hip_atomic_load(&slab[bm.x], __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_AGENT);
                 const int64_t tq = t0 + q;
                 const uint64_t cm = c.mask[tq];
                 const int64_t cb = c.tile_nnz[tq];
-                const int at = c0 + ai;
-                const int aoff = g.a_off[at];
-                const int aspan = g.a_off[at + 1] - aoff;
-                const int boff = g.b_off[b];
-                const int bspan = g.b_off[b + 1] - boff;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    uint32_t rws = stride4_bits(am, k);
-                    const uint32_t cols = stride4_bits(bm, k);
+                    uint32_t rws = byte_of(ab, k);
+                    const uint32_t cols = byte_of(bb, k);
                     if (!cols)
                         continue;
                     while (rws) {
                         const int r = __builtin_ctz(rws);
                         rws &= rws - 1;
-                        const double va = g.a_vals[aoff + payload_slot(am, aspan, r * 4 + k)];
+                        const int pa = r * 4 + k;
+                        const double va =
+                            g.a_vals[aoff + (adense ? pa : __popc(am & ((1u << pa) - 1u)))];
                         uint32_t cc = cols;
                         while (cc) {
                             const int cl = __builtin_ctz(cc);
                             cc &= cc - 1;
+                            const int pb = cl * 4 + k;
                             const double vb =
-                                g.b_vals[boff + payload_slot(bm, bspan, cl * 4 + k)];
+                                g.b_vals[boff + (bdense ? pb : __popc(bmask & ((1u << pb) - 1u)))];
                             const int pos = r * 8 + cl;
                             unsafeAtomicAdd(&c.val[cb + __popcll(cm & ((1ull << pos) - 1ull))],
                                             va * vb);
@@ -602,26 +716,37 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
     }
 }
 
-__global__ void collect_rows_kernel(int n, const int *__restrict__ level, int want,
+// Lists rows for a fallback path.  mode 0: symbolic slab rows; mode 1:
+// numeric rows beyond the small LDS instance, split into the large LDS
+// instance's list (out_lds) and the slab list (out_slab).
+__global__ void collect_rows_kernel(int n, int mode, const int *__restrict__ level,
                                     const int64_t *__restrict__ tile_ptr,
-                                    const int64_t *__restrict__ nnz_off, int *__restrict__ out,
-                                    int *__restrict__ count)
+                                    const int64_t *__restrict__ nnz_off,
+                                    const int *__restrict__ a_trp, const int *__restrict__ a_off,
+                                    int *__restrict__ out_slab, int *__restrict__ n_slab,
+                                    int *__restrict__ out_lds, int *__restrict__ n_lds)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    bool take;
-    if (want == kLevelSlab && !tile_ptr) {
-        take = level[i] == kLevelSlab;
-    } else {
-        // numeric fallback: slab-level rows and wave rows that do not fit
-        const int64_t nt = tile_ptr[i + 1] - tile_ptr[i];
-        const int64_t nz = nnz_off[i + 1] - nnz_off[i];
-        take = level[i] == kLevelSlab ||
-               (level[i] == kLevelWave && (nt > kSymLimit || nz > kNumAcc));
+    if (mode == 0) {
+        if (level[i] == kLevelSlab)
+            out_slab[atomicAdd(n_slab, 1)] = i;
+        return;
     }
-    if (take)
-        out[atomicAdd(count, 1)] = i;
+    if (level[i] == kLevelSlab) {
+        out_slab[atomicAdd(n_slab, 1)] = i;
+        return;
+    }
+    if (level[i] != kLevelLds)
+        return;
+    const int64_t nt = tile_ptr[i + 1] - tile_ptr[i];
+    const int64_t nz = nnz_off[i + 1] - nnz_off[i];
+    const int64_t alen = a_off[a_trp[i + 1]] - a_off[a_trp[i]];
+    if (nt > kNumTiles || nz > kAccLarge || alen > kAStageLarge)
+        out_slab[atomicAdd(n_slab, 1)] = i;
+    else if (nz > kAccSmall || alen > kAStageSmall)
+        out_lds[atomicAdd(n_lds, 1)] = i;
 }
 
 __global__ void fill_i32_kernel(int *p, int64_t n, int v)
@@ -659,8 +784,8 @@ double event_ms(hipEvent_t a, hipEvent_t b)
 
 } // namespace
 
-int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w,
-                Events *ev, CoreTimes *times)
+int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Events *ev,
+                CoreTimes *times)
 {
     RTT_TRY(require_ctx());
     RTT_TRY(ev->create());
@@ -675,50 +800,53 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w,
     Operands g;
     g.a_tile_rows = R;
     g.a_trp = a.tile_row_ptr.as<int>();
-    g.a_tci = a.tile_col_idx.as<int>();
-    g.a_masks = a.masks.as<uint32_t>();
+    g.a_meta = a.meta.as<int4>();
     g.a_off = a.value_offsets.as<int>();
     g.a_vals = a.values.as<double>();
     g.b_trp = b.tile_row_ptr.as<int>();
-    g.b_tci = b.tile_col_idx.as<int>();
-    g.b_masks = b.masks.as<uint32_t>();
-    g.b_off = b.value_offsets.as<int>();
-    g.b_vals = b.values.as<double>();
+    g.b_meta = b.meta.as<int4>();
+    g.b_vals = b.payload_values();
     g.b_tile_cols = BC;
 
     const int num_cus = ctx().num_cus > 0 ? ctx().num_cus : 256;
     auto t_sym = Clock::now();
-    // ---- symbolic: per-row counts ------------------------------------
+    // ---- symbolic: per-row tile sets, staged ------------------------------
+    const int stage_limit = 3 * kSymSlots / 4;
     RTT_TRY(w->row_tiles.ensure((R + 1) * sizeof(int64_t)));
     RTT_TRY(w->row_nnz.ensure((R + 1) * sizeof(int64_t)));
     RTT_TRY(w->row_level.ensure((R + 1) * sizeof(int)));
-    RTT_TRY(w->counters.ensure(4 * sizeof(unsigned long long)));
+    RTT_TRY(w->counters.ensure(8 * sizeof(unsigned long long)));
     RTT_TRY(w->l3_rows.ensure((R + 1) * sizeof(int)));
+    RTT_TRY(w->lds_rows.ensure((R + 1) * sizeof(int)));
+    RTT_TRY(w->stage_j.ensure(static_cast<size_t>(R) * stage_limit * sizeof(int)));
+    RTT_TRY(w->stage_m.ensure(static_cast<size_t>(R) * stage_limit * sizeof(uint64_t)));
     RTT_TRY(c->tile_ptr.ensure((R + 1) * sizeof(int64_t)));
     RTT_TRY(w->c_row_nnz_off.ensure((R + 1) * sizeof(int64_t)));
+    Stage st{w->stage_j.as<int>(), w->stage_m.as<unsigned long long>(), stage_limit};
     unsigned long long *ctr = w->counters.as<unsigned long long>();
     int *slab_count = reinterpret_cast<int *>(ctr + 1);
-    int *num_count = reinterpret_cast<int *>(ctr + 2);
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(4), 0, s, ctr, int64_t(4));
+    int *num_slab_count = reinterpret_cast<int *>(ctr + 2);
+    int *num_lds_count = reinterpret_cast<int *>(ctr + 3);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(8), 0, s, ctr, int64_t(8));
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, s,
                        reinterpret_cast<unsigned long long *>(w->row_tiles.as<int64_t>() + R),
                        int64_t(1));
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, s,
                        reinterpret_cast<unsigned long long *>(w->row_nnz.as<int64_t>() + R),
                        int64_t(1));
-    COut none{};
     RTT_HIP(hipEventRecord(ev->ev[0], s));
     if (R > 0)
-        hipLaunchKernelGGL((symbolic_wave_kernel<false>), dim3(blocks_for(R, kWavesPerBlock)),
-                           dim3(kThreads), 0, s, g, w->row_tiles.as<int64_t>(),
-                           w->row_nnz.as<int64_t>(), w->row_level.as<int>(), ctr, none);
+        hipLaunchKernelGGL(symbolic_count_kernel, dim3(R), dim3(kThreads), 0, s, g, stage_limit,
+                           st, w->row_tiles.as<int64_t>(), w->row_nnz.as<int64_t>(),
+                           w->row_level.as<int>(), ctr);
     RTT_HIP(hipGetLastError());
     RTT_HIP(hipEventRecord(ev->ev[1], s));
     int n_slab = 0;
+    COut none{};
     if (R > 0) {
         hipLaunchKernelGGL(collect_rows_kernel, dim3(blocks_for(R, kThreads)), dim3(kThreads), 0,
-                           s, R, w->row_level.as<int>(), int(kLevelSlab), nullptr, nullptr,
-                           w->l3_rows.as<int>(), slab_count);
+                           s, R, 0, w->row_level.as<int>(), nullptr, nullptr, nullptr, nullptr,
+                           w->l3_rows.as<int>(), slab_count, nullptr, nullptr);
         RTT_TRY(read_scalar(slab_count, &n_slab, sizeof(int), s));
     }
     int slab_blocks = 0;
@@ -745,7 +873,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w,
     RTT_TRY(read_scalar(ctr, &pairs, sizeof(pairs), s));
     const double sym_count_ms = ms_since(t_sym);
 
-    // ---- allocate C ---------------------------------------------------
+    // ---- allocate C -----------------------------------------------------
     auto t_alloc = Clock::now();
     c->num_tiles = totals[0];
     c->nnz = totals[1];
@@ -757,7 +885,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w,
     RTT_TRY(c->tile_csr_val.ensure(c->nnz * sizeof(double)));
     times->allocation_ms = ms_since(t_alloc);
 
-    // ---- symbolic: exact structure ---------------------------------------
+    // ---- symbolic: final structure -------------------------------------------
     auto t_fill = Clock::now();
     COut co;
     co.tile_ptr = c->tile_ptr.as<int64_t>();
@@ -770,9 +898,8 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w,
     co.val = c->tile_csr_val.as<double>();
     RTT_HIP(hipEventRecord(ev->ev[2], s));
     if (R > 0 && c->num_tiles > 0)
-        hipLaunchKernelGGL((symbolic_wave_kernel<true>), dim3(blocks_for(R, kWavesPerBlock)),
-                           dim3(kThreads), 0, s, g, nullptr, nullptr, w->row_level.as<int>(),
-                           nullptr, co);
+        hipLaunchKernelGGL(symbolic_write_kernel, dim3(blocks_for(R, kWavesPerBlock)),
+                           dim3(kThreads), 0, s, R, st, w->row_level.as<int>(), co);
     RTT_HIP(hipGetLastError());
     RTT_HIP(hipEventRecord(ev->ev[3], s));
     if (n_slab > 0)
@@ -786,22 +913,32 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w,
     RTT_HIP(hipStreamSynchronize(s));
     times->symbolic_ms = sym_count_ms + ms_since(t_fill);
 
-    // ---- numeric --------------------------------------------------------
+    // ---- numeric ----------------------------------------------------------
     auto t_num = Clock::now();
-    int n_num_slab = 0;
+    int n_num_slab = 0, n_num_lds = 0;
     if (R > 0) {
         hipLaunchKernelGGL(collect_rows_kernel, dim3(blocks_for(R, kThreads)), dim3(kThreads), 0,
-                           s, R, w->row_level.as<int>(), int(kLevelWave),
-                           c->tile_ptr.as<int64_t>(), w->c_row_nnz_off.as<int64_t>(),
-                           w->l3_rows.as<int>(), num_count);
-        RTT_TRY(read_scalar(num_count, &n_num_slab, sizeof(int), s));
+                           s, R, 1, w->row_level.as<int>(), c->tile_ptr.as<int64_t>(),
+                           w->c_row_nnz_off.as<int64_t>(), g.a_trp, g.a_off,
+                           w->l3_rows.as<int>(), num_slab_count, w->lds_rows.as<int>(),
+                           num_lds_count);
+        int counts[2] = {0, 0};
+        RTT_TRY(read_scalar(num_slab_count, &counts[0], sizeof(int), s));
+        RTT_TRY(read_scalar(num_lds_count, &counts[1], sizeof(int), s));
+        n_num_slab = counts[0];
+        n_num_lds = counts[1];
     }
     RTT_HIP(hipEventRecord(ev->ev[4], s));
     if (R > 0 && c->nnz > 0)
-        hipLaunchKernelGGL(numeric_wave_kernel, dim3(blocks_for(R, kWavesPerBlock)),
-                           dim3(kThreads), 0, s, g, w->row_level.as<int>(), co);
+        hipLaunchKernelGGL((numeric_row_kernel<kAccSmall, kAStageSmall>), dim3(R), dim3(kThreads),
+                           0, s, g, b.records.as<BRecord>(), nullptr, R, w->row_level.as<int>(),
+                           co);
     RTT_HIP(hipGetLastError());
     RTT_HIP(hipEventRecord(ev->ev[5], s));
+    if (n_num_lds > 0)
+        hipLaunchKernelGGL((numeric_row_kernel<kAccLarge, kAStageLarge>), dim3(n_num_lds),
+                           dim3(kThreads), 0, s, g, b.records.as<BRecord>(),
+                           w->lds_rows.as<int>(), n_num_lds, w->row_level.as<int>(), co);
     if (n_num_slab > 0) {
         const int blocks = std::min(n_num_slab, 2 * num_cus);
         DevBuf &idx = w->slab;   // reused: numeric runs after symbolic finished
@@ -820,7 +957,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w,
     times->sym_fill_kernel_ms = event_ms(ev->ev[2], ev->ev[3]);
     times->numeric_kernel_ms = event_ms(ev->ev[4], ev->ev[5]);
     times->pairs = pairs;
-    times->heavy_rows = static_cast<unsigned long long>(n_num_slab);
+    times->heavy_rows = static_cast<unsigned long long>(n_num_slab + n_num_lds);
     return RTT_OK;
 }
 

@@ -42,14 +42,14 @@ def oracle_tiles_equal(got, want, csc):
             assert np.array_equal(got[k], want[k]), k
 
 
-def check_a_tiles(rt, a, pa):
+def check_a_tiles(rt, a, pa, dense_threshold=24):
     ar, ai, _ = pa.info()
     m, n, rp, ci, v = as_tuple(a)
-    want = oracle.hybrid_tiles(*sub_rows((m, ai, rp, ci, v), ar), 8, 4, False)
+    want = oracle.hybrid_tiles(*sub_rows((m, ai, rp, ci, v), ar), 8, 4, False, dense_threshold)
     oracle_tiles_equal(pa.tiles(), want, csc=False)
 
 
-def check_b_tiles(rt, a, pa, pb, aat):
+def check_b_tiles(rt, a, pa, pb, aat, dense_threshold=24):
     ar, ai, _ = pa.info()
     m, n, rp, ci, v = as_tuple(a)
     if aat:
@@ -57,7 +57,7 @@ def check_b_tiles(rt, a, pa, pb, aat):
         src = (n, m, trp, tci, tv)
     else:
         src = (m, n, rp, ci, v)
-    want = oracle.hybrid_tiles(*sub_rows(src, ai), 4, 8, True, build_csc=True)
+    want = oracle.hybrid_tiles(*sub_rows(src, ai), 4, 8, True, dense_threshold, build_csc=True)
     oracle_tiles_equal(pb.tiles(), want, csc=True)
 
 
@@ -75,7 +75,11 @@ def check_c(rt, tc, ref, positive=False):
     ct = oracle.c_tiles(ref["m"], ref["n"], ref["rowptr"], ref["col"], ref["val"])
     assert tc.numtile == ct["num_tiles"]
     assert tc.nnz == ref["nnz"]
-    assert np.array_equal(tc.tile_ptr, ct["tile_ptr"])
+    # C has A's tile-row count, trimmed to the active row prefix
+    # (dmma_copy_output_to_host, src/dmma_spgemm.h:6409-6414); no oracle tile
+    # lies beyond it.
+    assert np.array_equal(tc.tile_ptr, ct["tile_ptr"][:tc.tilem + 1])
+    assert np.all(ct["tile_ptr"][tc.tilem:] == ct["num_tiles"])
     assert np.array_equal(tc.tile_columnidx, ct["tile_col"])
     assert np.array_equal(tc.tile_nnz, ct["tile_nnz"])
     assert np.array_equal(tc.tile_csr_Ptr, ct["tile_csr_ptr"])
@@ -98,9 +102,9 @@ def check_c(rt, tc, ref, positive=False):
 
 def run(rt, a, aat, positive=False, dense_threshold=24):
     pa = rt.prepare_identity_a(a, dense_threshold)
-    check_a_tiles(rt, a, pa)
+    check_a_tiles(rt, a, pa, dense_threshold)
     pb = rt.rebuild_b(pa, aat)
-    check_b_tiles(rt, a, pa, pb, aat)
+    check_b_tiles(rt, a, pa, pb, aat, dense_threshold)
     tc, stats = rt.tilespgemm(pa, pb)
     ref = oracle_product(a, aat)
     err = check_c(rt, tc, ref, positive)
