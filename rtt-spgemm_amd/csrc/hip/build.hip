@@ -253,22 +253,6 @@ __global__ void transpose_payload_kernel(int64_t n, const int *__restrict__ ids,
         b_vals[dst + s] = a_vals[src + s];
 }
 
-// B tile b (in B's tile-row order) is A tile ids[b]: its meta is A's with
-// the tile column replaced by A's tile row; the payload stays A's.
-__global__ void transposed_meta_kernel(int64_t n, const int *__restrict__ ids,
-                                       const int *__restrict__ a_row_of,
-                                       const int4 *__restrict__ a_meta,
-                                       int4 *__restrict__ b_meta)
-{
-    const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (b < n) {
-        const int a = ids[b];
-        int4 m = a_meta[a];
-        m.x = a_row_of[a];
-        b_meta[b] = m;
-    }
-}
-
 // Row pointer of a sorted key array: ptr[k] = first index with key >= k.
 __global__ void row_ptr_from_sorted_kernel(const int *__restrict__ keys, int64_t n, int rows,
                                            int *__restrict__ ptr)
@@ -282,23 +266,67 @@ __global__ void row_ptr_from_sorted_kernel(const int *__restrict__ keys, int64_t
         ptr[k] = static_cast<int>(i);
 }
 
-// One 80-byte record per B tile: meta + the first kRecVals payload values.
-__global__ void records_kernel(int64_t n, const int4 *__restrict__ meta,
-                               const double *__restrict__ payload, BRecord *__restrict__ recs)
+// One 80-byte record per B tile (meta + the first kRecVals payload values),
+// written as five 16-byte parts by five consecutive threads.
+__device__ __forceinline__ void write_record_part(BRecord *recs, int64_t t, int part,
+                                                  const int4 &m, const double *payload)
 {
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= n)
+    int4 *dst = reinterpret_cast<int4 *>(recs + t) + part;
+    if (part == 0) {
+        *dst = m;
         return;
-    const int4 m = meta[t];
+    }
     const uint32_t w = static_cast<uint32_t>(m.w);
     const uint32_t off = w & 0x7FFFFFFFu;
     const int nv = (w & 0x80000000u) ? kRecVals : min(kRecVals, __popc(static_cast<uint32_t>(m.y)));
-    BRecord r;
-    r.meta = m;
-#pragma unroll
-    for (int j = 0; j < kRecVals; ++j)
-        r.v[j] = j < nv ? payload[off + j] : 0.0;
-    recs[t] = r;
+    const int j = 2 * (part - 1);
+    double2 v;
+    v.x = j < nv ? payload[off + j] : 0.0;
+    v.y = j + 1 < nv ? payload[off + j + 1] : 0.0;
+    *reinterpret_cast<double2 *>(dst) = v;
+}
+
+__global__ void records_kernel(int64_t n, const int4 *__restrict__ meta,
+                               const double *__restrict__ payload, BRecord *__restrict__ recs)
+{
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t t = gid / 5;
+    if (t >= n)
+        return;
+    write_record_part(recs, t, static_cast<int>(gid % 5), meta[t], payload);
+}
+
+// B = A^T: B tile b is A tile ids[b]; its meta is A's with the tile column
+// replaced by A's tile row, and the record's values come from A's payload.
+__global__ void transposed_records_kernel(int64_t n, const int *__restrict__ ids,
+                                          const int *__restrict__ a_row_of,
+                                          const int4 *__restrict__ a_meta,
+                                          const double *__restrict__ a_vals,
+                                          int4 *__restrict__ b_meta, BRecord *__restrict__ recs)
+{
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t t = gid / 5;
+    if (t >= n)
+        return;
+    const int part = static_cast<int>(gid % 5);
+    const int a = ids[t];
+    int4 m = a_meta[a];
+    if (part == 0) {
+        m.x = a_row_of[a];
+        b_meta[t] = m;
+    }
+    write_record_part(recs, t, part, m, a_vals);
+}
+
+__global__ void pair_extent_kernel(int64_t n, const int4 *__restrict__ a_meta,
+                                   const int *__restrict__ b_trp, int2 *__restrict__ out)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < n) {
+        const int k = a_meta[t].x;
+        const int b0 = b_trp[k];
+        out[t] = make_int2(b0, b_trp[k + 1] - b0);
+    }
 }
 
 __global__ void entry_rows_kernel(int rows, const int *__restrict__ rp,
@@ -521,9 +549,20 @@ int build_records(DevTiles *t)
 {
     RTT_TRY(t->records.ensure(t->num_tiles * sizeof(BRecord)));
     if (t->num_tiles > 0)
-        hipLaunchKernelGGL(records_kernel, dim3(blocks_for(t->num_tiles, kThreads)),
+        hipLaunchKernelGGL(records_kernel, dim3(blocks_for(t->num_tiles * 5, kThreads)),
                            dim3(kThreads), 0, ctx().stream, t->num_tiles, t->meta.as<int4>(),
                            t->payload_values(), t->records.as<BRecord>());
+    RTT_HIP(hipGetLastError());
+    return RTT_OK;
+}
+
+int build_pair_extents(const DevTiles &a, DevTiles *b)
+{
+    RTT_TRY(b->a_brange.ensure(a.num_tiles * sizeof(int2)));
+    if (a.num_tiles > 0)
+        hipLaunchKernelGGL(pair_extent_kernel, dim3(blocks_for(a.num_tiles, kThreads)),
+                           dim3(kThreads), 0, ctx().stream, a.num_tiles, a.meta.as<int4>(),
+                           b->tile_row_ptr.as<int>(), b->a_brange.as<int2>());
     RTT_HIP(hipGetLastError());
     return RTT_OK;
 }
@@ -675,9 +714,11 @@ int transpose_tiles(const DevTiles &a, int b_cols, DevTiles *b, BuildWork *w)
                            w->ids_in.as<int>(), n);
         RTT_TRY(sort_pairs(a.tile_col_idx.as<int>(), w->keys_out.as<int>(), w->ids_in.as<int>(),
                            b->sorted_ids.as<int>(), n, bits_for(R), &w->scan_tmp, s));
-        hipLaunchKernelGGL(transposed_meta_kernel, dim3(blocks_for(n, kThreads)),
+        RTT_TRY(b->records.ensure(n * sizeof(BRecord)));
+        hipLaunchKernelGGL(transposed_records_kernel, dim3(blocks_for(n * 5, kThreads)),
                            dim3(kThreads), 0, s, n, b->sorted_ids.as<int>(),
-                           a.tile_row_of.as<int>(), a.meta.as<int4>(), b->meta.as<int4>());
+                           a.tile_row_of.as<int>(), a.meta.as<int4>(), a.values.as<double>(),
+                           b->meta.as<int4>(), b->records.as<BRecord>());
     }
     hipLaunchKernelGGL(row_ptr_from_sorted_kernel, dim3(blocks_for(n + 1, kThreads)),
                        dim3(kThreads), 0, s, w->keys_out.as<int>(), n, R,

@@ -56,6 +56,19 @@ struct DevBuf {
     DevBuf &operator=(const DevBuf &) = delete;
 };
 
+// Grow-only pinned host buffer (hipHostMalloc) for small readbacks.
+struct PinnedBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    template <typename T>
+    T *as() const { return static_cast<T *>(ptr); }
+    ~PinnedBuf();
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+};
+
 // Device CSR with rows sorted by column (duplicates adjacent).
 struct DevCsr {
     int rows = 0, cols = 0;
@@ -78,6 +91,7 @@ struct DevTiles {
     DevBuf tile_row_of;   // tile -> tile row (A: used by the B transpose)
     DevBuf meta;          // int4 per tile
     DevBuf records;       // BRecord per tile (B only)
+    DevBuf a_brange;      // B only: per A tile, {first B tile, count} of B row K
     // B built as A^T at tile level: meta offsets index A's payload, and
     // sorted_ids[b] is the A tile behind B tile b.
     const DevTiles *transpose_of = nullptr;
@@ -100,7 +114,8 @@ struct DevC {
 // Scratch of the SpGEMM core, reused across calls.
 struct SpgemmWork {
     DevBuf row_tiles, row_nnz, row_level;   // per A tile row
-    DevBuf scan_tmp, counters, l3_rows, lds_rows, slab;
+    DevBuf scan_tmp, counters, l3_rows, num_rows, lds_rows, slab;
+    PinnedBuf host_counters;
     DevBuf stage_j, stage_m;                // staged sorted tile lists
     DevBuf c_row_nnz_off;                   // per tile row, int64
 };
@@ -151,6 +166,8 @@ int nnz_cub(const DevCsr &csr, bool aat, unsigned long long *out, BuildWork *wor
 int fill_tile_row_of(DevTiles *t);
 // Fills t->records from t->meta and the payload the meta offsets index.
 int build_records(DevTiles *t);
+// b->a_brange[t] = {B.tile_row_ptr[K], count} for every A tile t in column K.
+int build_pair_extents(const DevTiles &a, DevTiles *b);
 
 // ---- SpGEMM core (spgemm.hip) ---------------------------------------------
 struct CoreTimes {

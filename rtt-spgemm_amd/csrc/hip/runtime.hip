@@ -53,6 +53,29 @@ void DevBuf::release()
     bytes = 0;
 }
 
+int PinnedBuf::ensure(size_t need)
+{
+    if (need <= bytes && ptr)
+        return RTT_OK;
+    if (ptr)
+        (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    if (hipHostMalloc(&ptr, need < 64 ? 64 : need, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        ptr = nullptr;
+        return fail(RTT_ERR_OOM, "hipHostMalloc(%zu) failed", need);
+    }
+    bytes = need < 64 ? 64 : need;
+    return RTT_OK;
+}
+
+PinnedBuf::~PinnedBuf()
+{
+    if (ptr)
+        (void)hipHostFree(ptr);
+}
+
 int Events::create()
 {
     if (ready)

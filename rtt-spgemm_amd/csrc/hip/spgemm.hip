@@ -9,10 +9,15 @@
 // src/super16_symbolic.cuh:2406-2774, and a K-merge per C tile,
 // src/dmma_spgemm.h:4307-4410).
 //
+// Pairs are enumerated per chunk of up to kChunk A tiles: every A tile's
+// B-row extent comes from a per-A-tile table built with B, the pair count is
+// scanned, and an LDS segment table maps pair index -> A tile, so a thread
+// reaches its (A tile, B tile) with two LDS reads.
+//
 //  * symbolic: the 8x8 masks of the row's C tiles are ORed into an LDS hash
-//    table, compacted, bitonic-sorted by tile column and staged; after the
-//    size scan a light pass writes the final structure.  Rows whose tile set
-//    outgrows the table use a direct-mapped mask slab in HBM instead.
+//    table, compacted, ranked by tile column and staged; after the size scan
+//    a light pass writes the final structure.  Rows whose tile set outgrows
+//    the table use a direct-mapped mask slab in HBM instead.
 //  * numeric: the row's exact C layout (tile columns, masks, value offsets)
 //    and its A payload are loaded into LDS.  Each thread takes pairs in
 //    batches, loading every pair's 80-byte B record (packed meta + first
@@ -32,18 +37,26 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / kWave;
 
+constexpr int kChunk = 128;        // A tiles per enumeration step
+constexpr int kSeg = 1024;         // pairs per segment-table window
+
 constexpr int kSymSlots = 1024;    // LDS hash slots (power of 2)
 constexpr int kSymSlotBits = 10;
 constexpr int kSymPerThread = kSymSlots / kThreads;
 constexpr int kSymBatch = 4;       // pairs whose loads a thread issues together
 
-constexpr int kNumSlots = 1024;    // numeric J -> local tile hash
-constexpr int kNumSlotBits = 10;
-constexpr int kNumTiles = 512;     // C tiles per row on the LDS path
-constexpr int kChunk = 128;        // A tiles loaded per step
-constexpr int kNumBatch = 2;       // pairs whose loads a thread issues together
-constexpr int kAccSmall = 1024, kAStageSmall = 256;   // LDS budgets, common rows
-constexpr int kAccLarge = 4096, kAStageLarge = 2048;  // LDS budgets, large rows
+constexpr int kNumBatch = 2;       // pairs whose records a thread loads together
+
+// LDS budgets of the two numeric instances: accumulator slots, staged A
+// values, C tiles, hash slots.
+struct SmallRow {
+    static constexpr int kAcc = 1024, kAStage = 256, kTiles = 256, kSlots = 512, kSlotBits = 9;
+};
+struct LargeRow {
+    static constexpr int kAcc = 4096, kAStage = 2048, kTiles = 512, kSlots = 1024, kSlotBits = 10;
+};
+
+constexpr int kWriteBuf = 2048;    // column bytes staged per row in the write pass
 
 constexpr int kEmpty = -1;
 constexpr uint32_t kDenseBit = 0x80000000u;
@@ -54,6 +67,7 @@ struct Operands {
     int a_tile_rows;
     const int *__restrict__ a_trp;
     const int4 *__restrict__ a_meta;
+    const int2 *__restrict__ a_brange;   // per A tile: first B tile, count
     const int *__restrict__ a_off;
     const double *__restrict__ a_vals;
     const int *__restrict__ b_trp;
@@ -166,30 +180,18 @@ __device__ __forceinline__ T block_sum(T v, T *s_tmp)
     return all;
 }
 
-// Loads up to 256 A tiles of a row (their column-byte sets and B-row
-// extents) and an exclusive prefix of their pair counts.
-__device__ __forceinline__ int load_a_chunk(const Operands &g, int c0, int n, uint32_t *s_ab,
-                                            int *s_b0, int *s_pre, int *s_tmp)
+// Segment table of pairs [r0, r1): s_seg[p - r0] = chunk-local A tile.
+// Thread t < n owns A tile t with pairs [pre_t, pre_t + nb_t).
+__device__ __forceinline__ void fill_segments(unsigned char *s_seg, int pre, int nb, int r0,
+                                              int r1)
 {
-    const int t = threadIdx.x;
-    int nb = 0;
-    if (t < n) {
-        const int4 m = g.a_meta[c0 + t];
-        const int b0 = g.b_trp[m.x];
-        nb = g.b_trp[m.x + 1] - b0;
-        s_ab[t] = static_cast<uint32_t>(m.z);
-        s_b0[t] = b0;
-    }
-    int total = 0;
-    const int excl = block_exclusive_scan(nb, s_tmp, &total);
-    if (t < n)
-        s_pre[t] = excl;
-    __syncthreads();
-    return total;
+    const int lo = max(pre, r0), hi = min(pre + nb, r1);
+    for (int p = lo; p < hi; ++p)
+        s_seg[p - r0] = static_cast<unsigned char>(threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
-// Symbolic: per-row hash of C tile masks, compaction, bitonic sort, staging.
+// Symbolic: per-row hash of C tile masks, compaction, ranking, staging.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
     Operands g, int stage_limit, Stage st, int64_t *__restrict__ row_tiles,
@@ -198,9 +200,10 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
 {
     __shared__ int s_key[kSymSlots];
     __shared__ unsigned long long s_msk[kSymSlots];
-    __shared__ uint32_t s_ab[kThreads];
-    __shared__ int s_b0[kThreads];
-    __shared__ int s_pre[kThreads];
+    __shared__ unsigned long long s_spread[4][kChunk];
+    __shared__ int s_b0[kChunk];
+    __shared__ int s_pre[kChunk];
+    __shared__ unsigned char s_seg[kSeg];
     __shared__ int s_tmp[kWavesPerBlock];
     __shared__ long long s_tmp64[kWavesPerBlock];
     __shared__ int s_cnt, s_ovf;
@@ -215,60 +218,86 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
         s_cnt = 0;
         s_ovf = 0;
     }
-    __syncthreads();
     const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
     long long pairs = 0;
-    for (int c0 = a0; c0 < a1; c0 += kThreads) {
-        const int n = min(kThreads, a1 - c0);
-        const int total = load_a_chunk(g, c0, n, s_ab, s_b0, s_pre, s_tmp);
-        // Each thread takes kSymBatch pairs at a time and issues their
-        // B-record loads together before using any of them.
-        for (int p0 = tid; p0 < total; p0 += kSymBatch * kThreads) {
-            int4 bm[kSymBatch];
-            int ai[kSymBatch];
+    for (int c0 = a0; c0 < a1; c0 += kChunk) {
+        const int n = min(kChunk, a1 - c0);
+        int nb = 0;
+        if (tid < n) {
+            const uint32_t ab = static_cast<uint32_t>(g.a_meta[c0 + tid].z);
+            const int2 br = g.a_brange[c0 + tid];
+            nb = br.y;
+            s_b0[tid] = br.x;
 #pragma unroll
-            for (int u = 0; u < kSymBatch; ++u) {
-                const int p = p0 + u * kThreads;
-                ai[u] = 0;
-                bm[u] = make_int4(0, 0, 0, 0);
-                if (p < total) {
-                    ai[u] = upper_index(s_pre, n, p);
-                    bm[u] = g.b_meta[s_b0[ai[u]] + (p - s_pre[ai[u]])];
+            for (int k = 0; k < 4; ++k)
+                s_spread[k][tid] = rows_to_bytes(byte_of(ab, k));
+        }
+        int total = 0;
+        const int pre = block_exclusive_scan(nb, s_tmp, &total);
+        if (tid < n)
+            s_pre[tid] = pre;
+        for (int r0 = 0; r0 < total; r0 += kSeg) {
+            const int r1 = min(total, r0 + kSeg);
+            if (tid < n)
+                fill_segments(s_seg, pre, nb, r0, r1);
+            __syncthreads();
+            // Each thread takes kSymBatch pairs and issues their B-meta loads
+            // together before using any of them.
+            for (int p0 = r0 + tid; p0 < r1; p0 += kSymBatch * kThreads) {
+                int4 bm[kSymBatch];
+                int ai[kSymBatch];
+#pragma unroll
+                for (int u = 0; u < kSymBatch; ++u) {
+                    const int p = p0 + u * kThreads;
+                    ai[u] = -1;
+                    bm[u] = make_int4(0, 0, 0, 0);
+                    if (p < r1) {
+                        ai[u] = s_seg[p - r0];
+                        bm[u] = g.b_meta[s_b0[ai[u]] + (p - s_pre[ai[u]])];
+                    }
                 }
-            }
 #pragma unroll
-            for (int u = 0; u < kSymBatch; ++u) {
-                const uint64_t pm = bytes_product(s_ab[ai[u]], static_cast<uint32_t>(bm[u].z));
-                if (!pm)
-                    continue;
-                const int j = bm[u].x;
-                uint32_t h = slot_of(j, kSymSlotBits);
-                for (int probe = 0; probe < kSymSlots; ++probe) {
-                    int k = s_key[h];
-                    if (k == kEmpty) {
-                        k = atomicCAS(&s_key[h], kEmpty, j);
+                for (int u = 0; u < kSymBatch; ++u) {
+                    if (ai[u] < 0)
+                        continue;
+                    const uint32_t bb = static_cast<uint32_t>(bm[u].z);
+                    uint64_t pm = 0;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        pm |= s_spread[k][ai[u]] &
+                              (static_cast<uint64_t>(byte_of(bb, k)) * 0x0101010101010101ull);
+                    if (!pm)
+                        continue;
+                    const int j = bm[u].x;
+                    uint32_t h = slot_of(j, kSymSlotBits);
+                    for (int probe = 0; probe < kSymSlots; ++probe) {
+                        int k = s_key[h];
                         if (k == kEmpty) {
+                            k = atomicCAS(&s_key[h], kEmpty, j);
+                            if (k == kEmpty) {
+                                atomicOr(&s_msk[h], static_cast<unsigned long long>(pm));
+                                if (atomicAdd(&s_cnt, 1) >= stage_limit)
+                                    s_ovf = 1;
+                                break;
+                            }
+                        }
+                        if (k == j) {
                             atomicOr(&s_msk[h], static_cast<unsigned long long>(pm));
-                            if (atomicAdd(&s_cnt, 1) >= stage_limit)
-                                s_ovf = 1;
                             break;
                         }
+                        h = (h + 1) & (kSymSlots - 1);
+                        if (probe == kSymSlots - 1)
+                            s_ovf = 1;
                     }
-                    if (k == j) {
-                        atomicOr(&s_msk[h], static_cast<unsigned long long>(pm));
-                        break;
-                    }
-                    h = (h + 1) & (kSymSlots - 1);
-                    if (probe == kSymSlots - 1)
-                        s_ovf = 1;
                 }
             }
+            __syncthreads();
         }
         pairs += total;
-        __syncthreads();
         if (s_ovf)
             break;
     }
+    __syncthreads();
     if (s_ovf) {
         if (tid == 0) {
             row_level[row] = kLevelSlab;
@@ -278,7 +307,7 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
         return;
     }
     const int nt = s_cnt;
-    // Compact occupied slots to the front.
+    // Compact the occupied slots to the front.
     int rk[kSymPerThread];
     unsigned long long rm[kSymPerThread];
     int mine = 0;
@@ -299,36 +328,48 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
             nz += __popcll(rm[i]);
             ++pos;
         }
-    int npow = 1;
-    while (npow < nt)
-        npow <<= 1;
-    for (int i = nt + tid; i < npow; i += kThreads)
-        s_key[i] = INT_MAX;
     __syncthreads();
-    // Bitonic sort of the nt (padded to npow) entries by tile column.
-    for (int k = 2; k <= npow; k <<= 1) {
-        for (int jj = k >> 1; jj > 0; jj >>= 1) {
-            for (int i = tid; i < npow; i += kThreads) {
-                const int ixj = i ^ jj;
-                if (ixj > i) {
-                    const int ki = s_key[i], kx = s_key[ixj];
-                    const bool up = (i & k) == 0;
-                    if ((ki > kx) == up) {
-                        s_key[i] = kx;
-                        s_key[ixj] = ki;
-                        const unsigned long long t = s_msk[i];
-                        s_msk[i] = s_msk[ixj];
-                        s_msk[ixj] = t;
+    const int64_t sb = static_cast<int64_t>(row) * st.stride;
+    if (nt <= kThreads) {
+        // Rank by counting: entry t goes to the number of smaller columns.
+        if (tid < nt) {
+            const int mj = s_key[tid];
+            int rank = 0;
+            for (int f = 0; f < nt; ++f)
+                rank += s_key[f] < mj ? 1 : 0;
+            st.j[sb + rank] = mj;
+            st.m[sb + rank] = s_msk[tid];
+        }
+    } else {
+        int npow = 1;
+        while (npow < nt)
+            npow <<= 1;
+        for (int i = nt + tid; i < npow; i += kThreads)
+            s_key[i] = INT_MAX;
+        __syncthreads();
+        for (int k = 2; k <= npow; k <<= 1) {
+            for (int jj = k >> 1; jj > 0; jj >>= 1) {
+                for (int i = tid; i < npow; i += kThreads) {
+                    const int ixj = i ^ jj;
+                    if (ixj > i) {
+                        const int ki = s_key[i], kx = s_key[ixj];
+                        const bool up = (i & k) == 0;
+                        if ((ki > kx) == up) {
+                            s_key[i] = kx;
+                            s_key[ixj] = ki;
+                            const unsigned long long t = s_msk[i];
+                            s_msk[i] = s_msk[ixj];
+                            s_msk[ixj] = t;
+                        }
                     }
                 }
+                __syncthreads();
             }
-            __syncthreads();
         }
-    }
-    const int64_t sb = static_cast<int64_t>(row) * st.stride;
-    for (int i = tid; i < nt; i += kThreads) {
-        st.j[sb + i] = s_key[i];
-        st.m[sb + i] = s_msk[i];
+        for (int i = tid; i < nt; i += kThreads) {
+            st.j[sb + i] = s_key[i];
+            st.m[sb + i] = s_msk[i];
+        }
     }
     const long long row_nz = block_sum(nz, s_tmp64);
     if (tid == 0) {
@@ -339,27 +380,54 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
     }
 }
 
-// Staged (column, mask) lists -> final C structure; one wave per row.
+// Staged (column, mask) lists -> final C structure; one wave per row.  The
+// row's local-column bytes are assembled in LDS and written coalesced.
 __global__ __launch_bounds__(kThreads) void symbolic_write_kernel(
     int rows, Stage st, const int *__restrict__ row_level, COut c)
 {
-    const int row = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    __shared__ unsigned char s_buf[kWavesPerBlock][kWriteBuf];
+    const int w = threadIdx.x / kWave;
+    const int row = blockIdx.x * kWavesPerBlock + w;
     const int lane = lane_id();
     if (row >= rows || row_level[row] != kLevelLds)
         return;
     const int64_t tb = c.tile_ptr[row];
     const int nt = static_cast<int>(c.tile_ptr[row + 1] - tb);
     const int64_t zb = c.row_nnz_off[row];
+    const int64_t nnz_row = c.row_nnz_off[row + 1] - zb;
+    const bool staged = nnz_row <= kWriteBuf;
     const int64_t sb = static_cast<int64_t>(row) * st.stride;
+    unsigned char *buf = s_buf[w];
     int64_t carry = 0;
     for (int q0 = 0; q0 < nt; q0 += kWave) {
         const int q = q0 + lane;
-        const uint64_t m = q < nt ? st.m[sb + q] : 0ull;
+        uint64_t m = q < nt ? st.m[sb + q] : 0ull;
         const int pc = __popcll(m);
         const int incl = wave_inclusive_scan(pc);
-        if (q < nt)
-            write_tile_structure(c, tb + q, st.j[sb + q], m, zb + carry + (incl - pc));
+        if (q < nt) {
+            const int64_t off = carry + (incl - pc);
+            const int64_t t = tb + q;
+            c.col[t] = st.j[sb + q];
+            c.mask[t] = m;
+            c.tile_nnz[t] = zb + off;
+            reinterpret_cast<uint64_t *>(c.csr_ptr)[t] = packed_row_offsets(m);
+            int i = 0;
+            while (m) {
+                const int pos = __builtin_ctzll(m);
+                m &= m - 1;
+                if (staged)
+                    buf[off + i] = static_cast<unsigned char>(pos & 7);
+                else
+                    c.csr_col[zb + off + i] = static_cast<uint8_t>(pos & 7);
+                ++i;
+            }
+        }
         carry += __shfl(incl, kWave - 1, kWave);
+    }
+    if (staged) {
+        wave_sync();
+        for (int64_t i = lane; i < nnz_row; i += kWave)
+            c.csr_col[zb + i] = buf[i];
     }
 }
 
@@ -368,6 +436,26 @@ __global__ __launch_bounds__(kThreads) void symbolic_write_kernel(
 __device__ __forceinline__ unsigned long long load_coherent(const unsigned long long *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Chunk of up to kThreads A tiles for the slab kernels (binary search).
+__device__ __forceinline__ int load_slab_chunk(const Operands &g, int c0, int n, uint32_t *s_ab,
+                                               int *s_b0, int *s_pre, int *s_tmp)
+{
+    const int t = threadIdx.x;
+    int nb = 0;
+    if (t < n) {
+        const int2 br = g.a_brange[c0 + t];
+        nb = br.y;
+        s_ab[t] = static_cast<uint32_t>(g.a_meta[c0 + t].z);
+        s_b0[t] = br.x;
+    }
+    int total = 0;
+    const int excl = block_exclusive_scan(nb, s_tmp, &total);
+    if (t < n)
+        s_pre[t] = excl;
+    __syncthreads();
+    return total;
 }
 
 template <bool FILL>
@@ -395,14 +483,13 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
         for (int c0 = a0; c0 < a1; c0 += kThreads) {
             const int n = min(kThreads, a1 - c0);
             if (tid < n) {
-                const int4 m = g.a_meta[c0 + tid];
-                const int b0 = g.b_trp[m.x], b1 = g.b_trp[m.x + 1];
-                if (b1 > b0) {
-                    atomicMin(&s_lo, g.b_meta[b0].x);
-                    atomicMax(&s_hi, g.b_meta[b1 - 1].x);
+                const int2 br = g.a_brange[c0 + tid];
+                if (br.y > 0) {
+                    atomicMin(&s_lo, g.b_meta[br.x].x);
+                    atomicMax(&s_hi, g.b_meta[br.x + br.y - 1].x);
                 }
             }
-            const int total = load_a_chunk(g, c0, n, s_ab, s_b0, s_pre, s_tmp);
+            const int total = load_slab_chunk(g, c0, n, s_ab, s_b0, s_pre, s_tmp);
             for (int p = tid; p < total; p += kThreads) {
                 const int ai = upper_index(s_pre, n, p);
                 const int4 bm = g.b_meta[s_b0[ai] + (p - s_pre[ai])];
@@ -508,25 +595,26 @@ __device__ __forceinline__ void accumulate_pair(
     }
 }
 
-// A rows must fit the LDS budgets (tiles, accumulator, staged A values); the
-// instance launched over every row skips the others, which the host routes
-// to the larger instance or to the slab path.
-template <int ACC, int AST>
+// Rows must fit the instance's LDS budgets; the instance launched over every
+// row skips the others, which the host routes to the larger instance or to
+// the slab path.
+template <class Cfg>
 __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
-    Operands g, const BRecord *__restrict__ recs, const int *__restrict__ rows, int nrows,
+    Operands g, const BRecord *__restrict__ recs, const int *__restrict__ rows,
     const int *__restrict__ row_level, COut c)
 {
-    __shared__ double s_acc[ACC];
-    __shared__ double s_aval[AST];
-    __shared__ unsigned long long s_cmask[kNumTiles];
-    __shared__ unsigned short s_cbase[kNumTiles];
-    __shared__ int s_hk[kNumSlots];
-    __shared__ unsigned short s_hq[kNumSlots];
+    __shared__ double s_acc[Cfg::kAcc];
+    __shared__ double s_aval[Cfg::kAStage];
+    __shared__ unsigned long long s_cmask[Cfg::kTiles];
+    __shared__ unsigned short s_cbase[Cfg::kTiles];
+    __shared__ int s_hk[Cfg::kSlots];
+    __shared__ unsigned short s_hq[Cfg::kSlots];
     __shared__ uint32_t s_ab[kChunk];
     __shared__ uint32_t s_am[kChunk];
     __shared__ uint32_t s_ao[kChunk];
     __shared__ int s_b0[kChunk];
     __shared__ int s_pre[kChunk];
+    __shared__ unsigned char s_seg[kSeg];
     __shared__ int s_tmp[kWavesPerBlock];
 
     const int tid = threadIdx.x;
@@ -540,9 +628,9 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
     const int a_base = g.a_off[a0];
     const int a_len = g.a_off[a1] - a_base;
-    if (nt > kNumTiles || nnz_row > ACC || a_len > AST)
+    if (nt > Cfg::kTiles || nnz_row > Cfg::kAcc || a_len > Cfg::kAStage)
         return;
-    for (int i = tid; i < kNumSlots; i += kThreads)
+    for (int i = tid; i < Cfg::kSlots; i += kThreads)
         s_hk[i] = kEmpty;
     for (int i = tid; i < nnz_row; i += kThreads)
         s_acc[i] = 0.0;
@@ -553,9 +641,9 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
         const int j = c.col[t0 + q];
         s_cmask[q] = c.mask[t0 + q];
         s_cbase[q] = static_cast<unsigned short>(c.tile_nnz[t0 + q] - zb);
-        uint32_t h = slot_of(j, kNumSlotBits);
+        uint32_t h = slot_of(j, Cfg::kSlotBits);
         while (atomicCAS(&s_hk[h], kEmpty, j) != kEmpty)
-            h = (h + 1) & (kNumSlots - 1);
+            h = (h + 1) & (Cfg::kSlots - 1);
         s_hq[h] = static_cast<unsigned short>(q);
     }
     for (int c0 = a0; c0 < a1; c0 += kChunk) {
@@ -563,69 +651,75 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
         int nb = 0;
         if (tid < n) {
             const int4 m = g.a_meta[c0 + tid];
-            const int b0 = g.b_trp[m.x];
-            nb = g.b_trp[m.x + 1] - b0;
+            const int2 br = g.a_brange[c0 + tid];
+            nb = br.y;
             s_ab[tid] = static_cast<uint32_t>(m.z);
             s_am[tid] = static_cast<uint32_t>(m.y);
             const uint32_t off = static_cast<uint32_t>(m.w);
             s_ao[tid] = ((off & ~kDenseBit) - static_cast<uint32_t>(a_base)) | (off & kDenseBit);
-            s_b0[tid] = b0;
+            s_b0[tid] = br.x;
         }
         int total = 0;
-        const int excl = block_exclusive_scan(nb, s_tmp, &total);
+        const int pre = block_exclusive_scan(nb, s_tmp, &total);
         if (tid < n)
-            s_pre[tid] = excl;
-        __syncthreads();
-        for (int p0 = tid; p0 < total; p0 += kNumBatch * kThreads) {
-            int ai[kNumBatch];
-            int4 meta[kNumBatch];
-            double v[kNumBatch][kRecVals];
+            s_pre[tid] = pre;
+        for (int r0 = 0; r0 < total; r0 += kSeg) {
+            const int r1 = min(total, r0 + kSeg);
+            if (tid < n)
+                fill_segments(s_seg, pre, nb, r0, r1);
+            __syncthreads();
+            for (int p0 = r0 + tid; p0 < r1; p0 += kNumBatch * kThreads) {
+                int ai[kNumBatch];
+                int4 meta[kNumBatch];
+                double v[kNumBatch][kRecVals];
 #pragma unroll
-            for (int u = 0; u < kNumBatch; ++u) {
-                const int p = p0 + u * kThreads;
-                ai[u] = -1;
-                meta[u] = make_int4(0, 0, 0, 0);
-#pragma unroll
-                for (int j = 0; j < kRecVals; ++j)
-                    v[u][j] = 0.0;
-                if (p < total) {
-                    ai[u] = upper_index(s_pre, n, p);
-                    const BRecord *rp = recs + s_b0[ai[u]] + (p - s_pre[ai[u]]);
-                    meta[u] = rp->meta;
+                for (int u = 0; u < kNumBatch; ++u) {
+                    const int p = p0 + u * kThreads;
+                    ai[u] = -1;
+                    meta[u] = make_int4(0, 0, 0, 0);
 #pragma unroll
                     for (int j = 0; j < kRecVals; ++j)
-                        v[u][j] = rp->v[j];
-                }
-            }
+                        v[u][j] = 0.0;
+                    if (p < r1) {
+                        ai[u] = s_seg[p - r0];
+                        const BRecord *rp = recs + s_b0[ai[u]] + (p - s_pre[ai[u]]);
+                        meta[u] = rp->meta;
 #pragma unroll
-            for (int u = 0; u < kNumBatch; ++u) {
-                if (ai[u] < 0)
-                    continue;
-                const uint32_t ab = s_ab[ai[u]];
-                if (!product_count(ab, static_cast<uint32_t>(meta[u].z)))
-                    continue;
-                // Every tile of the row is in the table and at least half of
-                // its slots are empty, so the probe ends.
-                uint32_t h = slot_of(meta[u].x, kNumSlotBits);
-                int q = -1;
-                for (;;) {
-                    const int k = s_hk[h];
-                    if (k == meta[u].x) {
-                        q = s_hq[h];
-                        break;
+                        for (int j = 0; j < kRecVals; ++j)
+                            v[u][j] = rp->v[j];
                     }
-                    if (k == kEmpty)
-                        break;
-                    h = (h + 1) & (kNumSlots - 1);
                 }
-                if (q < 0)
-                    continue;
-                accumulate_pair(g, s_acc, s_aval, ab, s_am[ai[u]], s_ao[ai[u]], meta[u], v[u],
-                                s_cmask[q], s_cbase[q]);
+#pragma unroll
+                for (int u = 0; u < kNumBatch; ++u) {
+                    if (ai[u] < 0)
+                        continue;
+                    const uint32_t ab = s_ab[ai[u]];
+                    if (!product_count(ab, static_cast<uint32_t>(meta[u].z)))
+                        continue;
+                    // Every tile of the row is in the table and at least half
+                    // of its slots are empty, so the probe ends.
+                    uint32_t h = slot_of(meta[u].x, Cfg::kSlotBits);
+                    int q = -1;
+                    for (;;) {
+                        const int k = s_hk[h];
+                        if (k == meta[u].x) {
+                            q = s_hq[h];
+                            break;
+                        }
+                        if (k == kEmpty)
+                            break;
+                        h = (h + 1) & (Cfg::kSlots - 1);
+                    }
+                    if (q < 0)
+                        continue;
+                    accumulate_pair(g, s_acc, s_aval, ab, s_am[ai[u]], s_ao[ai[u]], meta[u],
+                                    v[u], s_cmask[q], s_cbase[q]);
+                }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
+    __syncthreads();
     for (int64_t i = tid; i < nnz_row; i += kThreads)
         c.val[zb + i] = s_acc[i];
 }
@@ -658,7 +752,7 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
         const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
         for (int c0 = a0; c0 < a1; c0 += kThreads) {
             const int n = min(kThreads, a1 - c0);
-            const int total = load_a_chunk(g, c0, n, s_ab, s_b0, s_pre, s_tmp);
+            const int total = load_slab_chunk(g, c0, n, s_ab, s_b0, s_pre, s_tmp);
             for (int p = tid; p < total; p += kThreads) {
                 const int ai = upper_index(s_pre, n, p);
                 const int4 bm = g.b_meta[s_b0[ai] + (p - s_pre[ai])];
@@ -743,9 +837,9 @@ __global__ void collect_rows_kernel(int n, int mode, const int *__restrict__ lev
     const int64_t nt = tile_ptr[i + 1] - tile_ptr[i];
     const int64_t nz = nnz_off[i + 1] - nnz_off[i];
     const int64_t alen = a_off[a_trp[i + 1]] - a_off[a_trp[i]];
-    if (nt > kNumTiles || nz > kAccLarge || alen > kAStageLarge)
+    if (nt > LargeRow::kTiles || nz > LargeRow::kAcc || alen > LargeRow::kAStage)
         out_slab[atomicAdd(n_slab, 1)] = i;
-    else if (nz > kAccSmall || alen > kAStageSmall)
+    else if (nt > SmallRow::kTiles || nz > SmallRow::kAcc || alen > SmallRow::kAStage)
         out_lds[atomicAdd(n_lds, 1)] = i;
 }
 
@@ -764,13 +858,6 @@ __global__ void fill_u64_kernel(unsigned long long *p, int64_t n)
 }
 
 __global__ void set_last_kernel(int64_t *p, const int64_t *v) { *p = *v; }
-
-int read_scalar(const void *d, void *h, size_t bytes, hipStream_t s)
-{
-    RTT_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s));
-    RTT_HIP(hipStreamSynchronize(s));
-    return RTT_OK;
-}
 
 double event_ms(hipEvent_t a, hipEvent_t b)
 {
@@ -801,6 +888,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     g.a_tile_rows = R;
     g.a_trp = a.tile_row_ptr.as<int>();
     g.a_meta = a.meta.as<int4>();
+    g.a_brange = b.a_brange.as<int2>();
     g.a_off = a.value_offsets.as<int>();
     g.a_vals = a.values.as<double>();
     g.b_trp = b.tile_row_ptr.as<int>();
@@ -809,24 +897,31 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     g.b_tile_cols = BC;
 
     const int num_cus = ctx().num_cus > 0 ? ctx().num_cus : 256;
+    // Counters read back by the host: [0] pairs, [1] symbolic slab rows,
+    // [2] numeric slab rows, [3] numeric large-LDS rows, [4] C tiles,
+    // [5] nnz(C).
+    RTT_TRY(w->counters.ensure(8 * sizeof(unsigned long long)));
+    RTT_TRY(w->host_counters.ensure(8 * sizeof(unsigned long long)));
+    unsigned long long *ctr = w->counters.as<unsigned long long>();
+    unsigned long long *hctr = w->host_counters.as<unsigned long long>();
+    int *slab_count = reinterpret_cast<int *>(ctr + 1);
+    int *num_slab_count = reinterpret_cast<int *>(ctr + 2);
+    int *num_lds_count = reinterpret_cast<int *>(ctr + 3);
+
     auto t_sym = Clock::now();
     // ---- symbolic: per-row tile sets, staged ------------------------------
     const int stage_limit = 3 * kSymSlots / 4;
     RTT_TRY(w->row_tiles.ensure((R + 1) * sizeof(int64_t)));
     RTT_TRY(w->row_nnz.ensure((R + 1) * sizeof(int64_t)));
     RTT_TRY(w->row_level.ensure((R + 1) * sizeof(int)));
-    RTT_TRY(w->counters.ensure(8 * sizeof(unsigned long long)));
     RTT_TRY(w->l3_rows.ensure((R + 1) * sizeof(int)));
+    RTT_TRY(w->num_rows.ensure((R + 1) * sizeof(int)));
     RTT_TRY(w->lds_rows.ensure((R + 1) * sizeof(int)));
     RTT_TRY(w->stage_j.ensure(static_cast<size_t>(R) * stage_limit * sizeof(int)));
     RTT_TRY(w->stage_m.ensure(static_cast<size_t>(R) * stage_limit * sizeof(uint64_t)));
     RTT_TRY(c->tile_ptr.ensure((R + 1) * sizeof(int64_t)));
     RTT_TRY(w->c_row_nnz_off.ensure((R + 1) * sizeof(int64_t)));
     Stage st{w->stage_j.as<int>(), w->stage_m.as<unsigned long long>(), stage_limit};
-    unsigned long long *ctr = w->counters.as<unsigned long long>();
-    int *slab_count = reinterpret_cast<int *>(ctr + 1);
-    int *num_slab_count = reinterpret_cast<int *>(ctr + 2);
-    int *num_lds_count = reinterpret_cast<int *>(ctr + 3);
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(8), 0, s, ctr, int64_t(8));
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, s,
                        reinterpret_cast<unsigned long long *>(w->row_tiles.as<int64_t>() + R),
@@ -847,7 +942,10 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
         hipLaunchKernelGGL(collect_rows_kernel, dim3(blocks_for(R, kThreads)), dim3(kThreads), 0,
                            s, R, 0, w->row_level.as<int>(), nullptr, nullptr, nullptr, nullptr,
                            w->l3_rows.as<int>(), slab_count, nullptr, nullptr);
-        RTT_TRY(read_scalar(slab_count, &n_slab, sizeof(int), s));
+        RTT_HIP(hipMemcpyAsync(hctr, ctr, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                               s));
+        RTT_HIP(hipStreamSynchronize(s));
+        n_slab = static_cast<int>(hctr[1] & 0xffffffffu);
     }
     int slab_blocks = 0;
     if (n_slab > 0) {
@@ -866,17 +964,31 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
                                &w->scan_tmp, s));
     RTT_TRY(exclusive_scan_i64(w->row_nnz.as<int64_t>(), w->c_row_nnz_off.as<int64_t>(), R,
                                &w->scan_tmp, s));
-    int64_t totals[2] = {0, 0};
-    RTT_TRY(read_scalar(c->tile_ptr.as<int64_t>() + R, &totals[0], sizeof(int64_t), s));
-    RTT_TRY(read_scalar(w->c_row_nnz_off.as<int64_t>() + R, &totals[1], sizeof(int64_t), s));
-    unsigned long long pairs = 0;
-    RTT_TRY(read_scalar(ctr, &pairs, sizeof(pairs), s));
+    // Numeric row routing needs only the per-row sizes: list the fallback
+    // rows now and read every count back with one copy.
+    if (R > 0)
+        hipLaunchKernelGGL(collect_rows_kernel, dim3(blocks_for(R, kThreads)), dim3(kThreads), 0,
+                           s, R, 1, w->row_level.as<int>(), c->tile_ptr.as<int64_t>(),
+                           w->c_row_nnz_off.as<int64_t>(), g.a_trp, g.a_off,
+                           w->num_rows.as<int>(), num_slab_count, w->lds_rows.as<int>(),
+                           num_lds_count);
+    hipLaunchKernelGGL(set_last_kernel, dim3(1), dim3(1), 0, s,
+                       reinterpret_cast<int64_t *>(ctr + 4), c->tile_ptr.as<int64_t>() + R);
+    hipLaunchKernelGGL(set_last_kernel, dim3(1), dim3(1), 0, s,
+                       reinterpret_cast<int64_t *>(ctr + 5), w->c_row_nnz_off.as<int64_t>() + R);
+    RTT_HIP(hipMemcpyAsync(hctr, ctr, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    RTT_HIP(hipStreamSynchronize(s));
+    const unsigned long long pairs = hctr[0];
+    const int n_num_slab = static_cast<int>(hctr[2] & 0xffffffffu);
+    const int n_num_lds = static_cast<int>(hctr[3] & 0xffffffffu);
+    const int64_t total_tiles = static_cast<int64_t>(hctr[4]);
+    const int64_t total_nnz = static_cast<int64_t>(hctr[5]);
     const double sym_count_ms = ms_since(t_sym);
 
     // ---- allocate C -----------------------------------------------------
     auto t_alloc = Clock::now();
-    c->num_tiles = totals[0];
-    c->nnz = totals[1];
+    c->num_tiles = total_tiles;
+    c->nnz = total_nnz;
     RTT_TRY(c->tile_col.ensure(c->num_tiles * sizeof(int)));
     RTT_TRY(c->tile_mask.ensure(c->num_tiles * sizeof(uint64_t)));
     RTT_TRY(c->tile_nnz.ensure((c->num_tiles + 1) * sizeof(int64_t)));
@@ -910,49 +1022,38 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
                        c->tile_nnz.as<int64_t>() + c->num_tiles,
                        w->c_row_nnz_off.as<int64_t>() + R);
     RTT_HIP(hipGetLastError());
-    RTT_HIP(hipStreamSynchronize(s));
-    times->symbolic_ms = sym_count_ms + ms_since(t_fill);
-
+    RTT_HIP(hipEventRecord(ev->ev[6], s));
     // ---- numeric ----------------------------------------------------------
-    auto t_num = Clock::now();
-    int n_num_slab = 0, n_num_lds = 0;
-    if (R > 0) {
-        hipLaunchKernelGGL(collect_rows_kernel, dim3(blocks_for(R, kThreads)), dim3(kThreads), 0,
-                           s, R, 1, w->row_level.as<int>(), c->tile_ptr.as<int64_t>(),
-                           w->c_row_nnz_off.as<int64_t>(), g.a_trp, g.a_off,
-                           w->l3_rows.as<int>(), num_slab_count, w->lds_rows.as<int>(),
-                           num_lds_count);
-        int counts[2] = {0, 0};
-        RTT_TRY(read_scalar(num_slab_count, &counts[0], sizeof(int), s));
-        RTT_TRY(read_scalar(num_lds_count, &counts[1], sizeof(int), s));
-        n_num_slab = counts[0];
-        n_num_lds = counts[1];
-    }
     RTT_HIP(hipEventRecord(ev->ev[4], s));
     if (R > 0 && c->nnz > 0)
-        hipLaunchKernelGGL((numeric_row_kernel<kAccSmall, kAStageSmall>), dim3(R), dim3(kThreads),
-                           0, s, g, b.records.as<BRecord>(), nullptr, R, w->row_level.as<int>(),
-                           co);
+        hipLaunchKernelGGL((numeric_row_kernel<SmallRow>), dim3(R), dim3(kThreads), 0, s, g,
+                           b.records.as<BRecord>(), nullptr, w->row_level.as<int>(), co);
     RTT_HIP(hipGetLastError());
     RTT_HIP(hipEventRecord(ev->ev[5], s));
     if (n_num_lds > 0)
-        hipLaunchKernelGGL((numeric_row_kernel<kAccLarge, kAStageLarge>), dim3(n_num_lds),
-                           dim3(kThreads), 0, s, g, b.records.as<BRecord>(),
-                           w->lds_rows.as<int>(), n_num_lds, w->row_level.as<int>(), co);
+        hipLaunchKernelGGL((numeric_row_kernel<LargeRow>), dim3(n_num_lds), dim3(kThreads), 0, s,
+                           g, b.records.as<BRecord>(), w->lds_rows.as<int>(),
+                           w->row_level.as<int>(), co);
     if (n_num_slab > 0) {
         const int blocks = std::min(n_num_slab, 2 * num_cus);
-        DevBuf &idx = w->slab;   // reused: numeric runs after symbolic finished
+        DevBuf &idx = w->slab;   // reused: the symbolic slab pass has been queued before
         RTT_TRY(idx.ensure(static_cast<size_t>(blocks) * BC * sizeof(int)));
         hipLaunchKernelGGL(fill_i32_kernel,
                            dim3(blocks_for(static_cast<int64_t>(blocks) * BC, kThreads)),
                            dim3(kThreads), 0, s, idx.as<int>(), static_cast<int64_t>(blocks) * BC,
                            -1);
         hipLaunchKernelGGL(numeric_slab_kernel, dim3(blocks), dim3(kThreads), 0, s, g,
-                           w->l3_rows.as<int>(), n_num_slab, idx.as<int>(), co);
+                           w->num_rows.as<int>(), n_num_slab, idx.as<int>(), co);
         RTT_HIP(hipGetLastError());
     }
+    RTT_HIP(hipEventRecord(ev->ev[7], s));
     RTT_HIP(hipStreamSynchronize(s));
-    times->numeric_ms = ms_since(t_num);
+    // Structure write and numeric share the stream without a host wait
+    // between them; their split comes from the device events.
+    const double fill_ms = event_ms(ev->ev[2], ev->ev[6]);
+    const double tail_ms = ms_since(t_fill);
+    times->symbolic_ms = sym_count_ms + fill_ms;
+    times->numeric_ms = std::max(0.0, tail_ms - fill_ms);
     times->sym_count_kernel_ms = event_ms(ev->ev[0], ev->ev[1]);
     times->sym_fill_kernel_ms = event_ms(ev->ev[2], ev->ev[3]);
     times->numeric_kernel_ms = event_ms(ev->ev[4], ev->ev[5]);
