@@ -85,6 +85,40 @@ def cpu_baseline(a, aat, nnz_cub, budget_s=12.0):
                       f"workload, {dt:.2f} s, OpenMP {threads} threads"}
 
 
+# Kernel names in the rocprofv3 counter summaries, per breakdown key.
+PROFILE_KERNELS = {
+    "numeric": "numeric_row_kernel<SmallRow",
+    "symbolic_count": "symbolic_count_kernel",
+    "symbolic_fill": "symbolic_write_kernel",
+}
+
+
+def profile_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload: FETCH_SIZE and WRITE_SIZE (KiB per dispatch),
+    each scaled by what the counter reads for a device copy of known size.
+    None when no summary covers the kernel."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"{workload}_counters.md")
+    if not os.path.exists(path):
+        return None, None
+    text = open(path).read()
+    m = re.search(r"FETCH_SIZE x 1024 x ([0-9]+(?:\.[0-9]+)?) and WRITE_SIZE x 1024 x ([0-9]+(?:\.[0-9]+)?)", text)
+    if not m:
+        return None, None
+    sf, sw = float(m.group(1)), float(m.group(2))
+    name = PROFILE_KERNELS.get(kernel)
+    for line in text.splitlines():
+        cells = [c.strip(" `") for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 3 and name and cells[0].startswith(name):
+            try:
+                fetch_kib, write_kib = float(cells[1]), float(cells[2])
+            except ValueError:
+                continue
+            return fetch_kib * 1024 * sf + write_kib * 1024 * sw, os.path.relpath(path, ROOT)
+    return None, None
+
+
 def copy_calibration(nbytes):
     """One device-to-device copy of nbytes through the HIP runtime: the plain
     copy kernel whose FETCH_SIZE / WRITE_SIZE counters scale the traffic
@@ -170,6 +204,7 @@ def main():
     }
     dom = max(kern, key=kern.get)
     achieved = alg[dom] / (kern[dom] * 1e-3) / 1e9
+    traffic, traffic_src = profile_traffic(args.workload, dom)
     b_alg = csr_bytes(a.m, a.nnz) + csr_bytes(m_b, a.nnz) + csr_bytes(a.m, nnz_c)
     breakdown = {
         "b_update_ms": float(np.median([s[1]["total_ms"] for s in steps])),
@@ -204,7 +239,10 @@ def main():
                                               5)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None if traffic is None else int(traffic),
+                     "algorithmic_bytes": int(alg[dom]),
+                     "traffic_source": traffic_src},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

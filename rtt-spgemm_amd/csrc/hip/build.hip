@@ -296,12 +296,26 @@ __global__ void records_kernel(int64_t n, const int4 *__restrict__ meta,
     write_record_part(recs, t, static_cast<int>(gid % 5), meta[t], payload);
 }
 
-// B = A^T: B tile b is A tile ids[b]; its meta is A's with the tile column
-// replaced by A's tile row, and the record's values come from A's payload.
+// Offline, per A tile: the record a B tile built as A^T takes over, its meta
+// with the tile column replaced by the tile row.
+__global__ void transpose_source_kernel(int64_t n, const int *__restrict__ a_row_of,
+                                        const int4 *__restrict__ a_meta,
+                                        const double *__restrict__ a_vals,
+                                        BRecord *__restrict__ out)
+{
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t t = gid / 5;
+    if (t >= n)
+        return;
+    int4 m = a_meta[t];
+    m.x = a_row_of[t];
+    write_record_part(out, t, static_cast<int>(gid % 5), m, a_vals);
+}
+
+// B = A^T: B tile b is A tile ids[b], so its record is a copy of that A
+// tile's transpose record, one 16-byte part per thread.
 __global__ void transposed_records_kernel(int64_t n, const int *__restrict__ ids,
-                                          const int *__restrict__ a_row_of,
-                                          const int4 *__restrict__ a_meta,
-                                          const double *__restrict__ a_vals,
+                                          const BRecord *__restrict__ src,
                                           int4 *__restrict__ b_meta, BRecord *__restrict__ recs)
 {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -309,13 +323,10 @@ __global__ void transposed_records_kernel(int64_t n, const int *__restrict__ ids
     if (t >= n)
         return;
     const int part = static_cast<int>(gid % 5);
-    const int a = ids[t];
-    int4 m = a_meta[a];
-    if (part == 0) {
-        m.x = a_row_of[a];
-        b_meta[t] = m;
-    }
-    write_record_part(recs, t, part, m, a_vals);
+    const int4 v = reinterpret_cast<const int4 *>(src + ids[t])[part];
+    reinterpret_cast<int4 *>(recs + t)[part] = v;
+    if (part == 0)
+        b_meta[t] = v;
 }
 
 __global__ void pair_extent_kernel(int64_t n, const int4 *__restrict__ a_meta,
@@ -358,12 +369,13 @@ __global__ void nnz_cub_kernel(int64_t nnz, const int *__restrict__ ci,
                                unsigned long long *__restrict__ out)
 {
     __shared__ unsigned long long part[kThreads / kWave];
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     unsigned long long v = 0;
-    if (i < nnz) {
+    // Grid-stride, so that only a few hundred workgroups add into *out.
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < nnz;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const int c = ci[i];
-        v = coldeg ? static_cast<unsigned long long>(coldeg[c])
-                   : static_cast<unsigned long long>(rp[c + 1] - rp[c]);
+        v += coldeg ? static_cast<unsigned long long>(coldeg[c])
+                    : static_cast<unsigned long long>(rp[c + 1] - rp[c]);
     }
     v = wave_sum(v);
     if (lane_id() == 0)
@@ -567,6 +579,63 @@ int build_pair_extents(const DevTiles &a, DevTiles *b)
     return RTT_OK;
 }
 
+// Thread per A tile: its payload re-packed column-major.  Entry (r, k) goes
+// to off + (rows of columns < k) + (rows < r in column k).
+__global__ void colmajor_values_kernel(int64_t n, const int4 *__restrict__ meta,
+                                       const double *__restrict__ vals, double *__restrict__ out)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    const int4 m = meta[t];
+    const uint32_t rb = static_cast<uint32_t>(m.z);
+    const uint32_t w = static_cast<uint32_t>(m.w);
+    const bool dense = (w & 0x80000000u) != 0;
+    const uint32_t off = w & 0x7FFFFFFFu;
+    int start[4];
+    int acc = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        start[k] = acc;
+        acc += __popc((rb >> (8 * k)) & 0xFFu);
+    }
+    uint32_t mm = static_cast<uint32_t>(m.y);
+    int j = 0;
+    while (mm) {
+        const int phys = __builtin_ctz(mm);
+        mm &= mm - 1;
+        const int r = phys >> 2, k = phys & 3;
+        const uint32_t col_rows = (rb >> (8 * k)) & 0xFFu;
+        out[off + start[k] + __popc(col_rows & ((1u << r) - 1u))] = vals[off + (dense ? phys : j)];
+        ++j;
+    }
+}
+
+int build_transpose_source(DevTiles *t)
+{
+    RTT_TRY(t->transpose_src.ensure(static_cast<size_t>(t->num_tiles > 0 ? t->num_tiles : 1) *
+                                    sizeof(BRecord)));
+    if (t->num_tiles > 0)
+        hipLaunchKernelGGL(transpose_source_kernel, dim3(blocks_for(t->num_tiles * 5, kThreads)),
+                           dim3(kThreads), 0, ctx().stream, t->num_tiles,
+                           t->tile_row_of.as<int>(), t->meta.as<int4>(), t->values.as<double>(),
+                           t->transpose_src.as<BRecord>());
+    RTT_HIP(hipGetLastError());
+    return RTT_OK;
+}
+
+int build_colmajor_values(DevTiles *t)
+{
+    RTT_TRY(t->values_cm.ensure(static_cast<size_t>(t->payload > 0 ? t->payload : 1) *
+                                sizeof(double)));
+    if (t->num_tiles > 0)
+        hipLaunchKernelGGL(colmajor_values_kernel, dim3(blocks_for(t->num_tiles, kThreads)),
+                           dim3(kThreads), 0, ctx().stream, t->num_tiles, t->meta.as<int4>(),
+                           t->values.as<double>(), t->values_cm.as<double>());
+    RTT_HIP(hipGetLastError());
+    return RTT_OK;
+}
+
 int fill_tile_row_of(DevTiles *t)
 {
     RTT_TRY(t->tile_row_of.ensure(t->num_tiles * sizeof(int)));
@@ -717,8 +786,8 @@ int transpose_tiles(const DevTiles &a, int b_cols, DevTiles *b, BuildWork *w)
         RTT_TRY(b->records.ensure(n * sizeof(BRecord)));
         hipLaunchKernelGGL(transposed_records_kernel, dim3(blocks_for(n * 5, kThreads)),
                            dim3(kThreads), 0, s, n, b->sorted_ids.as<int>(),
-                           a.tile_row_of.as<int>(), a.meta.as<int4>(), a.values.as<double>(),
-                           b->meta.as<int4>(), b->records.as<BRecord>());
+                           a.transpose_src.as<BRecord>(), b->meta.as<int4>(),
+                           b->records.as<BRecord>());
     }
     hipLaunchKernelGGL(row_ptr_from_sorted_kernel, dim3(blocks_for(n + 1, kThreads)),
                        dim3(kThreads), 0, s, w->keys_out.as<int>(), n, R,
@@ -855,7 +924,9 @@ int nnz_cub(const DevCsr &csr, bool aat, unsigned long long *result, BuildWork *
         deg = d;
     }
     if (csr.nnz > 0)
-        hipLaunchKernelGGL(nnz_cub_kernel, dim3(blocks_for(csr.nnz, kThreads)), dim3(kThreads),
+        hipLaunchKernelGGL(nnz_cub_kernel,
+                           dim3(static_cast<unsigned>(std::min<int64_t>(1024, blocks_for(csr.nnz, kThreads)))),
+                           dim3(kThreads),
                            0, s, csr.nnz, csr.col_idx.as<int>(), csr.row_ptr.as<int>(), deg,
                            acc);
     RTT_HIP(hipGetLastError());

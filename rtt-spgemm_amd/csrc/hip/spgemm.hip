@@ -26,6 +26,8 @@
 //    is finally streamed to HBM.  Rows that exceed the LDS budgets fall back
 //    to a workgroup path with FP64 atomics in HBM.
 #include <climits>
+#include <cstdio>
+#include <cstdlib>
 
 #include "device.hpp"
 #include "internal.hpp"
@@ -45,15 +47,21 @@ constexpr int kSymSlotBits = 10;
 constexpr int kSymPerThread = kSymSlots / kThreads;
 constexpr int kSymBatch = 4;       // pairs whose loads a thread issues together
 
-constexpr int kNumBatch = 2;       // pairs whose records a thread loads together
+constexpr int kNumChunk = 64;      // A tiles per numeric chunk
+constexpr int kNumBatch = 2;       // records a thread loads together (unsorted numeric)
+constexpr int kNumWin = 1024;      // pairs per numeric window
+constexpr int kPerThread = kNumWin / kThreads;
+constexpr int kClasses = 6;        // numeric pair classes: products 1, 2-3, 4-7, 8-15, 16-31, 32+
 
 // LDS budgets of the two numeric instances: accumulator slots, staged A
-// values, C tiles, hash slots.
+// values, C tiles, hash slots (at least twice the tiles).
 struct SmallRow {
-    static constexpr int kAcc = 1024, kAStage = 256, kTiles = 256, kSlots = 512, kSlotBits = 9;
+    static constexpr int kAcc = 1024, kAStage = 256, kTiles = 128, kSlots = 256, kSlotBits = 8;
+    using QType = unsigned char;
 };
 struct LargeRow {
     static constexpr int kAcc = 4096, kAStage = 2048, kTiles = 512, kSlots = 1024, kSlotBits = 10;
+    using QType = unsigned short;
 };
 
 constexpr int kWriteBuf = 2048;    // column bytes staged per row in the write pass
@@ -70,6 +78,7 @@ struct Operands {
     const int2 *__restrict__ a_brange;   // per A tile: first B tile, count
     const int *__restrict__ a_off;
     const double *__restrict__ a_vals;
+    const double *__restrict__ a_vals_cm;   // A payload, each tile column-major
     const int *__restrict__ b_trp;
     const int4 *__restrict__ b_meta;
     const double *__restrict__ b_vals;
@@ -92,6 +101,7 @@ struct Stage {
     unsigned long long *__restrict__ m;
     int stride;
 };
+
 
 __device__ __forceinline__ uint32_t slot_of(int j, int bits)
 {
@@ -195,10 +205,9 @@ __device__ __forceinline__ void fill_segments(unsigned char *s_seg, int pre, int
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
     Operands g, int stage_limit, Stage st, int64_t *__restrict__ row_tiles,
-    int64_t *__restrict__ row_nnz, int *__restrict__ row_level,
-    unsigned long long *__restrict__ pair_counter)
+    int64_t *__restrict__ row_nnz, int *__restrict__ row_level)
 {
-    __shared__ int s_key[kSymSlots];
+    __shared__ __align__(16) int s_key[kSymSlots];
     __shared__ unsigned long long s_msk[kSymSlots];
     __shared__ unsigned long long s_spread[4][kChunk];
     __shared__ int s_b0[kChunk];
@@ -206,9 +215,9 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
     __shared__ unsigned char s_seg[kSeg];
     __shared__ int s_tmp[kWavesPerBlock];
     __shared__ long long s_tmp64[kWavesPerBlock];
-    __shared__ int s_cnt, s_ovf;
+    __shared__ int s_cnt, s_ovf, s_jmin, s_jmax;
 
-    const int row = blockIdx.x;
+    const int row = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
     for (int i = tid; i < kSymSlots; i += kThreads) {
         s_key[i] = kEmpty;
@@ -217,9 +226,10 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
     if (tid == 0) {
         s_cnt = 0;
         s_ovf = 0;
+        s_jmin = INT_MAX;
+        s_jmax = INT_MIN;
     }
     const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
-    long long pairs = 0;
     for (int c0 = a0; c0 < a1; c0 += kChunk) {
         const int n = min(kChunk, a1 - c0);
         int nb = 0;
@@ -293,7 +303,6 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
             }
             __syncthreads();
         }
-        pairs += total;
         if (s_ovf)
             break;
     }
@@ -320,23 +329,70 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
     int dummy = 0;
     int pos = block_exclusive_scan(mine, s_tmp, &dummy);
     long long nz = 0;
+    int jlo = INT_MAX, jhi = INT_MIN;
 #pragma unroll
     for (int i = 0; i < kSymPerThread; ++i)
         if (rk[i] != kEmpty) {
             s_key[pos] = rk[i];
             s_msk[pos] = rm[i];
             nz += __popcll(rm[i]);
+            jlo = min(jlo, rk[i]);
+            jhi = max(jhi, rk[i]);
             ++pos;
         }
+    if (mine) {
+        atomicMin(&s_jmin, jlo);
+        atomicMax(&s_jmax, jhi);
+    }
     __syncthreads();
     const int64_t sb = static_cast<int64_t>(row) * st.stride;
-    if (nt <= kThreads) {
+    const int jmin = s_jmin;
+    const int64_t span = nt > 0 ? static_cast<int64_t>(s_jmax) - jmin + 1 : 0;
+    if (nt > 0 && span <= 32 * kThreads) {
+        // Rank by a bitmap of the row's column span: a word prefix count
+        // plus the set bits below the column in its word.  s_spread is free
+        // once the pairs are done and holds the bitmap and the word prefixes.
+        uint32_t *bits = reinterpret_cast<uint32_t *>(&s_spread[0][0]);
+        const int words = static_cast<int>((span + 31) >> 5);
+        if (tid < words)
+            bits[tid] = 0u;
+        __syncthreads();
+        for (int e = tid; e < nt; e += kThreads) {
+            const int rel = s_key[e] - jmin;
+            atomicOr(&bits[rel >> 5], 1u << (rel & 31));
+        }
+        __syncthreads();
+        const int pc = tid < words ? __popc(bits[tid]) : 0;
+        int all = 0;
+        const int wpre = block_exclusive_scan(pc, s_tmp, &all);
+        if (tid < words)
+            bits[kThreads + tid] = static_cast<uint32_t>(wpre);
+        __syncthreads();
+        for (int e = tid; e < nt; e += kThreads) {
+            const int mj = s_key[e];
+            const int rel = mj - jmin;
+            const int rank = static_cast<int>(bits[kThreads + (rel >> 5)]) +
+                             __popc(bits[rel >> 5] & ((1u << (rel & 31)) - 1u));
+            st.j[sb + rank] = mj;
+            st.m[sb + rank] = s_msk[e];
+        }
+    } else if (nt <= kThreads) {
         // Rank by counting: entry t goes to the number of smaller columns.
+        // The keys are read four at a time; the tail up to a multiple of
+        // four holds INT_MAX, which counts as larger than every key.
+        const int nt4 = (nt + 3) & ~3;
+        if (tid < nt4 - nt)
+            s_key[nt + tid] = INT_MAX;
+        __syncthreads();
         if (tid < nt) {
             const int mj = s_key[tid];
+            const int4 *k4 = reinterpret_cast<const int4 *>(s_key);
             int rank = 0;
-            for (int f = 0; f < nt; ++f)
-                rank += s_key[f] < mj ? 1 : 0;
+            for (int f = 0; f < nt4 / 4; ++f) {
+                const int4 kk = k4[f];
+                rank += (kk.x < mj ? 1 : 0) + (kk.y < mj ? 1 : 0) + (kk.z < mj ? 1 : 0) +
+                        (kk.w < mj ? 1 : 0);
+            }
             st.j[sb + rank] = mj;
             st.m[sb + rank] = s_msk[tid];
         }
@@ -376,7 +432,6 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
         row_level[row] = nt > 0 ? kLevelLds : kLevelEmpty;
         row_tiles[row] = nt;
         row_nnz[row] = row_nz;
-        atomicAdd(pair_counter, static_cast<unsigned long long>(pairs));   // same on all threads
     }
 }
 
@@ -387,7 +442,7 @@ __global__ __launch_bounds__(kThreads) void symbolic_write_kernel(
 {
     __shared__ unsigned char s_buf[kWavesPerBlock][kWriteBuf];
     const int w = threadIdx.x / kWave;
-    const int row = blockIdx.x * kWavesPerBlock + w;
+    const int row = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + w;
     const int lane = lane_id();
     if (row >= rows || row_level[row] != kLevelLds)
         return;
@@ -461,8 +516,7 @@ __device__ __forceinline__ int load_slab_chunk(const Operands &g, int c0, int n,
 template <bool FILL>
 __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
     Operands g, const int *__restrict__ rows, int nrows, unsigned long long *__restrict__ slabs,
-    int64_t *__restrict__ row_tiles, int64_t *__restrict__ row_nnz,
-    unsigned long long *__restrict__ pair_counter, COut c)
+    int64_t *__restrict__ row_tiles, int64_t *__restrict__ row_nnz, COut c)
 {
     __shared__ uint32_t s_ab[kThreads];
     __shared__ int s_b0[kThreads];
@@ -479,7 +533,6 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
             s_hi = -1;
         }
         __syncthreads();
-        unsigned long long pairs = 0;
         for (int c0 = a0; c0 < a1; c0 += kThreads) {
             const int n = min(kThreads, a1 - c0);
             if (tid < n) {
@@ -497,7 +550,6 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
                 if (pm)
                     atomicOr(&slab[bm.x], static_cast<unsigned long long>(pm));
             }
-            pairs += total;
             __syncthreads();
         }
         __threadfence();
@@ -533,7 +585,6 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
             if (tid == 0) {
                 row_tiles[row] = tiles;
                 row_nnz[row] = nz;
-                atomicAdd(pair_counter, pairs);
             }
         }
         __threadfence();
@@ -544,81 +595,46 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
 // ---------------------------------------------------------------------------
 // Numeric, LDS path: one workgroup per row, LDS accumulator.
 // ---------------------------------------------------------------------------
-// Adds the products of one (A tile, B tile) pair into the row accumulator.
-// B's nonzeros are walked in payload order (the first kRecVals come from the
-// record in registers); for each, the rows of A's matching column.
-__device__ __forceinline__ void accumulate_pair(
-    const Operands &g, double *s_acc, const double *s_aval, uint32_t ab, uint32_t am, uint32_t ao,
-    const int4 &meta, const double (&v)[kRecVals], uint64_t cm, int cb)
-{
-    const bool adense = (ao & kDenseBit) != 0;
-    const uint32_t aoff = ao & ~kDenseBit;
-    const uint32_t bmask = static_cast<uint32_t>(meta.y);
-    const uint32_t bw = static_cast<uint32_t>(meta.w);
-    const bool bdense = (bw & kDenseBit) != 0;
-    const uint32_t boff = bw & ~kDenseBit;
-    uint32_t mm = bmask;
-    auto column = [&](int pos, double vb) {
-        const int k = pos & 3, cl = pos >> 2;
-        uint32_t rws = byte_of(ab, k);
-        while (rws) {
-            const int r = __builtin_ctz(rws);
-            rws &= rws - 1;
-            const int pa = r * 4 + k;
-            const double va = s_aval[aoff + (adense ? pa : __popc(am & ((1u << pa) - 1u)))];
-            const int pos_c = r * 8 + cl;
-            atomicAdd(&s_acc[cb + __popcll(cm & ((1ull << pos_c) - 1ull))], va * vb);
-        }
-    };
-    if (!bdense) {
-#pragma unroll
-        for (int j = 0; j < kRecVals; ++j) {
-            if (mm) {
-                const int pos = __builtin_ctz(mm);
-                mm &= mm - 1;
-                column(pos, v[j]);
-            }
-        }
-        int j = kRecVals;
-        while (mm) {
-            const int pos = __builtin_ctz(mm);
-            mm &= mm - 1;
-            column(pos, g.b_vals[boff + j]);
-            ++j;
-        }
-    } else {
-        while (mm) {
-            const int pos = __builtin_ctz(mm);
-            mm &= mm - 1;
-            column(pos, g.b_vals[boff + pos]);
-        }
-    }
-}
-
-// Rows must fit the instance's LDS budgets; the instance launched over every
-// row skips the others, which the host routes to the larger instance or to
-// the slab path.
-template <class Cfg>
+// The row's exact output segment is an FP64 accumulator in LDS.  The row's A
+// payload is staged column-major (the rows of column k of a tile are
+// contiguous), every C tile of the row is in a hash table with its mask and
+// value offset, and each scalar product is added into the accumulator slot
+// given by a popcount of the C tile mask.
+//
+// SORT = false: each thread takes the pairs p, p + kThreads, ... of a
+// segment, kNumBatch records loaded together.
+// SORT = true: pairs are staged in windows of kNumWin; phase A loads their B
+// meta, drops pairs without a product, resolves the C tile and ranks each
+// pair in a class given by its product count; a counting sort orders the
+// window by class and phase B processes it in that order, so that the lanes
+// of a wave run pairs of similar length.
+template <class Cfg, bool SORT>
 __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     Operands g, const BRecord *__restrict__ recs, const int *__restrict__ rows,
     const int *__restrict__ row_level, COut c)
 {
+    using QT = typename Cfg::QType;
+    constexpr int kWinP = SORT ? kNumWin : kSeg;
     __shared__ double s_acc[Cfg::kAcc];
-    __shared__ double s_aval[Cfg::kAStage];
+    __shared__ double s_acol[Cfg::kAStage];
     __shared__ unsigned long long s_cmask[Cfg::kTiles];
     __shared__ unsigned short s_cbase[Cfg::kTiles];
     __shared__ int s_hk[Cfg::kSlots];
-    __shared__ unsigned short s_hq[Cfg::kSlots];
-    __shared__ uint32_t s_ab[kChunk];
-    __shared__ uint32_t s_am[kChunk];
-    __shared__ uint32_t s_ao[kChunk];
-    __shared__ int s_b0[kChunk];
-    __shared__ int s_pre[kChunk];
-    __shared__ unsigned char s_seg[kSeg];
+    __shared__ QT s_hq[Cfg::kSlots];
+    __shared__ uint32_t s_ab[kNumChunk];    // A tile: rows present in column k (byte k)
+    __shared__ uint32_t s_apf[kNumChunk];   // A tile: first staged value of column k (byte k)
+    __shared__ int s_acb[kNumChunk];        // A tile: first staged value
+    __shared__ int s_b0[kNumChunk];
+    __shared__ int s_pre[kNumChunk];
+    __shared__ unsigned char s_seg[kWinP];
+    __shared__ QT s_rq[SORT ? kNumWin : 1];
+    __shared__ unsigned short s_order[SORT ? kNumWin : 1];
+    __shared__ int s_ccnt[kClasses];
     __shared__ int s_tmp[kWavesPerBlock];
+    __shared__ int s_tot;
 
     const int tid = threadIdx.x;
-    const int row = rows ? rows[blockIdx.x] : static_cast<int>(blockIdx.x);
+    const int row = rows ? rows[blockIdx.x] : xcd_block(blockIdx.x, gridDim.x);
     if (row_level[row] != kLevelLds)
         return;
     const int64_t t0 = c.tile_ptr[row];
@@ -635,7 +651,7 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     for (int i = tid; i < nnz_row; i += kThreads)
         s_acc[i] = 0.0;
     for (int i = tid; i < a_len; i += kThreads)
-        s_aval[i] = g.a_vals[a_base + i];
+        s_acol[i] = g.a_vals_cm[a_base + i];
     __syncthreads();
     for (int q = tid; q < nt; q += kThreads) {
         const int j = c.col[t0 + q];
@@ -644,76 +660,189 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
         uint32_t h = slot_of(j, Cfg::kSlotBits);
         while (atomicCAS(&s_hk[h], kEmpty, j) != kEmpty)
             h = (h + 1) & (Cfg::kSlots - 1);
-        s_hq[h] = static_cast<unsigned short>(q);
+        s_hq[h] = static_cast<QT>(q);
     }
-    for (int c0 = a0; c0 < a1; c0 += kChunk) {
-        const int n = min(kChunk, a1 - c0);
+    // C tile of column j; every tile of the row is in the table and at least
+    // half of its slots are empty, so the probe ends.
+    auto lookup = [&](int j) {
+        uint32_t h = slot_of(j, Cfg::kSlotBits);
+        for (;;) {
+            const int k = s_hk[h];
+            if (k == j)
+                return static_cast<int>(s_hq[h]);
+            if (k == kEmpty)
+                return -1;
+            h = (h + 1) & (Cfg::kSlots - 1);
+        }
+    };
+    // All products of one pair: B entry (k, cl) in payload order times the
+    // rows of A's column k, two rows per step so that their LDS reads overlap.
+    auto pair_products = [&](int pa, int q, const BRecord &rec) {
+        const uint32_t ab = s_ab[pa];
+        const uint32_t apf = s_apf[pa];
+        const double *acol = s_acol + s_acb[pa];
+        double *acc = s_acc + s_cbase[q];
+        const uint64_t cm = s_cmask[q];
+        const uint32_t bw = static_cast<uint32_t>(rec.meta.w);
+        const uint32_t boff = bw & ~kDenseBit;
+        auto column = [&](int pos, double vb) {
+            const int k = pos & 3, cl = pos >> 2;
+            const double *ap = acol + byte_of(apf, k);
+            uint32_t rws = byte_of(ab, k);
+            while (rws) {
+                const int r1 = __builtin_ctz(rws);
+                rws &= rws - 1;
+                const double v1 = ap[0];
+                const int s1 = __popcll(cm & ((1ull << (r1 * 8 + cl)) - 1ull));
+                if (rws) {
+                    const int r2 = __builtin_ctz(rws);
+                    rws &= rws - 1;
+                    const double v2 = ap[1];
+                    const int s2 = __popcll(cm & ((1ull << (r2 * 8 + cl)) - 1ull));
+                    atomicAdd(&acc[s2], v2 * vb);
+                }
+                atomicAdd(&acc[s1], v1 * vb);
+                ap += 2;
+            }
+        };
+        uint32_t mm = static_cast<uint32_t>(rec.meta.y);
+        if (!(bw & kDenseBit)) {
+#pragma unroll
+            for (int j = 0; j < kRecVals; ++j) {
+                if (mm) {
+                    const int pos = __builtin_ctz(mm);
+                    mm &= mm - 1;
+                    column(pos, rec.v[j]);
+                }
+            }
+            int j = kRecVals;
+            while (mm) {
+                const int pos = __builtin_ctz(mm);
+                mm &= mm - 1;
+                column(pos, g.b_vals[boff + j]);
+                ++j;
+            }
+        } else {
+            while (mm) {
+                const int pos = __builtin_ctz(mm);
+                mm &= mm - 1;
+                column(pos, g.b_vals[boff + pos]);
+            }
+        }
+    };
+    for (int c0 = a0; c0 < a1;) {
+        const int n0 = min(kNumChunk, a1 - c0);
         int nb = 0;
-        if (tid < n) {
-            const int4 m = g.a_meta[c0 + tid];
+        int4 am = make_int4(0, 0, 0, 0);
+        int bstart = 0;
+        if (tid < n0) {
+            am = g.a_meta[c0 + tid];
             const int2 br = g.a_brange[c0 + tid];
             nb = br.y;
-            s_ab[tid] = static_cast<uint32_t>(m.z);
-            s_am[tid] = static_cast<uint32_t>(m.y);
-            const uint32_t off = static_cast<uint32_t>(m.w);
-            s_ao[tid] = ((off & ~kDenseBit) - static_cast<uint32_t>(a_base)) | (off & kDenseBit);
-            s_b0[tid] = br.x;
+            bstart = br.x;
         }
         int total = 0;
         const int pre = block_exclusive_scan(nb, s_tmp, &total);
-        if (tid < n)
+        // The chunk: the A tiles whose pairs end within one window (a
+        // prefix, since the prefix sums grow), at least one tile.
+        const int n = max(1, __syncthreads_count(tid < n0 && pre + nb <= kWinP));
+        if (tid == n - 1)
+            s_tot = pre + nb;
+        if (tid < n) {
+            const uint32_t ab = static_cast<uint32_t>(am.z);
+            const uint32_t p1 = __popc(byte_of(ab, 0));
+            const uint32_t p2 = p1 + __popc(byte_of(ab, 1));
+            const uint32_t p3 = p2 + __popc(byte_of(ab, 2));
+            s_ab[tid] = ab;
+            s_apf[tid] = (p1 << 8) | (p2 << 16) | (p3 << 24);
+            s_acb[tid] = static_cast<int>(static_cast<uint32_t>(am.w) & ~kDenseBit) - a_base;
+            s_b0[tid] = bstart;
             s_pre[tid] = pre;
-        for (int r0 = 0; r0 < total; r0 += kSeg) {
-            const int r1 = min(total, r0 + kSeg);
+        }
+        __syncthreads();
+        total = s_tot;
+        c0 += n;
+        for (int r0 = 0; r0 < total; r0 += kWinP) {
+            const int r1 = min(total, r0 + kWinP);
+            if (tid < kClasses)
+                s_ccnt[tid] = 0;
             if (tid < n)
                 fill_segments(s_seg, pre, nb, r0, r1);
             __syncthreads();
-            for (int p0 = r0 + tid; p0 < r1; p0 += kNumBatch * kThreads) {
-                int ai[kNumBatch];
-                int4 meta[kNumBatch];
-                double v[kNumBatch][kRecVals];
+            if constexpr (!SORT) {
+                for (int p0 = r0 + tid; p0 < r1; p0 += kNumBatch * kThreads) {
+                    int pa[kNumBatch];
+                    BRecord rec[kNumBatch];
 #pragma unroll
-                for (int u = 0; u < kNumBatch; ++u) {
-                    const int p = p0 + u * kThreads;
-                    ai[u] = -1;
-                    meta[u] = make_int4(0, 0, 0, 0);
+                    for (int u = 0; u < kNumBatch; ++u) {
+                        const int p = p0 + u * kThreads;
+                        pa[u] = -1;
+                        if (p < r1) {
+                            pa[u] = s_seg[p - r0];
+                            rec[u] = recs[s_b0[pa[u]] + (p - s_pre[pa[u]])];
+                        }
+                    }
 #pragma unroll
-                    for (int j = 0; j < kRecVals; ++j)
-                        v[u][j] = 0.0;
-                    if (p < r1) {
-                        ai[u] = s_seg[p - r0];
-                        const BRecord *rp = recs + s_b0[ai[u]] + (p - s_pre[ai[u]]);
-                        meta[u] = rp->meta;
-#pragma unroll
-                        for (int j = 0; j < kRecVals; ++j)
-                            v[u][j] = rp->v[j];
+                    for (int u = 0; u < kNumBatch; ++u) {
+                        if (pa[u] < 0)
+                            continue;
+                        if (!product_count(s_ab[pa[u]], static_cast<uint32_t>(rec[u].meta.z)))
+                            continue;
+                        const int q = lookup(rec[u].meta.x);
+                        if (q >= 0)
+                            pair_products(pa[u], q, rec[u]);
                     }
                 }
+            } else {
+                // Phase A: meta of this thread's pairs, loads issued together.
+                int4 bm[kPerThread];
+                int ai[kPerThread];
 #pragma unroll
-                for (int u = 0; u < kNumBatch; ++u) {
+                for (int u = 0; u < kPerThread; ++u) {
+                    const int p = r0 + tid + u * kThreads;
+                    ai[u] = -1;
+                    bm[u] = make_int4(0, 0, 0, 0);
+                    if (p < r1) {
+                        ai[u] = s_seg[p - r0];
+                        bm[u] = g.b_meta[s_b0[ai[u]] + (p - s_pre[ai[u]])];
+                    }
+                }
+                int tag[kPerThread];
+#pragma unroll
+                for (int u = 0; u < kPerThread; ++u) {
+                    tag[u] = -1;
                     if (ai[u] < 0)
                         continue;
-                    const uint32_t ab = s_ab[ai[u]];
-                    if (!product_count(ab, static_cast<uint32_t>(meta[u].z)))
+                    const int cnt = product_count(s_ab[ai[u]], static_cast<uint32_t>(bm[u].z));
+                    if (!cnt)
                         continue;
-                    // Every tile of the row is in the table and at least half
-                    // of its slots are empty, so the probe ends.
-                    uint32_t h = slot_of(meta[u].x, Cfg::kSlotBits);
-                    int q = -1;
-                    for (;;) {
-                        const int k = s_hk[h];
-                        if (k == meta[u].x) {
-                            q = s_hq[h];
-                            break;
-                        }
-                        if (k == kEmpty)
-                            break;
-                        h = (h + 1) & (Cfg::kSlots - 1);
-                    }
+                    const int q = lookup(bm[u].x);
                     if (q < 0)
                         continue;
-                    accumulate_pair(g, s_acc, s_aval, ab, s_am[ai[u]], s_ao[ai[u]], meta[u],
-                                    v[u], s_cmask[q], s_cbase[q]);
+                    s_rq[tid + u * kThreads] = static_cast<QT>(q);
+                    const int cls = min(31 - __clz(cnt), kClasses - 1);
+                    tag[u] = (cls << 16) | atomicAdd(&s_ccnt[cls], 1);
+                }
+                __syncthreads();
+                int off[kClasses];
+                int staged = 0;
+#pragma unroll
+                for (int k = 0; k < kClasses; ++k) {
+                    off[k] = staged;
+                    staged += s_ccnt[k];
+                }
+#pragma unroll
+                for (int u = 0; u < kPerThread; ++u)
+                    if (tag[u] >= 0)
+                        s_order[off[tag[u] >> 16] + (tag[u] & 0xFFFF)] =
+                            static_cast<unsigned short>(tid + u * kThreads);
+                __syncthreads();
+                // Phase B: sorted entries, one record in flight per thread.
+                for (int i = tid; i < staged; i += kThreads) {
+                    const int sl = s_order[i];
+                    const int pa = s_seg[sl];
+                    const BRecord rec = recs[s_b0[pa] + (r0 + sl - s_pre[pa])];
+                    pair_products(pa, s_rq[sl], rec);
                 }
             }
             __syncthreads();
@@ -743,8 +872,13 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
         const int nt = static_cast<int>(c.tile_ptr[row + 1] - t0);
         const int64_t zb = c.row_nnz_off[row];
         const int64_t nnz_row = c.row_nnz_off[row + 1] - zb;
-        for (int q = tid; q < nt; q += kThreads)
-            slab[c.col[t0 + q]] = q;
+        // Columns are checked against the slab's extent, so that a damaged
+        // structure cannot send a store outside it.
+        for (int q = tid; q < nt; q += kThreads) {
+            const int j = c.col[t0 + q];
+            if (j >= 0 && j < g.b_tile_cols)
+                slab[j] = q;
+        }
         for (int64_t i = tid; i < nnz_row; i += kThreads)
             c.val[zb + i] = 0.0;
         __threadfence();
@@ -770,6 +904,8 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
                 const uint32_t boff = bw & ~kDenseBit;
                 const int q = __hip_atomic_load(&slab[bm.x], __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_AGENT);
+                if (q < 0 || q >= nt)
+                    continue;
                 const int64_t tq = t0 + q;
                 const uint64_t cm = c.mask[tq];
                 const int64_t cb = c.tile_nnz[tq];
@@ -803,8 +939,11 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
         }
         __threadfence();
         __syncthreads();
-        for (int q = tid; q < nt; q += kThreads)
-            slab[c.col[t0 + q]] = -1;
+        for (int q = tid; q < nt; q += kThreads) {
+            const int j = c.col[t0 + q];
+            if (j >= 0 && j < g.b_tile_cols)
+                slab[j] = -1;
+        }
         __threadfence();
         __syncthreads();
     }
@@ -841,6 +980,22 @@ __global__ void collect_rows_kernel(int n, int mode, const int *__restrict__ lev
         out_slab[atomicAdd(n_slab, 1)] = i;
     else if (nt > SmallRow::kTiles || nz > SmallRow::kAcc || alen > SmallRow::kAStage)
         out_lds[atomicAdd(n_lds, 1)] = i;
+}
+
+// Number of (A tile, B tile) pairs: the sum of the per-A-tile extents, one
+// atomic per workgroup.
+__global__ __launch_bounds__(kThreads) void pair_count_kernel(int64_t n,
+                                                              const int2 *__restrict__ brange,
+                                                              unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long s_tmp[kWavesPerBlock];
+    unsigned long long sum = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kThreads)
+        sum += static_cast<unsigned long long>(brange[i].y);
+    sum = block_sum(sum, s_tmp);
+    if (threadIdx.x == 0 && sum)
+        atomicAdd(out, sum);
 }
 
 __global__ void fill_i32_kernel(int *p, int64_t n, int v)
@@ -891,6 +1046,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     g.a_brange = b.a_brange.as<int2>();
     g.a_off = a.value_offsets.as<int>();
     g.a_vals = a.values.as<double>();
+    g.a_vals_cm = a.values_cm.as<double>();
     g.b_trp = b.tile_row_ptr.as<int>();
     g.b_meta = b.meta.as<int4>();
     g.b_vals = b.payload_values();
@@ -929,11 +1085,15 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, s,
                        reinterpret_cast<unsigned long long *>(w->row_nnz.as<int64_t>() + R),
                        int64_t(1));
+    if (a.num_tiles > 0)
+        hipLaunchKernelGGL(pair_count_kernel,
+                           dim3(static_cast<unsigned>(std::min<int64_t>(4 * num_cus, blocks_for(a.num_tiles, kThreads)))),
+                           dim3(kThreads), 0, s, a.num_tiles, g.a_brange, ctr);
     RTT_HIP(hipEventRecord(ev->ev[0], s));
     if (R > 0)
         hipLaunchKernelGGL(symbolic_count_kernel, dim3(R), dim3(kThreads), 0, s, g, stage_limit,
                            st, w->row_tiles.as<int64_t>(), w->row_nnz.as<int64_t>(),
-                           w->row_level.as<int>(), ctr);
+                           w->row_level.as<int>());
     RTT_HIP(hipGetLastError());
     RTT_HIP(hipEventRecord(ev->ev[1], s));
     int n_slab = 0;
@@ -957,7 +1117,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
                            static_cast<int64_t>(slab_blocks) * BC);
         hipLaunchKernelGGL((symbolic_slab_kernel<false>), dim3(slab_blocks), dim3(kThreads), 0, s,
                            g, w->l3_rows.as<int>(), n_slab, w->slab.as<unsigned long long>(),
-                           w->row_tiles.as<int64_t>(), w->row_nnz.as<int64_t>(), ctr, none);
+                           w->row_tiles.as<int64_t>(), w->row_nnz.as<int64_t>(), none);
         RTT_HIP(hipGetLastError());
     }
     RTT_TRY(exclusive_scan_i64(w->row_tiles.as<int64_t>(), c->tile_ptr.as<int64_t>(), R,
@@ -1017,7 +1177,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     if (n_slab > 0)
         hipLaunchKernelGGL((symbolic_slab_kernel<true>), dim3(slab_blocks), dim3(kThreads), 0, s,
                            g, w->l3_rows.as<int>(), n_slab, w->slab.as<unsigned long long>(),
-                           nullptr, nullptr, nullptr, co);
+                           nullptr, nullptr, co);
     hipLaunchKernelGGL(set_last_kernel, dim3(1), dim3(1), 0, s,
                        c->tile_nnz.as<int64_t>() + c->num_tiles,
                        w->c_row_nnz_off.as<int64_t>() + R);
@@ -1025,14 +1185,20 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     RTT_HIP(hipEventRecord(ev->ev[6], s));
     // ---- numeric ----------------------------------------------------------
     RTT_HIP(hipEventRecord(ev->ev[4], s));
-    if (R > 0 && c->nnz > 0)
-        hipLaunchKernelGGL((numeric_row_kernel<SmallRow>), dim3(R), dim3(kThreads), 0, s, g,
-                           b.records.as<BRecord>(), nullptr, w->row_level.as<int>(), co);
+    if (R > 0 && c->nnz > 0) {
+        static const int sorted = getenv("RTT_NUM_SORT") ? atoi(getenv("RTT_NUM_SORT")) : 0;
+        if (sorted)
+            hipLaunchKernelGGL((numeric_row_kernel<SmallRow, true>), dim3(R), dim3(kThreads), 0, s,
+                               g, b.records.as<BRecord>(), nullptr, w->row_level.as<int>(), co);
+        else
+            hipLaunchKernelGGL((numeric_row_kernel<SmallRow, false>), dim3(R), dim3(kThreads), 0, s,
+                               g, b.records.as<BRecord>(), nullptr, w->row_level.as<int>(), co);
+    }
     RTT_HIP(hipGetLastError());
     RTT_HIP(hipEventRecord(ev->ev[5], s));
     if (n_num_lds > 0)
-        hipLaunchKernelGGL((numeric_row_kernel<LargeRow>), dim3(n_num_lds), dim3(kThreads), 0, s,
-                           g, b.records.as<BRecord>(), w->lds_rows.as<int>(),
+        hipLaunchKernelGGL((numeric_row_kernel<LargeRow, false>), dim3(n_num_lds), dim3(kThreads), 0,
+                           s, g, b.records.as<BRecord>(), w->lds_rows.as<int>(),
                            w->row_level.as<int>(), co);
     if (n_num_slab > 0) {
         const int blocks = std::min(n_num_slab, 2 * num_cus);
