@@ -266,10 +266,11 @@ __global__ void row_ptr_from_sorted_kernel(const int *__restrict__ keys, int64_t
         ptr[k] = static_cast<int>(i);
 }
 
-// One 80-byte record per B tile (meta + the first kRecVals payload values),
+// One 80-byte record per B tile (meta + the first kRecVals values of the
+// tile's k-major payload: row k's entries contiguous, columns ascending),
 // written as five 16-byte parts by five consecutive threads.
 __device__ __forceinline__ void write_record_part(BRecord *recs, int64_t t, int part,
-                                                  const int4 &m, const double *payload)
+                                                  const int4 &m, const double *kmajor)
 {
     int4 *dst = reinterpret_cast<int4 *>(recs + t) + part;
     if (part == 0) {
@@ -278,7 +279,8 @@ __device__ __forceinline__ void write_record_part(BRecord *recs, int64_t t, int 
     }
     const uint32_t w = static_cast<uint32_t>(m.w);
     const uint32_t off = w & 0x7FFFFFFFu;
-    const int nv = (w & 0x80000000u) ? kRecVals : min(kRecVals, __popc(static_cast<uint32_t>(m.y)));
+    const int nv = min(kRecVals, __popc(static_cast<uint32_t>(m.y)));
+    const double *payload = kmajor;
     const int j = 2 * (part - 1);
     double2 v;
     v.x = j < nv ? payload[off + j] : 0.0;
@@ -287,20 +289,21 @@ __device__ __forceinline__ void write_record_part(BRecord *recs, int64_t t, int 
 }
 
 __global__ void records_kernel(int64_t n, const int4 *__restrict__ meta,
-                               const double *__restrict__ payload, BRecord *__restrict__ recs)
+                               const double *__restrict__ kmajor, BRecord *__restrict__ recs)
 {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const int64_t t = gid / 5;
     if (t >= n)
         return;
-    write_record_part(recs, t, static_cast<int>(gid % 5), meta[t], payload);
+    write_record_part(recs, t, static_cast<int>(gid % 5), meta[t], kmajor);
 }
 
 // Offline, per A tile: the record a B tile built as A^T takes over, its meta
 // with the tile column replaced by the tile row.
+// The B tile's k-major order is the A tile's column-major order.
 __global__ void transpose_source_kernel(int64_t n, const int *__restrict__ a_row_of,
                                         const int4 *__restrict__ a_meta,
-                                        const double *__restrict__ a_vals,
+                                        const double *__restrict__ a_vals_cm,
                                         BRecord *__restrict__ out)
 {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -309,7 +312,7 @@ __global__ void transpose_source_kernel(int64_t n, const int *__restrict__ a_row
         return;
     int4 m = a_meta[t];
     m.x = a_row_of[t];
-    write_record_part(out, t, static_cast<int>(gid % 5), m, a_vals);
+    write_record_part(out, t, static_cast<int>(gid % 5), m, a_vals_cm);
 }
 
 // B = A^T: B tile b is A tile ids[b], so its record is a copy of that A
@@ -559,11 +562,12 @@ void DevTiles::view(rtt_device_tiles *v) const
 
 int build_records(DevTiles *t)
 {
+    RTT_TRY(build_colmajor_values(t));
     RTT_TRY(t->records.ensure(t->num_tiles * sizeof(BRecord)));
     if (t->num_tiles > 0)
         hipLaunchKernelGGL(records_kernel, dim3(blocks_for(t->num_tiles * 5, kThreads)),
                            dim3(kThreads), 0, ctx().stream, t->num_tiles, t->meta.as<int4>(),
-                           t->payload_values(), t->records.as<BRecord>());
+                           t->values_cm.as<double>(), t->records.as<BRecord>());
     RTT_HIP(hipGetLastError());
     return RTT_OK;
 }
@@ -618,7 +622,7 @@ int build_transpose_source(DevTiles *t)
     if (t->num_tiles > 0)
         hipLaunchKernelGGL(transpose_source_kernel, dim3(blocks_for(t->num_tiles * 5, kThreads)),
                            dim3(kThreads), 0, ctx().stream, t->num_tiles,
-                           t->tile_row_of.as<int>(), t->meta.as<int4>(), t->values.as<double>(),
+                           t->tile_row_of.as<int>(), t->meta.as<int4>(), t->values_cm.as<double>(),
                            t->transpose_src.as<BRecord>());
     RTT_HIP(hipGetLastError());
     return RTT_OK;

@@ -92,8 +92,11 @@ struct DevTiles {
     DevBuf meta;          // int4 per tile
     DevBuf records;       // BRecord per tile (B only)
     DevBuf a_brange;      // B only: per A tile, {first B tile, count} of B row K
-    DevBuf values_cm;     // A only: each tile's values column-major (rows of column k
-                          // ascending, columns in order), packed from the tile's offset
+    DevBuf values_cm;     // each tile's values with the stride-4 bit groups in order:
+                          // A (8x4): column-major; B (4x8): k-major (row k's entries
+                          // contiguous).  Packed from the tile's offset.  For B built as
+                          // A^T this is A's array (payload_kmajor()).
+
     DevBuf transpose_src; // A only: per tile, the BRecord of the B = A^T tile it becomes
     // B built as A^T at tile level: meta offsets index A's payload, and
     // sorted_ids[b] is the A tile behind B tile b.
@@ -102,6 +105,10 @@ struct DevTiles {
     const double *payload_values() const
     {
         return transpose_of ? transpose_of->values.as<double>() : values.as<double>();
+    }
+    const double *payload_kmajor() const
+    {
+        return transpose_of ? transpose_of->values_cm.as<double>() : values_cm.as<double>();
     }
     void view(rtt_device_tiles *v) const;
 };
@@ -167,11 +174,11 @@ int materialize_tiles(DevTiles *t, BuildWork *work);
 int transpose_csr(const DevCsr &src, DevCsr *out, BuildWork *work);
 int nnz_cub(const DevCsr &csr, bool aat, unsigned long long *out, BuildWork *work);
 int fill_tile_row_of(DevTiles *t);
-// Fills t->values_cm from t->meta and t->values (A tiles, offline).
+// Fills t->values_cm from t->meta and t->values.
 int build_colmajor_values(DevTiles *t);
 // Fills t->transpose_src (A tiles, offline; needs tile_row_of).
 int build_transpose_source(DevTiles *t);
-// Fills t->records from t->meta and the payload the meta offsets index.
+// Fills t->values_cm and t->records (meta + first k-major values).
 int build_records(DevTiles *t);
 // b->a_brange[t] = {B.tile_row_ptr[K], count} for every A tile t in column K.
 int build_pair_extents(const DevTiles &a, DevTiles *b);

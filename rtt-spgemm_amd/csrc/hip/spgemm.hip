@@ -18,16 +18,15 @@
 //    table, compacted, ranked by tile column and staged; after the size scan
 //    a light pass writes the final structure.  Rows whose tile set outgrows
 //    the table use a direct-mapped mask slab in HBM instead.
-//  * numeric: the row's exact C layout (tile columns, masks, value offsets)
-//    and its A payload are loaded into LDS.  Each thread takes pairs in
-//    batches, loading every pair's 80-byte B record (packed meta + first
-//    eight values) before using any, and adds each scalar product into an LDS
-//    FP64 accumulator laid out exactly like the row's output segment, which
-//    is finally streamed to HBM.  Rows that exceed the LDS budgets fall back
-//    to a workgroup path with FP64 atomics in HBM.
+//  * numeric: the row's C layout (tile columns, masks, value offsets) and
+//    its A payload (column-major per tile) are loaded into LDS.  Each thread
+//    takes a pair, loads its 80-byte B record (packed meta + the first eight
+//    values in k-major order), walks B's entries row by row with A's column
+//    in registers, and adds each scalar product into an LDS FP64 accumulator
+//    laid out exactly like the row's output segment, which is finally
+//    streamed to HBM.  Rows that exceed the LDS budgets fall back to a
+//    workgroup path with FP64 atomics in HBM.
 #include <climits>
-#include <cstdio>
-#include <cstdlib>
 
 #include "device.hpp"
 #include "internal.hpp"
@@ -48,10 +47,7 @@ constexpr int kSymPerThread = kSymSlots / kThreads;
 constexpr int kSymBatch = 4;       // pairs whose loads a thread issues together
 
 constexpr int kNumChunk = 64;      // A tiles per numeric chunk
-constexpr int kNumBatch = 2;       // records a thread loads together (unsorted numeric)
-constexpr int kNumWin = 1024;      // pairs per numeric window
-constexpr int kPerThread = kNumWin / kThreads;
-constexpr int kClasses = 6;        // numeric pair classes: products 1, 2-3, 4-7, 8-15, 16-31, 32+
+constexpr int kColRegs = 4;        // values of an A column a numeric lane holds in registers
 
 // LDS budgets of the two numeric instances: accumulator slots, staged A
 // values, C tiles, hash slots (at least twice the tiles).
@@ -79,6 +75,7 @@ struct Operands {
     const int *__restrict__ a_off;
     const double *__restrict__ a_vals;
     const double *__restrict__ a_vals_cm;   // A payload, each tile column-major
+    const double *__restrict__ b_vals_km;   // B payload, each tile k-major
     const int *__restrict__ b_trp;
     const int4 *__restrict__ b_meta;
     const double *__restrict__ b_vals;
@@ -131,13 +128,15 @@ __device__ __forceinline__ int product_count(uint32_t a_bytes, uint32_t b_bytes)
     return n;
 }
 
+// Byte r of the result is the number of set bits in bytes 0..r-1 of m (the
+// u8 row offsets of a C tile): per-byte popcounts, then an exclusive byte
+// prefix sum by multiplication (sums stay below 256, so no byte carries).
 __device__ __forceinline__ uint64_t packed_row_offsets(uint64_t m)
 {
-    uint64_t out = 0;
-#pragma unroll
-    for (int r = 1; r < 8; ++r)
-        out |= static_cast<uint64_t>(__popcll(m & ((1ull << (8 * r)) - 1ull))) << (8 * r);
-    return out;
+    uint64_t x = m - ((m >> 1) & 0x5555555555555555ull);
+    x = (x & 0x3333333333333333ull) + ((x >> 2) & 0x3333333333333333ull);
+    x = (x + (x >> 4)) & 0x0F0F0F0F0F0F0F0Full;
+    return (x * 0x0101010101010101ull) << 8;
 }
 
 __device__ __forceinline__ void write_tile_structure(const COut &c, int64_t t, int j,
@@ -207,7 +206,7 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
     Operands g, int stage_limit, Stage st, int64_t *__restrict__ row_tiles,
     int64_t *__restrict__ row_nnz, int *__restrict__ row_level)
 {
-    __shared__ __align__(16) int s_key[kSymSlots];
+    __shared__ int s_key[kSymSlots];
     __shared__ unsigned long long s_msk[kSymSlots];
     __shared__ unsigned long long s_spread[4][kChunk];
     __shared__ int s_b0[kChunk];
@@ -215,7 +214,7 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
     __shared__ unsigned char s_seg[kSeg];
     __shared__ int s_tmp[kWavesPerBlock];
     __shared__ long long s_tmp64[kWavesPerBlock];
-    __shared__ int s_cnt, s_ovf, s_jmin, s_jmax;
+    __shared__ int s_cnt, s_ovf;
 
     const int row = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
@@ -226,8 +225,6 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
     if (tid == 0) {
         s_cnt = 0;
         s_ovf = 0;
-        s_jmin = INT_MAX;
-        s_jmax = INT_MIN;
     }
     const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
     for (int c0 = a0; c0 < a1; c0 += kChunk) {
@@ -329,70 +326,23 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
     int dummy = 0;
     int pos = block_exclusive_scan(mine, s_tmp, &dummy);
     long long nz = 0;
-    int jlo = INT_MAX, jhi = INT_MIN;
 #pragma unroll
     for (int i = 0; i < kSymPerThread; ++i)
         if (rk[i] != kEmpty) {
             s_key[pos] = rk[i];
             s_msk[pos] = rm[i];
             nz += __popcll(rm[i]);
-            jlo = min(jlo, rk[i]);
-            jhi = max(jhi, rk[i]);
             ++pos;
         }
-    if (mine) {
-        atomicMin(&s_jmin, jlo);
-        atomicMax(&s_jmax, jhi);
-    }
     __syncthreads();
     const int64_t sb = static_cast<int64_t>(row) * st.stride;
-    const int jmin = s_jmin;
-    const int64_t span = nt > 0 ? static_cast<int64_t>(s_jmax) - jmin + 1 : 0;
-    if (nt > 0 && span <= 32 * kThreads) {
-        // Rank by a bitmap of the row's column span: a word prefix count
-        // plus the set bits below the column in its word.  s_spread is free
-        // once the pairs are done and holds the bitmap and the word prefixes.
-        uint32_t *bits = reinterpret_cast<uint32_t *>(&s_spread[0][0]);
-        const int words = static_cast<int>((span + 31) >> 5);
-        if (tid < words)
-            bits[tid] = 0u;
-        __syncthreads();
-        for (int e = tid; e < nt; e += kThreads) {
-            const int rel = s_key[e] - jmin;
-            atomicOr(&bits[rel >> 5], 1u << (rel & 31));
-        }
-        __syncthreads();
-        const int pc = tid < words ? __popc(bits[tid]) : 0;
-        int all = 0;
-        const int wpre = block_exclusive_scan(pc, s_tmp, &all);
-        if (tid < words)
-            bits[kThreads + tid] = static_cast<uint32_t>(wpre);
-        __syncthreads();
-        for (int e = tid; e < nt; e += kThreads) {
-            const int mj = s_key[e];
-            const int rel = mj - jmin;
-            const int rank = static_cast<int>(bits[kThreads + (rel >> 5)]) +
-                             __popc(bits[rel >> 5] & ((1u << (rel & 31)) - 1u));
-            st.j[sb + rank] = mj;
-            st.m[sb + rank] = s_msk[e];
-        }
-    } else if (nt <= kThreads) {
+    if (nt <= kThreads) {
         // Rank by counting: entry t goes to the number of smaller columns.
-        // The keys are read four at a time; the tail up to a multiple of
-        // four holds INT_MAX, which counts as larger than every key.
-        const int nt4 = (nt + 3) & ~3;
-        if (tid < nt4 - nt)
-            s_key[nt + tid] = INT_MAX;
-        __syncthreads();
         if (tid < nt) {
             const int mj = s_key[tid];
-            const int4 *k4 = reinterpret_cast<const int4 *>(s_key);
             int rank = 0;
-            for (int f = 0; f < nt4 / 4; ++f) {
-                const int4 kk = k4[f];
-                rank += (kk.x < mj ? 1 : 0) + (kk.y < mj ? 1 : 0) + (kk.z < mj ? 1 : 0) +
-                        (kk.w < mj ? 1 : 0);
-            }
+            for (int f = 0; f < nt; ++f)
+                rank += s_key[f] < mj ? 1 : 0;
             st.j[sb + rank] = mj;
             st.m[sb + rank] = s_msk[tid];
         }
@@ -436,14 +386,29 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
 }
 
 // Staged (column, mask) lists -> final C structure; one wave per row.  The
-// row's local-column bytes are assembled in LDS and written coalesced.
+// row's local-column bytes are assembled in LDS and written coalesced: a
+// table gives, for a row byte x of a tile mask, the columns of its set bits
+// packed in increasing order, and each tile row ORs them into the zeroed
+// buffer with at most two 64-bit LDS ORs.
 __global__ __launch_bounds__(kThreads) void symbolic_write_kernel(
     int rows, Stage st, const int *__restrict__ row_level, COut c)
 {
-    __shared__ unsigned char s_buf[kWavesPerBlock][kWriteBuf];
+    __shared__ unsigned long long s_buf[kWavesPerBlock][kWriteBuf / 8];
+    __shared__ unsigned long long s_cols[256];
     const int w = threadIdx.x / kWave;
-    const int row = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + w;
     const int lane = lane_id();
+    {
+        const int x = threadIdx.x;
+        unsigned long long v = 0;
+        int nset = 0;
+#pragma unroll
+        for (int cl = 0; cl < 8; ++cl)
+            if ((x >> cl) & 1)
+                v |= static_cast<unsigned long long>(cl) << (8 * nset++);
+        s_cols[x] = v;
+    }
+    __syncthreads();
+    const int row = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + w;
     if (row >= rows || row_level[row] != kLevelLds)
         return;
     const int64_t tb = c.tile_ptr[row];
@@ -452,7 +417,12 @@ __global__ __launch_bounds__(kThreads) void symbolic_write_kernel(
     const int64_t nnz_row = c.row_nnz_off[row + 1] - zb;
     const bool staged = nnz_row <= kWriteBuf;
     const int64_t sb = static_cast<int64_t>(row) * st.stride;
-    unsigned char *buf = s_buf[w];
+    unsigned long long *buf = s_buf[w];
+    if (staged) {
+        for (int i = lane; i < kWriteBuf / 8; i += kWave)
+            buf[i] = 0ull;
+        wave_sync();
+    }
     int64_t carry = 0;
     for (int q0 = 0; q0 < nt; q0 += kWave) {
         const int q = q0 + lane;
@@ -462,27 +432,41 @@ __global__ __launch_bounds__(kThreads) void symbolic_write_kernel(
         if (q < nt) {
             const int64_t off = carry + (incl - pc);
             const int64_t t = tb + q;
+            const uint64_t ro = packed_row_offsets(m);
             c.col[t] = st.j[sb + q];
             c.mask[t] = m;
             c.tile_nnz[t] = zb + off;
-            reinterpret_cast<uint64_t *>(c.csr_ptr)[t] = packed_row_offsets(m);
-            int i = 0;
-            while (m) {
-                const int pos = __builtin_ctzll(m);
-                m &= m - 1;
-                if (staged)
-                    buf[off + i] = static_cast<unsigned char>(pos & 7);
-                else
+            reinterpret_cast<uint64_t *>(c.csr_ptr)[t] = ro;
+            if (staged) {
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const uint32_t x = static_cast<uint32_t>(m >> (8 * r)) & 0xFFu;
+                    if (!x)
+                        continue;
+                    const int p = static_cast<int>(off) + static_cast<int>((ro >> (8 * r)) & 0xFFu);
+                    const unsigned long long v = s_cols[x];
+                    const int sh = (p & 7) * 8;
+                    atomicOr(&buf[p >> 3], v << sh);
+                    if (sh && __popc(x) * 8 + sh > 64)
+                        atomicOr(&buf[(p >> 3) + 1], v >> (64 - sh));
+                }
+            } else {
+                int i = 0;
+                while (m) {
+                    const int pos = __builtin_ctzll(m);
+                    m &= m - 1;
                     c.csr_col[zb + off + i] = static_cast<uint8_t>(pos & 7);
-                ++i;
+                    ++i;
+                }
             }
         }
         carry += __shfl(incl, kWave - 1, kWave);
     }
     if (staged) {
         wave_sync();
+        const unsigned char *bytes = reinterpret_cast<const unsigned char *>(buf);
         for (int64_t i = lane; i < nnz_row; i += kWave)
-            c.csr_col[zb + i] = buf[i];
+            c.csr_col[zb + i] = bytes[i];
     }
 }
 
@@ -595,41 +579,35 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
 // ---------------------------------------------------------------------------
 // Numeric, LDS path: one workgroup per row, LDS accumulator.
 // ---------------------------------------------------------------------------
-// The row's exact output segment is an FP64 accumulator in LDS.  The row's A
-// payload is staged column-major (the rows of column k of a tile are
-// contiguous), every C tile of the row is in a hash table with its mask and
-// value offset, and each scalar product is added into the accumulator slot
-// given by a popcount of the C tile mask.
-//
-// SORT = false: each thread takes the pairs p, p + kThreads, ... of a
-// segment, kNumBatch records loaded together.
-// SORT = true: pairs are staged in windows of kNumWin; phase A loads their B
-// meta, drops pairs without a product, resolves the C tile and ranks each
-// pair in a class given by its product count; a counting sort orders the
-// window by class and phase B processes it in that order, so that the lanes
-// of a wave run pairs of similar length.
-template <class Cfg, bool SORT>
+// The row's exact output segment is an FP64 accumulator in LDS, every C tile
+// of the row is in a hash table, and each thread takes the pairs p,
+// p + kThreads, ... of a window.  A pair walks its B tile's entries in
+// k-major order (its record holds them so): when the walk reaches row k of
+// B, the lane loads the first values of column k of the A tile (staged
+// column-major in LDS) into registers, and every product of the row-k
+// entries with that column is one FMA into the accumulator slot given by a
+// popcount of the C tile mask.
+struct alignas(16) CTileInfo {
+    unsigned long long mask;
+    int base;      // first value of the tile within the row's segment
+    int pad;
+};
+
+template <class Cfg>
 __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     Operands g, const BRecord *__restrict__ recs, const int *__restrict__ rows,
     const int *__restrict__ row_level, COut c)
 {
     using QT = typename Cfg::QType;
-    constexpr int kWinP = SORT ? kNumWin : kSeg;
     __shared__ double s_acc[Cfg::kAcc];
-    __shared__ double s_acol[Cfg::kAStage];
-    __shared__ unsigned long long s_cmask[Cfg::kTiles];
-    __shared__ unsigned short s_cbase[Cfg::kTiles];
+    __shared__ double s_acol[Cfg::kAStage + kColRegs];
+    __shared__ CTileInfo s_ct[Cfg::kTiles];
     __shared__ int s_hk[Cfg::kSlots];
     __shared__ QT s_hq[Cfg::kSlots];
-    __shared__ uint32_t s_ab[kNumChunk];    // A tile: rows present in column k (byte k)
-    __shared__ uint32_t s_apf[kNumChunk];   // A tile: first staged value of column k (byte k)
-    __shared__ int s_acb[kNumChunk];        // A tile: first staged value
-    __shared__ int s_b0[kNumChunk];
-    __shared__ int s_pre[kNumChunk];
-    __shared__ unsigned char s_seg[kWinP];
-    __shared__ QT s_rq[SORT ? kNumWin : 1];
-    __shared__ unsigned short s_order[SORT ? kNumWin : 1];
-    __shared__ int s_ccnt[kClasses];
+    // Per chunk A tile: {rows present in column k (byte k), first staged
+    // value of column k (byte k), first staged value, first B record - first pair}.
+    __shared__ int4 s_at[kNumChunk];
+    __shared__ unsigned char s_seg[kSeg];
     __shared__ int s_tmp[kWavesPerBlock];
     __shared__ int s_tot;
 
@@ -652,84 +630,21 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
         s_acc[i] = 0.0;
     for (int i = tid; i < a_len; i += kThreads)
         s_acol[i] = g.a_vals_cm[a_base + i];
+    if (tid < kColRegs)
+        s_acol[a_len + tid] = 0.0;   // the register loads of a column may run past it
     __syncthreads();
     for (int q = tid; q < nt; q += kThreads) {
         const int j = c.col[t0 + q];
-        s_cmask[q] = c.mask[t0 + q];
-        s_cbase[q] = static_cast<unsigned short>(c.tile_nnz[t0 + q] - zb);
+        CTileInfo ct;
+        ct.mask = c.mask[t0 + q];
+        ct.base = static_cast<int>(c.tile_nnz[t0 + q] - zb);
+        ct.pad = 0;
+        s_ct[q] = ct;
         uint32_t h = slot_of(j, Cfg::kSlotBits);
         while (atomicCAS(&s_hk[h], kEmpty, j) != kEmpty)
             h = (h + 1) & (Cfg::kSlots - 1);
         s_hq[h] = static_cast<QT>(q);
     }
-    // C tile of column j; every tile of the row is in the table and at least
-    // half of its slots are empty, so the probe ends.
-    auto lookup = [&](int j) {
-        uint32_t h = slot_of(j, Cfg::kSlotBits);
-        for (;;) {
-            const int k = s_hk[h];
-            if (k == j)
-                return static_cast<int>(s_hq[h]);
-            if (k == kEmpty)
-                return -1;
-            h = (h + 1) & (Cfg::kSlots - 1);
-        }
-    };
-    // All products of one pair: B entry (k, cl) in payload order times the
-    // rows of A's column k, two rows per step so that their LDS reads overlap.
-    auto pair_products = [&](int pa, int q, const BRecord &rec) {
-        const uint32_t ab = s_ab[pa];
-        const uint32_t apf = s_apf[pa];
-        const double *acol = s_acol + s_acb[pa];
-        double *acc = s_acc + s_cbase[q];
-        const uint64_t cm = s_cmask[q];
-        const uint32_t bw = static_cast<uint32_t>(rec.meta.w);
-        const uint32_t boff = bw & ~kDenseBit;
-        auto column = [&](int pos, double vb) {
-            const int k = pos & 3, cl = pos >> 2;
-            const double *ap = acol + byte_of(apf, k);
-            uint32_t rws = byte_of(ab, k);
-            while (rws) {
-                const int r1 = __builtin_ctz(rws);
-                rws &= rws - 1;
-                const double v1 = ap[0];
-                const int s1 = __popcll(cm & ((1ull << (r1 * 8 + cl)) - 1ull));
-                if (rws) {
-                    const int r2 = __builtin_ctz(rws);
-                    rws &= rws - 1;
-                    const double v2 = ap[1];
-                    const int s2 = __popcll(cm & ((1ull << (r2 * 8 + cl)) - 1ull));
-                    atomicAdd(&acc[s2], v2 * vb);
-                }
-                atomicAdd(&acc[s1], v1 * vb);
-                ap += 2;
-            }
-        };
-        uint32_t mm = static_cast<uint32_t>(rec.meta.y);
-        if (!(bw & kDenseBit)) {
-#pragma unroll
-            for (int j = 0; j < kRecVals; ++j) {
-                if (mm) {
-                    const int pos = __builtin_ctz(mm);
-                    mm &= mm - 1;
-                    column(pos, rec.v[j]);
-                }
-            }
-            int j = kRecVals;
-            while (mm) {
-                const int pos = __builtin_ctz(mm);
-                mm &= mm - 1;
-                column(pos, g.b_vals[boff + j]);
-                ++j;
-            }
-        } else {
-            while (mm) {
-                const int pos = __builtin_ctz(mm);
-                mm &= mm - 1;
-                column(pos, g.b_vals[boff + pos]);
-            }
-        }
-    };
     for (int c0 = a0; c0 < a1;) {
         const int n0 = min(kNumChunk, a1 - c0);
         int nb = 0;
@@ -745,7 +660,7 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
         const int pre = block_exclusive_scan(nb, s_tmp, &total);
         // The chunk: the A tiles whose pairs end within one window (a
         // prefix, since the prefix sums grow), at least one tile.
-        const int n = max(1, __syncthreads_count(tid < n0 && pre + nb <= kWinP));
+        const int n = max(1, __syncthreads_count(tid < n0 && pre + nb <= kSeg));
         if (tid == n - 1)
             s_tot = pre + nb;
         if (tid < n) {
@@ -753,96 +668,100 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
             const uint32_t p1 = __popc(byte_of(ab, 0));
             const uint32_t p2 = p1 + __popc(byte_of(ab, 1));
             const uint32_t p3 = p2 + __popc(byte_of(ab, 2));
-            s_ab[tid] = ab;
-            s_apf[tid] = (p1 << 8) | (p2 << 16) | (p3 << 24);
-            s_acb[tid] = static_cast<int>(static_cast<uint32_t>(am.w) & ~kDenseBit) - a_base;
-            s_b0[tid] = bstart;
-            s_pre[tid] = pre;
+            s_at[tid] = make_int4(static_cast<int>(ab),
+                                  static_cast<int>((p1 << 8) | (p2 << 16) | (p3 << 24)),
+                                  static_cast<int>(static_cast<uint32_t>(am.w) & ~kDenseBit) - a_base,
+                                  bstart - pre);
         }
         __syncthreads();
         total = s_tot;
         c0 += n;
-        for (int r0 = 0; r0 < total; r0 += kWinP) {
-            const int r1 = min(total, r0 + kWinP);
-            if (tid < kClasses)
-                s_ccnt[tid] = 0;
+        for (int r0 = 0; r0 < total; r0 += kSeg) {
+            const int r1 = min(total, r0 + kSeg);
             if (tid < n)
                 fill_segments(s_seg, pre, nb, r0, r1);
             __syncthreads();
-            if constexpr (!SORT) {
-                for (int p0 = r0 + tid; p0 < r1; p0 += kNumBatch * kThreads) {
-                    int pa[kNumBatch];
-                    BRecord rec[kNumBatch];
+            // Consecutive lanes take consecutive pairs, so a wave's record
+            // loads are one contiguous stretch of B's records.
+            for (int p = r0 + tid; p < r1; p += kThreads) {
+                const int4 at = s_at[s_seg[p - r0]];
+                const BRecord rec = recs[at.w + p];
+                const uint32_t ab = static_cast<uint32_t>(at.x);
+                if (!product_count(ab, static_cast<uint32_t>(rec.meta.z)))
+                    continue;
+                // C tile of the pair; every tile of the row is in the table and
+                // at least half of its slots are empty, so the probe ends.
+                uint32_t h = slot_of(rec.meta.x, Cfg::kSlotBits);
+                int q = -1;
+                for (;;) {
+                    const int k = s_hk[h];
+                    if (k == rec.meta.x) {
+                        q = s_hq[h];
+                        break;
+                    }
+                    if (k == kEmpty)
+                        break;
+                    h = (h + 1) & (Cfg::kSlots - 1);
+                }
+                if (q < 0)
+                    continue;
+                const CTileInfo ct = s_ct[q];
+                double *acc = s_acc + ct.base;
+                const uint64_t cm = ct.mask;
+                const uint32_t apf = static_cast<uint32_t>(at.y);
+                const double *acol = s_acol + at.z;
+                const uint32_t boff = static_cast<uint32_t>(rec.meta.w) & ~kDenseBit;
+                int kcur = -1;
+                uint32_t rk = 0;
+                const double *ap = acol;
+                double av[kColRegs];
 #pragma unroll
-                    for (int u = 0; u < kNumBatch; ++u) {
-                        const int p = p0 + u * kThreads;
-                        pa[u] = -1;
-                        if (p < r1) {
-                            pa[u] = s_seg[p - r0];
-                            rec[u] = recs[s_b0[pa[u]] + (p - s_pre[pa[u]])];
+                for (int i = 0; i < kColRegs; ++i)
+                    av[i] = 0.0;
+                // Products of B entry (k, cl) with the rows of A's column k.
+                auto entry = [&](int pos, double vb) {
+                    const int k = pos >> 3, cl = pos & 7;
+                    if (k != kcur) {
+                        kcur = k;
+                        rk = byte_of(ab, k);
+                        ap = acol + byte_of(apf, k);
+#pragma unroll
+                        for (int i = 0; i < kColRegs; ++i)
+                            av[i] = ap[i];
+                    }
+                    uint32_t rr = rk;
+#pragma unroll
+                    for (int i = 0; i < kColRegs; ++i) {
+                        if (rr) {
+                            const int r = __builtin_ctz(rr);
+                            rr &= rr - 1;
+                            atomicAdd(&acc[__popcll(cm & ((1ull << (r * 8 + cl)) - 1ull))],
+                                      av[i] * vb);
                         }
                     }
+                    int i = kColRegs;
+                    while (rr) {
+                        const int r = __builtin_ctz(rr);
+                        rr &= rr - 1;
+                        atomicAdd(&acc[__popcll(cm & ((1ull << (r * 8 + cl)) - 1ull))],
+                                  ap[i++] * vb);
+                    }
+                };
+                uint32_t mz = static_cast<uint32_t>(rec.meta.z);
 #pragma unroll
-                    for (int u = 0; u < kNumBatch; ++u) {
-                        if (pa[u] < 0)
-                            continue;
-                        if (!product_count(s_ab[pa[u]], static_cast<uint32_t>(rec[u].meta.z)))
-                            continue;
-                        const int q = lookup(rec[u].meta.x);
-                        if (q >= 0)
-                            pair_products(pa[u], q, rec[u]);
+                for (int j = 0; j < kRecVals; ++j) {
+                    if (mz) {
+                        const int pos = __builtin_ctz(mz);
+                        mz &= mz - 1;
+                        entry(pos, rec.v[j]);
                     }
                 }
-            } else {
-                // Phase A: meta of this thread's pairs, loads issued together.
-                int4 bm[kPerThread];
-                int ai[kPerThread];
-#pragma unroll
-                for (int u = 0; u < kPerThread; ++u) {
-                    const int p = r0 + tid + u * kThreads;
-                    ai[u] = -1;
-                    bm[u] = make_int4(0, 0, 0, 0);
-                    if (p < r1) {
-                        ai[u] = s_seg[p - r0];
-                        bm[u] = g.b_meta[s_b0[ai[u]] + (p - s_pre[ai[u]])];
-                    }
-                }
-                int tag[kPerThread];
-#pragma unroll
-                for (int u = 0; u < kPerThread; ++u) {
-                    tag[u] = -1;
-                    if (ai[u] < 0)
-                        continue;
-                    const int cnt = product_count(s_ab[ai[u]], static_cast<uint32_t>(bm[u].z));
-                    if (!cnt)
-                        continue;
-                    const int q = lookup(bm[u].x);
-                    if (q < 0)
-                        continue;
-                    s_rq[tid + u * kThreads] = static_cast<QT>(q);
-                    const int cls = min(31 - __clz(cnt), kClasses - 1);
-                    tag[u] = (cls << 16) | atomicAdd(&s_ccnt[cls], 1);
-                }
-                __syncthreads();
-                int off[kClasses];
-                int staged = 0;
-#pragma unroll
-                for (int k = 0; k < kClasses; ++k) {
-                    off[k] = staged;
-                    staged += s_ccnt[k];
-                }
-#pragma unroll
-                for (int u = 0; u < kPerThread; ++u)
-                    if (tag[u] >= 0)
-                        s_order[off[tag[u] >> 16] + (tag[u] & 0xFFFF)] =
-                            static_cast<unsigned short>(tid + u * kThreads);
-                __syncthreads();
-                // Phase B: sorted entries, one record in flight per thread.
-                for (int i = tid; i < staged; i += kThreads) {
-                    const int sl = s_order[i];
-                    const int pa = s_seg[sl];
-                    const BRecord rec = recs[s_b0[pa] + (r0 + sl - s_pre[pa])];
-                    pair_products(pa, s_rq[sl], rec);
+                int j = kRecVals;
+                while (mz) {
+                    const int pos = __builtin_ctz(mz);
+                    mz &= mz - 1;
+                    entry(pos, g.b_vals_km[boff + j]);
+                    ++j;
                 }
             }
             __syncthreads();
@@ -1047,6 +966,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     g.a_off = a.value_offsets.as<int>();
     g.a_vals = a.values.as<double>();
     g.a_vals_cm = a.values_cm.as<double>();
+    g.b_vals_km = b.payload_kmajor();
     g.b_trp = b.tile_row_ptr.as<int>();
     g.b_meta = b.meta.as<int4>();
     g.b_vals = b.payload_values();
@@ -1185,19 +1105,13 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     RTT_HIP(hipEventRecord(ev->ev[6], s));
     // ---- numeric ----------------------------------------------------------
     RTT_HIP(hipEventRecord(ev->ev[4], s));
-    if (R > 0 && c->nnz > 0) {
-        static const int sorted = getenv("RTT_NUM_SORT") ? atoi(getenv("RTT_NUM_SORT")) : 0;
-        if (sorted)
-            hipLaunchKernelGGL((numeric_row_kernel<SmallRow, true>), dim3(R), dim3(kThreads), 0, s,
-                               g, b.records.as<BRecord>(), nullptr, w->row_level.as<int>(), co);
-        else
-            hipLaunchKernelGGL((numeric_row_kernel<SmallRow, false>), dim3(R), dim3(kThreads), 0, s,
-                               g, b.records.as<BRecord>(), nullptr, w->row_level.as<int>(), co);
-    }
+    if (R > 0 && c->nnz > 0)
+        hipLaunchKernelGGL((numeric_row_kernel<SmallRow>), dim3(R), dim3(kThreads), 0, s, g,
+                           b.records.as<BRecord>(), nullptr, w->row_level.as<int>(), co);
     RTT_HIP(hipGetLastError());
     RTT_HIP(hipEventRecord(ev->ev[5], s));
     if (n_num_lds > 0)
-        hipLaunchKernelGGL((numeric_row_kernel<LargeRow, false>), dim3(n_num_lds), dim3(kThreads), 0,
+        hipLaunchKernelGGL((numeric_row_kernel<LargeRow>), dim3(n_num_lds), dim3(kThreads), 0,
                            s, g, b.records.as<BRecord>(), w->lds_rows.as<int>(),
                            w->row_level.as<int>(), co);
     if (n_num_slab > 0) {
