@@ -383,8 +383,10 @@ int rtt_rebuild_b(const rtt_prepared_a *a_const, int aat, rtt_dynamic_b **inout,
         RTT_TRY(rtt::build_tiles(a->csr_t, inner_map, a->active_inner, a->rows, a->cols, a->rows,
                                  true, a->dense_threshold, false, &b->tiles, &b->build));
     }
-    if (!b->tiles.transpose_of)
+    if (!b->tiles.transpose_of) {
         RTT_TRY(rtt::build_records(&b->tiles));   // the transpose writes its records itself
+        RTT_TRY(rtt::order_by_class(&b->tiles, &b->build));
+    }
     RTT_TRY(rtt::build_pair_extents(a->tiles, &b->tiles));
     RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
     st.total_ms = rtt::ms_since(t0);

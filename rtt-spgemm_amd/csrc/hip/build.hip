@@ -28,6 +28,13 @@ __global__ void set_i64_kernel(int64_t *p, int64_t n, int64_t v)
         p[i] = v;
 }
 
+__global__ void fill_int_kernel(int *p, int64_t n, int v)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
 __global__ void iota_kernel(int *p, int64_t n)
 {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -332,15 +339,84 @@ __global__ void transposed_records_kernel(int64_t n, const int *__restrict__ ids
         b_meta[t] = v;
 }
 
+// Class of a B tile within its row: 0 for heavy tiles (kHeavyMin entries or
+// more, or entries in more than one row k), else 1 + k for a light tile whose
+// entries all lie in row k.  bytes holds the per-k column bytes of the tile
+// (for B = A^T, the per-k row bytes of the A tile).
+__device__ __forceinline__ int b_tile_class(uint32_t mask, uint32_t bytes)
+{
+    uint32_t rows = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        rows |= ((bytes >> (8 * k)) & 0xFFu) ? (1u << k) : 0u;
+    if (__popc(mask) >= kHeavyMin || __popc(rows) != 1)
+        return 0;
+    return 1 + __builtin_ctz(rows);
+}
+
+// Per A tile t in column K: {first B tile of row K, heavy tiles, all tiles}
+// and the light run its pairs need: for an A tile whose entries all lie in
+// one column k, only the light tiles of row k can meet it.
 __global__ void pair_extent_kernel(int64_t n, const int4 *__restrict__ a_meta,
-                                   const int *__restrict__ b_trp, int2 *__restrict__ out)
+                                   const int *__restrict__ groups, int4 *__restrict__ out,
+                                   int2 *__restrict__ light)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    const int4 m = a_meta[t];
+    const int *gp = groups + static_cast<int64_t>(kClassSlots) * m.x;
+    const int b0 = gp[0];
+    out[t] = make_int4(b0, gp[1] - b0, gp[kClassSlots] - b0, 0);
+    const uint32_t bytes = static_cast<uint32_t>(m.z);
+    uint32_t cols = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        cols |= ((bytes >> (8 * k)) & 0xFFu) ? (1u << k) : 0u;
+    if (__popc(cols) == 1) {
+        const int k0 = __builtin_ctz(cols);
+        light[t] = make_int2(gp[1 + k0], gp[2 + k0] - gp[1 + k0]);
+    } else {
+        light[t] = make_int2(gp[1], gp[5] - gp[1]);
+    }
+}
+
+// Sort key of a B tile: its row times kClassSlots plus its class.
+__global__ void class_keys_kernel(int64_t n, const int *__restrict__ row,
+                                  const int4 *__restrict__ meta, int *__restrict__ keys)
 {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t < n) {
-        const int k = a_meta[t].x;
-        const int b0 = b_trp[k];
-        out[t] = make_int2(b0, b_trp[k + 1] - b0);
+        const int4 m = meta[t];
+        keys[t] = kClassSlots * row[t] +
+                  b_tile_class(static_cast<uint32_t>(m.y), static_cast<uint32_t>(m.z));
     }
+}
+
+// B's row pointer from the row pointer over (row, class) keys.
+__global__ void group_row_ptr_kernel(int rows, const int *__restrict__ groups,
+                                     int *__restrict__ trp)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r <= rows)
+        trp[r] = groups[static_cast<int64_t>(kClassSlots) * r];
+}
+
+// Gathers meta and records of a B built from CSR into class order.
+__global__ void gather_records_kernel(int64_t n, const int *__restrict__ perm,
+                                      const int4 *__restrict__ meta_in,
+                                      const BRecord *__restrict__ rec_in,
+                                      int4 *__restrict__ meta_out, BRecord *__restrict__ rec_out)
+{
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t t = gid / 5;
+    if (t >= n)
+        return;
+    const int part = static_cast<int>(gid % 5);
+    const int src = perm[t];
+    reinterpret_cast<int4 *>(rec_out + t)[part] = reinterpret_cast<const int4 *>(rec_in + src)[part];
+    if (part == 0)
+        meta_out[t] = meta_in[src];
 }
 
 __global__ void entry_rows_kernel(int rows, const int *__restrict__ rp,
@@ -574,11 +650,50 @@ int build_records(DevTiles *t)
 
 int build_pair_extents(const DevTiles &a, DevTiles *b)
 {
-    RTT_TRY(b->a_brange.ensure(a.num_tiles * sizeof(int2)));
+    RTT_TRY(b->a_brange.ensure(a.num_tiles * sizeof(int4)));
+    RTT_TRY(b->a_blight.ensure(a.num_tiles * sizeof(int2)));
     if (a.num_tiles > 0)
         hipLaunchKernelGGL(pair_extent_kernel, dim3(blocks_for(a.num_tiles, kThreads)),
                            dim3(kThreads), 0, ctx().stream, a.num_tiles, a.meta.as<int4>(),
-                           b->tile_row_ptr.as<int>(), b->a_brange.as<int2>());
+                           b->row_groups.as<int>(), b->a_brange.as<int4>(),
+                           b->a_blight.as<int2>());
+    RTT_HIP(hipGetLastError());
+    return RTT_OK;
+}
+
+int order_by_class(DevTiles *b, BuildWork *w)
+{
+    hipStream_t s = ctx().stream;
+    const int64_t n = b->num_tiles;
+    const int R = b->tile_row_count;
+    const int64_t slots = static_cast<int64_t>(kClassSlots) * R;
+    RTT_TRY(b->row_groups.ensure((slots + 1) * sizeof(int)));
+    RTT_TRY(w->keys_in.ensure((n > 0 ? n : 1) * sizeof(int)));
+    RTT_TRY(w->keys_out.ensure((n > 0 ? n : 1) * sizeof(int)));
+    if (n > 0) {
+        RTT_TRY(fill_tile_row_of(b));
+        RTT_TRY(w->ids_in.ensure(n * sizeof(int)));
+        RTT_TRY(w->ids_out.ensure(n * sizeof(int)));
+        RTT_TRY(w->meta2.ensure(n * sizeof(int4)));
+        RTT_TRY(w->records2.ensure(n * sizeof(BRecord)));
+        hipLaunchKernelGGL(class_keys_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
+                           n, b->tile_row_of.as<int>(), b->meta.as<int4>(), w->keys_in.as<int>());
+        hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
+                           w->ids_in.as<int>(), n);
+        RTT_TRY(sort_pairs(w->keys_in.as<int>(), w->keys_out.as<int>(), w->ids_in.as<int>(),
+                           w->ids_out.as<int>(), n, bits_for(slots), &w->scan_tmp, s));
+        hipLaunchKernelGGL(gather_records_kernel, dim3(blocks_for(n * 5, kThreads)),
+                           dim3(kThreads), 0, s, n, w->ids_out.as<int>(), b->meta.as<int4>(),
+                           b->records.as<BRecord>(), w->meta2.as<int4>(),
+                           w->records2.as<BRecord>());
+        RTT_HIP(hipMemcpyAsync(b->meta.ptr, w->meta2.ptr, n * sizeof(int4),
+                               hipMemcpyDeviceToDevice, s));
+        RTT_HIP(hipMemcpyAsync(b->records.ptr, w->records2.ptr, n * sizeof(BRecord),
+                               hipMemcpyDeviceToDevice, s));
+    }
+    hipLaunchKernelGGL(row_ptr_from_sorted_kernel, dim3(blocks_for(n + 1, kThreads)),
+                       dim3(kThreads), 0, s, w->keys_out.as<int>(), n, static_cast<int>(slots),
+                       b->row_groups.as<int>());
     RTT_HIP(hipGetLastError());
     return RTT_OK;
 }
@@ -615,8 +730,21 @@ __global__ void colmajor_values_kernel(int64_t n, const int4 *__restrict__ meta,
     }
 }
 
+__global__ void row_payload_off_kernel(int rows, const int *__restrict__ trp,
+                                       const int *__restrict__ off, int *__restrict__ out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r <= rows)
+        out[r] = off[trp[r]];
+}
+
 int build_transpose_source(DevTiles *t)
 {
+    RTT_TRY(t->row_payload_off.ensure((static_cast<size_t>(t->tile_row_count) + 1) * sizeof(int)));
+    hipLaunchKernelGGL(row_payload_off_kernel, dim3(blocks_for(t->tile_row_count + 1, kThreads)),
+                       dim3(kThreads), 0, ctx().stream, t->tile_row_count,
+                       t->tile_row_ptr.as<int>(), t->value_offsets.as<int>(),
+                       t->row_payload_off.as<int>());
     RTT_TRY(t->transpose_src.ensure(static_cast<size_t>(t->num_tiles > 0 ? t->num_tiles : 1) *
                                     sizeof(BRecord)));
     if (t->num_tiles > 0)
@@ -776,17 +904,24 @@ int transpose_tiles(const DevTiles &a, int b_cols, DevTiles *b, BuildWork *w)
     b->arrays_ready = false;
     b->has_csc = false;
     const int R = b->tile_row_count;
+    const int64_t slots = static_cast<int64_t>(kClassSlots) * R;
     RTT_TRY(b->tile_row_ptr.ensure((R + 1) * sizeof(int)));
+    RTT_TRY(b->row_groups.ensure((slots + 1) * sizeof(int)));
     RTT_TRY(b->meta.ensure(n * sizeof(int4)));
     RTT_TRY(b->sorted_ids.ensure(n * sizeof(int)));
     RTT_TRY(w->ids_in.ensure(n * sizeof(int)));
+    RTT_TRY(w->keys_in.ensure(n * sizeof(int)));
     RTT_TRY(w->keys_out.ensure(n * sizeof(int)));
     if (n > 0) {
-        // Stable radix sort of A's tiles by tile column: B's tile-row order.
+        // Stable radix sort of A's tiles by (tile column, class): B's row
+        // order with every row's heavy tiles first, then its light tiles by
+        // the row k they occupy.
         hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
                            w->ids_in.as<int>(), n);
-        RTT_TRY(sort_pairs(a.tile_col_idx.as<int>(), w->keys_out.as<int>(), w->ids_in.as<int>(),
-                           b->sorted_ids.as<int>(), n, bits_for(R), &w->scan_tmp, s));
+        hipLaunchKernelGGL(class_keys_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
+                           n, a.tile_col_idx.as<int>(), a.meta.as<int4>(), w->keys_in.as<int>());
+        RTT_TRY(sort_pairs(w->keys_in.as<int>(), w->keys_out.as<int>(), w->ids_in.as<int>(),
+                           b->sorted_ids.as<int>(), n, bits_for(slots), &w->scan_tmp, s));
         RTT_TRY(b->records.ensure(n * sizeof(BRecord)));
         hipLaunchKernelGGL(transposed_records_kernel, dim3(blocks_for(n * 5, kThreads)),
                            dim3(kThreads), 0, s, n, b->sorted_ids.as<int>(),
@@ -794,8 +929,10 @@ int transpose_tiles(const DevTiles &a, int b_cols, DevTiles *b, BuildWork *w)
                            b->records.as<BRecord>());
     }
     hipLaunchKernelGGL(row_ptr_from_sorted_kernel, dim3(blocks_for(n + 1, kThreads)),
-                       dim3(kThreads), 0, s, w->keys_out.as<int>(), n, R,
-                       b->tile_row_ptr.as<int>());
+                       dim3(kThreads), 0, s, w->keys_out.as<int>(), n, static_cast<int>(slots),
+                       b->row_groups.as<int>());
+    hipLaunchKernelGGL(group_row_ptr_kernel, dim3(blocks_for(R + 1, kThreads)), dim3(kThreads), 0,
+                       s, R, b->row_groups.as<int>(), b->tile_row_ptr.as<int>());
     RTT_HIP(hipGetLastError());
     return RTT_OK;
 }
@@ -819,18 +956,30 @@ int materialize_tiles(DevTiles *b, BuildWork *w)
         RTT_TRY(w->spans.ensure((n + 1) * sizeof(int64_t)));
         RTT_TRY(w->counters.ensure((n + 1) * sizeof(int64_t)));
         if (n > 0) {
+            // The timed update orders each B row heavy tiles first; the
+            // standard arrays keep the tiles of a row in column order, which
+            // a sort of A's tiles by column alone gives.
+            RTT_TRY(w->ids_in.ensure(n * sizeof(int)));
+            RTT_TRY(w->keys_out.ensure(n * sizeof(int)));
+            RTT_TRY(w->ids_out.ensure(n * sizeof(int)));
+            hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
+                               w->ids_in.as<int>(), n);
+            RTT_TRY(sort_pairs(a.tile_col_idx.as<int>(), w->keys_out.as<int>(),
+                               w->ids_in.as<int>(), w->ids_out.as<int>(), n, bits_for(R),
+                               &w->scan_tmp, s));
+            const int *col_order = w->ids_out.as<int>();
             int64_t *spans = w->spans.as<int64_t>();
             hipLaunchKernelGGL(set_i64_kernel, dim3(1), dim3(1), 0, s, spans + n, int64_t(1),
                                int64_t(0));
             hipLaunchKernelGGL(transpose_meta_kernel, dim3(blocks_for(n, kThreads)),
-                               dim3(kThreads), 0, s, n, b->sorted_ids.as<int>(),
+                               dim3(kThreads), 0, s, n, col_order,
                                a.tile_row_of.as<int>(), a.masks.as<uint32_t>(),
                                a.value_offsets.as<int>(), b->tile_col_idx.as<int>(),
                                b->masks.as<uint32_t>(), spans, b->csc_tile_ids.as<int>());
             int64_t *off64 = w->counters.as<int64_t>();
             RTT_TRY(exclusive_scan_i64(spans, off64, n, &w->scan_tmp, s));
             hipLaunchKernelGGL(transpose_payload_kernel, dim3(blocks_for(n * 32, kThreads)),
-                               dim3(kThreads), 0, s, n, b->sorted_ids.as<int>(),
+                               dim3(kThreads), 0, s, n, col_order,
                                a.value_offsets.as<int>(), a.values.as<double>(), off64,
                                b->value_offsets.as<int>(), b->values.as<double>());
             hipLaunchKernelGGL(narrow_i64_kernel, dim3(1), dim3(1), 0, s, off64 + n,
@@ -853,7 +1002,6 @@ int materialize_tiles(DevTiles *b, BuildWork *w)
                                    tail.size() * sizeof(int), hipMemcpyHostToDevice, s));
             RTT_HIP(hipStreamSynchronize(s));
         }
-        (void)R;
         b->arrays_ready = true;
         b->has_csc = true;
     }

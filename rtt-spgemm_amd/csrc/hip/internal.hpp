@@ -15,6 +15,13 @@ constexpr int kTileM = 8;   // A tile rows / C tile rows
 constexpr int kTileK = 4;   // A tile cols / B tile rows
 constexpr int kTileN = 8;   // B tile cols / C tile cols
 constexpr int kInputElems = 32;
+// B tiles with at least this many entries, or with entries in more than one
+// row k, are "heavy"; the others are "light" and lie in one row k.  Every B
+// row keeps its heavy tiles first, then its light tiles grouped by k, so
+// that pairs of like cost can be taken together and an A tile with entries
+// in one column k meets only the light tiles of row k.
+constexpr int kHeavyMin = 4;
+constexpr int kClassSlots = 8;   // class slots per B row: heavy, light k = 0..3, unused
 
 // Per-B-tile record read by the numeric kernel: the packed meta and the
 // first kRecVals payload values, so that one (A tile, B tile) pair costs one
@@ -91,13 +98,16 @@ struct DevTiles {
     DevBuf tile_row_of;   // tile -> tile row (A: used by the B transpose)
     DevBuf meta;          // int4 per tile
     DevBuf records;       // BRecord per tile (B only)
-    DevBuf a_brange;      // B only: per A tile, {first B tile, count} of B row K
+    DevBuf a_brange;      // B only: per A tile, {first B tile, heavy, count} of B row K
+    DevBuf a_blight;      // B only: per A tile, {first, count} of the light tiles it can meet
+    DevBuf row_groups;    // B only: start of each (row, class) run, kClassSlots per row
     DevBuf values_cm;     // each tile's values with the stride-4 bit groups in order:
                           // A (8x4): column-major; B (4x8): k-major (row k's entries
                           // contiguous).  Packed from the tile's offset.  For B built as
                           // A^T this is A's array (payload_kmajor()).
 
     DevBuf transpose_src; // A only: per tile, the BRecord of the B = A^T tile it becomes
+    DevBuf row_payload_off;   // A only: payload offset of each tile row (tile_row_count + 1)
     // B built as A^T at tile level: meta offsets index A's payload, and
     // sorted_ids[b] is the A tile behind B tile b.
     const DevTiles *transpose_of = nullptr;
@@ -134,6 +144,7 @@ struct SpgemmWork {
 struct BuildWork {
     DevBuf row_tiles, row_payload, row_dense, scan_tmp, counters;
     DevBuf keys_in, keys_out, ids_in, ids_out, spans;
+    DevBuf meta2, records2;                     // heavy-first reordering
     DevBuf tcsr_row_ptr, tcsr_col, tcsr_val;   // transposed CSR (reordered AAT)
 };
 
@@ -176,12 +187,17 @@ int nnz_cub(const DevCsr &csr, bool aat, unsigned long long *out, BuildWork *wor
 int fill_tile_row_of(DevTiles *t);
 // Fills t->values_cm from t->meta and t->values.
 int build_colmajor_values(DevTiles *t);
-// Fills t->transpose_src (A tiles, offline; needs tile_row_of).
+// Fills t->transpose_src (A tiles, offline; needs tile_row_of) and
+// t->row_payload_off.
 int build_transpose_source(DevTiles *t);
 // Fills t->values_cm and t->records (meta + first k-major values).
 int build_records(DevTiles *t);
-// b->a_brange[t] = {B.tile_row_ptr[K], count} for every A tile t in column K.
+// b->a_brange[t] = {B.tile_row_ptr[K], heavy tiles, count} and b->a_blight[t]
+// for every A tile t in column K.
 int build_pair_extents(const DevTiles &a, DevTiles *b);
+// Reorders b->meta and b->records of a B built from CSR into class order
+// within every row (stable) and fills b->row_groups.
+int order_by_class(DevTiles *b, BuildWork *w);
 
 // ---- SpGEMM core (spgemm.hip) ---------------------------------------------
 struct CoreTimes {

@@ -48,6 +48,7 @@ constexpr int kSymBatch = 4;       // pairs whose loads a thread issues together
 
 constexpr int kNumChunk = 64;      // A tiles per numeric chunk
 constexpr int kColRegs = 4;        // values of an A column a numeric lane holds in registers
+static_assert(kThreads == 4 * kNumChunk, "one numeric thread per (pair run, chunk tile)");
 
 // LDS budgets of the two numeric instances: accumulator slots, staged A
 // values, C tiles, hash slots (at least twice the tiles).
@@ -71,10 +72,12 @@ struct Operands {
     int a_tile_rows;
     const int *__restrict__ a_trp;
     const int4 *__restrict__ a_meta;
-    const int2 *__restrict__ a_brange;   // per A tile: first B tile, count
+    const int4 *__restrict__ a_brange;   // per A tile: first B tile, heavy, count
+    const int2 *__restrict__ a_blight;   // per A tile: the light B tiles it can meet
     const int *__restrict__ a_off;
     const double *__restrict__ a_vals;
     const double *__restrict__ a_vals_cm;   // A payload, each tile column-major
+    const int *__restrict__ a_row_off;      // A payload offset of each tile row
     const double *__restrict__ b_vals_km;   // B payload, each tile k-major
     const int *__restrict__ b_trp;
     const int4 *__restrict__ b_meta;
@@ -206,7 +209,7 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
     Operands g, int stage_limit, Stage st, int64_t *__restrict__ row_tiles,
     int64_t *__restrict__ row_nnz, int *__restrict__ row_level)
 {
-    __shared__ int s_key[kSymSlots];
+    __shared__ __align__(16) int s_key[kSymSlots];
     __shared__ unsigned long long s_msk[kSymSlots];
     __shared__ unsigned long long s_spread[4][kChunk];
     __shared__ int s_b0[kChunk];
@@ -214,7 +217,7 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
     __shared__ unsigned char s_seg[kSeg];
     __shared__ int s_tmp[kWavesPerBlock];
     __shared__ long long s_tmp64[kWavesPerBlock];
-    __shared__ int s_cnt, s_ovf;
+    __shared__ int s_ovf;
 
     const int row = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
@@ -222,18 +225,16 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
         s_key[i] = kEmpty;
         s_msk[i] = 0ull;
     }
-    if (tid == 0) {
-        s_cnt = 0;
+    if (tid == 0)
         s_ovf = 0;
-    }
     const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
     for (int c0 = a0; c0 < a1; c0 += kChunk) {
         const int n = min(kChunk, a1 - c0);
         int nb = 0;
         if (tid < n) {
             const uint32_t ab = static_cast<uint32_t>(g.a_meta[c0 + tid].z);
-            const int2 br = g.a_brange[c0 + tid];
-            nb = br.y;
+            const int4 br = g.a_brange[c0 + tid];
+            nb = br.z;
             s_b0[tid] = br.x;
 #pragma unroll
             for (int k = 0; k < 4; ++k)
@@ -283,8 +284,6 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
                             k = atomicCAS(&s_key[h], kEmpty, j);
                             if (k == kEmpty) {
                                 atomicOr(&s_msk[h], static_cast<unsigned long long>(pm));
-                                if (atomicAdd(&s_cnt, 1) >= stage_limit)
-                                    s_ovf = 1;
                                 break;
                             }
                         }
@@ -304,16 +303,9 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
             break;
     }
     __syncthreads();
-    if (s_ovf) {
-        if (tid == 0) {
-            row_level[row] = kLevelSlab;
-            row_tiles[row] = 0;
-            row_nnz[row] = 0;
-        }
-        return;
-    }
-    const int nt = s_cnt;
-    // Compact the occupied slots to the front.
+    // Compact the occupied slots to the front; the scan total is the row's
+    // tile count.  A full table (a probe that found no slot) or more tiles
+    // than the staging holds sends the row to the slab path.
     int rk[kSymPerThread];
     unsigned long long rm[kSymPerThread];
     int mine = 0;
@@ -323,8 +315,16 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
         rm[i] = s_msk[tid * kSymPerThread + i];
         mine += rk[i] != kEmpty ? 1 : 0;
     }
-    int dummy = 0;
-    int pos = block_exclusive_scan(mine, s_tmp, &dummy);
+    int nt = 0;
+    int pos = block_exclusive_scan(mine, s_tmp, &nt);
+    if (s_ovf || nt > stage_limit) {
+        if (tid == 0) {
+            row_level[row] = kLevelSlab;
+            row_tiles[row] = 0;
+            row_nnz[row] = 0;
+        }
+        return;
+    }
     long long nz = 0;
 #pragma unroll
     for (int i = 0; i < kSymPerThread; ++i)
@@ -338,10 +338,18 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
     const int64_t sb = static_cast<int64_t>(row) * st.stride;
     if (nt <= kThreads) {
         // Rank by counting: entry t goes to the number of smaller columns.
+        // The keys are read four at a time, the last nt % 4 one by one.
         if (tid < nt) {
             const int mj = s_key[tid];
+            const int4 *k4 = reinterpret_cast<const int4 *>(s_key);
             int rank = 0;
-            for (int f = 0; f < nt; ++f)
+            const int full = nt & ~3;
+            for (int f = 0; f < full / 4; ++f) {
+                const int4 kk = k4[f];
+                rank += (kk.x < mj ? 1 : 0) + (kk.y < mj ? 1 : 0) + (kk.z < mj ? 1 : 0) +
+                        (kk.w < mj ? 1 : 0);
+            }
+            for (int f = full; f < nt; ++f)
                 rank += s_key[f] < mj ? 1 : 0;
             st.j[sb + rank] = mj;
             st.m[sb + rank] = s_msk[tid];
@@ -484,8 +492,8 @@ __device__ __forceinline__ int load_slab_chunk(const Operands &g, int c0, int n,
     const int t = threadIdx.x;
     int nb = 0;
     if (t < n) {
-        const int2 br = g.a_brange[c0 + t];
-        nb = br.y;
+        const int4 br = g.a_brange[c0 + t];
+        nb = br.z;
         s_ab[t] = static_cast<uint32_t>(g.a_meta[c0 + t].z);
         s_b0[t] = br.x;
     }
@@ -520,10 +528,16 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
         for (int c0 = a0; c0 < a1; c0 += kThreads) {
             const int n = min(kThreads, a1 - c0);
             if (tid < n) {
-                const int2 br = g.a_brange[c0 + tid];
+                // A B row is two column-sorted runs: its heavy tiles, then
+                // its light ones.
+                const int4 br = g.a_brange[c0 + tid];
                 if (br.y > 0) {
                     atomicMin(&s_lo, g.b_meta[br.x].x);
                     atomicMax(&s_hi, g.b_meta[br.x + br.y - 1].x);
+                }
+                if (br.z > br.y) {
+                    atomicMin(&s_lo, g.b_meta[br.x + br.y].x);
+                    atomicMax(&s_hi, g.b_meta[br.x + br.z - 1].x);
                 }
             }
             const int total = load_slab_chunk(g, c0, n, s_ab, s_b0, s_pre, s_tmp);
@@ -604,40 +618,67 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     __shared__ CTileInfo s_ct[Cfg::kTiles];
     __shared__ int s_hk[Cfg::kSlots];
     __shared__ QT s_hq[Cfg::kSlots];
-    // Per chunk A tile: {rows present in column k (byte k), first staged
-    // value of column k (byte k), first staged value, first B record - first pair}.
+    // Per chunk A tile: its staged info and B runs (see stage_tile), and per
+    // (run, tile) the first B record minus the run's first pair.
     __shared__ int4 s_at[kNumChunk];
+    __shared__ int4 s_br[kNumChunk];
+    __shared__ int s_pw[kThreads];
     __shared__ unsigned char s_seg[kSeg];
     __shared__ int s_tmp[kWavesPerBlock];
-    __shared__ int s_tot;
 
     const int tid = threadIdx.x;
     const int row = rows ? rows[blockIdx.x] : xcd_block(blockIdx.x, gridDim.x);
     if (row_level[row] != kLevelLds)
         return;
+    // The row's inputs come in one round of independent loads: C's row
+    // extents, A's tile and payload extents (the payload base per tile row
+    // is precomputed), then the first C tile, A value and A tile of each
+    // thread, before the first barrier.
     const int64_t t0 = c.tile_ptr[row];
     const int nt = static_cast<int>(c.tile_ptr[row + 1] - t0);
     const int64_t zb = c.row_nnz_off[row];
     const int64_t nnz_row = c.row_nnz_off[row + 1] - zb;
     const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
-    const int a_base = g.a_off[a0];
-    const int a_len = g.a_off[a1] - a_base;
+    const int a_base = g.a_row_off[row];
+    const int a_len = g.a_row_off[row + 1] - a_base;
     if (nt > Cfg::kTiles || nnz_row > Cfg::kAcc || a_len > Cfg::kAStage)
         return;
+    int j0 = 0;
+    CTileInfo ct0{};
+    if (tid < nt) {
+        j0 = c.col[t0 + tid];
+        ct0.mask = c.mask[t0 + tid];
+        ct0.base = static_cast<int>(c.tile_nnz[t0 + tid] - zb);
+    }
+    double av0 = 0.0;
+    if (tid < a_len)
+        av0 = g.a_vals_cm[a_base + tid];
+    int4 am0 = make_int4(0, 0, 0, 0), br0 = make_int4(0, 0, 0, 0);
+    int2 bl0 = make_int2(0, 0);
+    if (tid < min(kNumChunk, a1 - a0)) {
+        am0 = g.a_meta[a0 + tid];
+        br0 = g.a_brange[a0 + tid];
+        bl0 = g.a_blight[a0 + tid];
+    }
     for (int i = tid; i < Cfg::kSlots; i += kThreads)
         s_hk[i] = kEmpty;
     for (int i = tid; i < nnz_row; i += kThreads)
         s_acc[i] = 0.0;
-    for (int i = tid; i < a_len; i += kThreads)
+    if (tid < a_len)
+        s_acol[tid] = av0;
+    for (int i = tid + kThreads; i < a_len; i += kThreads)
         s_acol[i] = g.a_vals_cm[a_base + i];
     if (tid < kColRegs)
         s_acol[a_len + tid] = 0.0;   // the register loads of a column may run past it
     __syncthreads();
     for (int q = tid; q < nt; q += kThreads) {
-        const int j = c.col[t0 + q];
-        CTileInfo ct;
-        ct.mask = c.mask[t0 + q];
-        ct.base = static_cast<int>(c.tile_nnz[t0 + q] - zb);
+        int j = j0;
+        CTileInfo ct = ct0;
+        if (q != tid) {
+            j = c.col[t0 + q];
+            ct.mask = c.mask[t0 + q];
+            ct.base = static_cast<int>(c.tile_nnz[t0 + q] - zb);
+        }
         ct.pad = 0;
         s_ct[q] = ct;
         uint32_t h = slot_of(j, Cfg::kSlotBits);
@@ -645,126 +686,133 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
             h = (h + 1) & (Cfg::kSlots - 1);
         s_hq[h] = static_cast<QT>(q);
     }
-    for (int c0 = a0; c0 < a1;) {
-        const int n0 = min(kNumChunk, a1 - c0);
-        int nb = 0;
-        int4 am = make_int4(0, 0, 0, 0);
-        int bstart = 0;
-        if (tid < n0) {
-            am = g.a_meta[c0 + tid];
-            const int2 br = g.a_brange[c0 + tid];
-            nb = br.y;
-            bstart = br.x;
+    // Chunk tile info: {rows present in column k (byte k), first staged
+    // value of column k (byte k), first staged value, heavy} and its B runs
+    // {first heavy tile, heavy count, first light tile, light count}.
+    auto stage_tile = [&](const int4 &am, const int4 &br, const int2 &bl) {
+        const uint32_t ab = static_cast<uint32_t>(am.z);
+        const uint32_t p1 = __popc(byte_of(ab, 0));
+        const uint32_t p2 = p1 + __popc(byte_of(ab, 1));
+        const uint32_t p3 = p2 + __popc(byte_of(ab, 2));
+        s_at[tid] = make_int4(static_cast<int>(ab),
+                              static_cast<int>((p1 << 8) | (p2 << 16) | (p3 << 24)),
+                              static_cast<int>(static_cast<uint32_t>(am.w) & ~kDenseBit) - a_base,
+                              __popc(static_cast<uint32_t>(am.y)) >= kHeavyMin ? 1 : 0);
+        s_br[tid] = make_int4(br.x, br.y, bl.x, bl.y);
+    };
+    if (tid < min(kNumChunk, a1 - a0))
+        stage_tile(am0, br0, bl0);
+    for (int c0 = a0; c0 < a1; c0 += kNumChunk) {
+        const int n = min(kNumChunk, a1 - c0);
+        if (c0 != a0 && tid < n)
+            stage_tile(g.a_meta[c0 + tid], g.a_brange[c0 + tid], g.a_blight[c0 + tid]);
+        __syncthreads();
+        // The chunk's pairs in four runs, one per (A class, B class): heavy
+        // and light A tiles times the heavy and light runs of their B rows,
+        // so that the lanes of a wave take pairs of like cost.  Thread v
+        // stands for run v / kNumChunk of tile v % kNumChunk.
+        const int run = tid / kNumChunk, vt = tid % kNumChunk;
+        int nb = 0, bstart = 0;
+        if (vt < n && s_at[vt].w == ((run & 1) == 0 ? 1 : 0)) {
+            const int4 br = s_br[vt];
+            nb = run < 2 ? br.y : br.w;
+            bstart = run < 2 ? br.x : br.z;
         }
         int total = 0;
         const int pre = block_exclusive_scan(nb, s_tmp, &total);
-        // The chunk: the A tiles whose pairs end within one window (a
-        // prefix, since the prefix sums grow), at least one tile.
-        const int n = max(1, __syncthreads_count(tid < n0 && pre + nb <= kSeg));
-        if (tid == n - 1)
-            s_tot = pre + nb;
-        if (tid < n) {
-            const uint32_t ab = static_cast<uint32_t>(am.z);
-            const uint32_t p1 = __popc(byte_of(ab, 0));
-            const uint32_t p2 = p1 + __popc(byte_of(ab, 1));
-            const uint32_t p3 = p2 + __popc(byte_of(ab, 2));
-            s_at[tid] = make_int4(static_cast<int>(ab),
-                                  static_cast<int>((p1 << 8) | (p2 << 16) | (p3 << 24)),
-                                  static_cast<int>(static_cast<uint32_t>(am.w) & ~kDenseBit) - a_base,
-                                  bstart - pre);
-        }
-        __syncthreads();
-        total = s_tot;
-        c0 += n;
-        for (int r0 = 0; r0 < total; r0 += kSeg) {
-            const int r1 = min(total, r0 + kSeg);
-            if (tid < n)
-                fill_segments(s_seg, pre, nb, r0, r1);
-            __syncthreads();
-            // Consecutive lanes take consecutive pairs, so a wave's record
-            // loads are one contiguous stretch of B's records.
-            for (int p = r0 + tid; p < r1; p += kThreads) {
-                const int4 at = s_at[s_seg[p - r0]];
-                const BRecord rec = recs[at.w + p];
-                const uint32_t ab = static_cast<uint32_t>(at.x);
-                if (!product_count(ab, static_cast<uint32_t>(rec.meta.z)))
-                    continue;
-                // C tile of the pair; every tile of the row is in the table and
-                // at least half of its slots are empty, so the probe ends.
-                uint32_t h = slot_of(rec.meta.x, Cfg::kSlotBits);
-                int q = -1;
-                for (;;) {
-                    const int k = s_hk[h];
-                    if (k == rec.meta.x) {
-                        q = s_hq[h];
-                        break;
+        s_pw[tid] = bstart - pre;
+        {
+            for (int r0 = 0; r0 < total; r0 += kSeg) {
+                const int r1 = min(total, r0 + kSeg);
+                if (nb > 0)
+                    fill_segments(s_seg, pre, nb, r0, r1);
+                __syncthreads();
+                // Consecutive lanes take consecutive pairs, so a wave's record
+                // loads are one contiguous stretch of B's records.
+                for (int p = r0 + tid; p < r1; p += kThreads) {
+                    const int v = s_seg[p - r0];
+                    const int4 at = s_at[v % kNumChunk];
+                    const BRecord rec = recs[s_pw[v] + p];
+                    const uint32_t ab = static_cast<uint32_t>(at.x);
+                    if (!product_count(ab, static_cast<uint32_t>(rec.meta.z)))
+                        continue;
+                    // C tile of the pair; every tile of the row is in the table and
+                    // at least half of its slots are empty, so the probe ends.
+                    uint32_t h = slot_of(rec.meta.x, Cfg::kSlotBits);
+                    int q = -1;
+                    for (;;) {
+                        const int k = s_hk[h];
+                        if (k == rec.meta.x) {
+                            q = s_hq[h];
+                            break;
+                        }
+                        if (k == kEmpty)
+                            break;
+                        h = (h + 1) & (Cfg::kSlots - 1);
                     }
-                    if (k == kEmpty)
-                        break;
-                    h = (h + 1) & (Cfg::kSlots - 1);
-                }
-                if (q < 0)
-                    continue;
-                const CTileInfo ct = s_ct[q];
-                double *acc = s_acc + ct.base;
-                const uint64_t cm = ct.mask;
-                const uint32_t apf = static_cast<uint32_t>(at.y);
-                const double *acol = s_acol + at.z;
-                const uint32_t boff = static_cast<uint32_t>(rec.meta.w) & ~kDenseBit;
-                int kcur = -1;
-                uint32_t rk = 0;
-                const double *ap = acol;
-                double av[kColRegs];
-#pragma unroll
-                for (int i = 0; i < kColRegs; ++i)
-                    av[i] = 0.0;
-                // Products of B entry (k, cl) with the rows of A's column k.
-                auto entry = [&](int pos, double vb) {
-                    const int k = pos >> 3, cl = pos & 7;
-                    if (k != kcur) {
-                        kcur = k;
-                        rk = byte_of(ab, k);
-                        ap = acol + byte_of(apf, k);
-#pragma unroll
-                        for (int i = 0; i < kColRegs; ++i)
-                            av[i] = ap[i];
-                    }
-                    uint32_t rr = rk;
-#pragma unroll
-                    for (int i = 0; i < kColRegs; ++i) {
-                        if (rr) {
+                    if (q < 0)
+                        continue;
+                    const CTileInfo ct = s_ct[q];
+                    double *acc = s_acc + ct.base;
+                    const uint64_t cm = ct.mask;
+                    const uint32_t apf = static_cast<uint32_t>(at.y);
+                    const double *acol = s_acol + at.z;
+                    const uint32_t boff = static_cast<uint32_t>(rec.meta.w) & ~kDenseBit;
+                    int kcur = -1;
+                    uint32_t rk = 0;
+                    const double *ap = acol;
+                    double av[kColRegs];
+    #pragma unroll
+                    for (int i = 0; i < kColRegs; ++i)
+                        av[i] = 0.0;
+                    // Products of B entry (k, cl) with the rows of A's column k.
+                    auto entry = [&](int pos, double vb) {
+                        const int k = pos >> 3, cl = pos & 7;
+                        if (k != kcur) {
+                            kcur = k;
+                            rk = byte_of(ab, k);
+                            ap = acol + byte_of(apf, k);
+    #pragma unroll
+                            for (int i = 0; i < kColRegs; ++i)
+                                av[i] = ap[i];
+                        }
+                        uint32_t rr = rk;
+    #pragma unroll
+                        for (int i = 0; i < kColRegs; ++i) {
+                            if (rr) {
+                                const int r = __builtin_ctz(rr);
+                                rr &= rr - 1;
+                                atomicAdd(&acc[__popcll(cm & ((1ull << (r * 8 + cl)) - 1ull))],
+                                          av[i] * vb);
+                            }
+                        }
+                        int i = kColRegs;
+                        while (rr) {
                             const int r = __builtin_ctz(rr);
                             rr &= rr - 1;
                             atomicAdd(&acc[__popcll(cm & ((1ull << (r * 8 + cl)) - 1ull))],
-                                      av[i] * vb);
+                                      ap[i++] * vb);
+                        }
+                    };
+                    uint32_t mz = static_cast<uint32_t>(rec.meta.z);
+    #pragma unroll
+                    for (int j = 0; j < kRecVals; ++j) {
+                        if (mz) {
+                            const int pos = __builtin_ctz(mz);
+                            mz &= mz - 1;
+                            entry(pos, rec.v[j]);
                         }
                     }
-                    int i = kColRegs;
-                    while (rr) {
-                        const int r = __builtin_ctz(rr);
-                        rr &= rr - 1;
-                        atomicAdd(&acc[__popcll(cm & ((1ull << (r * 8 + cl)) - 1ull))],
-                                  ap[i++] * vb);
-                    }
-                };
-                uint32_t mz = static_cast<uint32_t>(rec.meta.z);
-#pragma unroll
-                for (int j = 0; j < kRecVals; ++j) {
-                    if (mz) {
+                    int j = kRecVals;
+                    while (mz) {
                         const int pos = __builtin_ctz(mz);
                         mz &= mz - 1;
-                        entry(pos, rec.v[j]);
+                        entry(pos, g.b_vals_km[boff + j]);
+                        ++j;
                     }
                 }
-                int j = kRecVals;
-                while (mz) {
-                    const int pos = __builtin_ctz(mz);
-                    mz &= mz - 1;
-                    entry(pos, g.b_vals_km[boff + j]);
-                    ++j;
-                }
+                __syncthreads();
             }
-            __syncthreads();
         }
     }
     __syncthreads();
@@ -904,14 +952,14 @@ __global__ void collect_rows_kernel(int n, int mode, const int *__restrict__ lev
 // Number of (A tile, B tile) pairs: the sum of the per-A-tile extents, one
 // atomic per workgroup.
 __global__ __launch_bounds__(kThreads) void pair_count_kernel(int64_t n,
-                                                              const int2 *__restrict__ brange,
+                                                              const int4 *__restrict__ brange,
                                                               unsigned long long *__restrict__ out)
 {
     __shared__ unsigned long long s_tmp[kWavesPerBlock];
     unsigned long long sum = 0;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
          i += static_cast<int64_t>(gridDim.x) * kThreads)
-        sum += static_cast<unsigned long long>(brange[i].y);
+        sum += static_cast<unsigned long long>(brange[i].z);
     sum = block_sum(sum, s_tmp);
     if (threadIdx.x == 0 && sum)
         atomicAdd(out, sum);
@@ -962,10 +1010,12 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     g.a_tile_rows = R;
     g.a_trp = a.tile_row_ptr.as<int>();
     g.a_meta = a.meta.as<int4>();
-    g.a_brange = b.a_brange.as<int2>();
+    g.a_brange = b.a_brange.as<int4>();
+    g.a_blight = b.a_blight.as<int2>();
     g.a_off = a.value_offsets.as<int>();
     g.a_vals = a.values.as<double>();
     g.a_vals_cm = a.values_cm.as<double>();
+    g.a_row_off = a.row_payload_off.as<int>();
     g.b_vals_km = b.payload_kmajor();
     g.b_trp = b.tile_row_ptr.as<int>();
     g.b_meta = b.meta.as<int4>();
