@@ -34,6 +34,8 @@ WORKLOADS = {
     "stencil27-128-aat": dict(kind="stencil27", g=128, aat=1),
     "stencil27-64-aat": dict(kind="stencil27", g=64, aat=1),
     "laplace2d-64-aa": dict(kind="laplace2d", g=64, aat=0),
+    "rmat-20-aa": dict(kind="rmat", scale=20, aat=0),
+    "rmat-18-aa": dict(kind="rmat", scale=18, aat=0),
 }
 
 
@@ -41,6 +43,10 @@ def make_matrix(rt, spec):
     if spec["kind"] == "stencil27":
         return rt.gen_stencil27(spec["g"], 0.5, 1.5, 20260101), (
             f"3D 27-point stencil {spec['g']}^3, values U[0.5,1.5], seed 20260101")
+    if spec["kind"] == "rmat":
+        return rt.gen_rmat(spec["scale"], 16, 0.57, 0.19, 0.19, 20260101), (
+            f"R-MAT 2^{spec['scale']}, 16 draws per row, (a,b,c,d) = (0.57,0.19,0.19,0.05), "
+            f"duplicates removed, values U[-1,1], seed 20260101")
     return rt.gen_laplace2d(spec["g"], 0.1, 20260101), (
         f"2D 5-point Laplacian {spec['g']}^2, off-diagonals -1+U[-0.1,0.1], seed 20260101")
 

@@ -44,6 +44,7 @@ constexpr int kSeg = 1024;         // pairs per segment-table window
 constexpr int kSymSlots = 1024;    // LDS hash slots (power of 2)
 constexpr int kSymSlotBits = 10;
 constexpr int kSymPerThread = kSymSlots / kThreads;
+constexpr int kSymProbe = 32;      // longest probe before a row goes to the slab path
 constexpr int kSymBatch = 4;       // pairs whose loads a thread issues together
 
 constexpr int kNumChunk = 64;      // A tiles per numeric chunk
@@ -278,7 +279,10 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
                         continue;
                     const int j = bm[u].x;
                     uint32_t h = slot_of(j, kSymSlotBits);
-                    for (int probe = 0; probe < kSymSlots; ++probe) {
+                    // A probe longer than kSymProbe means the table is close
+                    // to full: the row is sent to the slab path, which is
+                    // always correct, instead of probing on.
+                    for (int probe = 0; probe < kSymProbe; ++probe) {
                         int k = s_key[h];
                         if (k == kEmpty) {
                             k = atomicCAS(&s_key[h], kEmpty, j);
@@ -292,19 +296,21 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
                             break;
                         }
                         h = (h + 1) & (kSymSlots - 1);
-                        if (probe == kSymSlots - 1)
+                        if (probe == kSymProbe - 1)
                             s_ovf = 1;
                     }
                 }
             }
             __syncthreads();
+            if (s_ovf)
+                break;
         }
         if (s_ovf)
             break;
     }
     __syncthreads();
     // Compact the occupied slots to the front; the scan total is the row's
-    // tile count.  A full table (a probe that found no slot) or more tiles
+    // tile count.  A probe that ran out (a table close to full) or more tiles
     // than the staging holds sends the row to the slab path.
     int rk[kSymPerThread];
     unsigned long long rm[kSymPerThread];
@@ -525,30 +531,27 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
             s_hi = -1;
         }
         __syncthreads();
+        // The column range the row's masks land in, from the pairs
+        // themselves (the tiles of a B row are not in column order).
+        int jlo = INT_MAX, jhi = -1;
         for (int c0 = a0; c0 < a1; c0 += kThreads) {
             const int n = min(kThreads, a1 - c0);
-            if (tid < n) {
-                // A B row is two column-sorted runs: its heavy tiles, then
-                // its light ones.
-                const int4 br = g.a_brange[c0 + tid];
-                if (br.y > 0) {
-                    atomicMin(&s_lo, g.b_meta[br.x].x);
-                    atomicMax(&s_hi, g.b_meta[br.x + br.y - 1].x);
-                }
-                if (br.z > br.y) {
-                    atomicMin(&s_lo, g.b_meta[br.x + br.y].x);
-                    atomicMax(&s_hi, g.b_meta[br.x + br.z - 1].x);
-                }
-            }
             const int total = load_slab_chunk(g, c0, n, s_ab, s_b0, s_pre, s_tmp);
             for (int p = tid; p < total; p += kThreads) {
                 const int ai = upper_index(s_pre, n, p);
                 const int4 bm = g.b_meta[s_b0[ai] + (p - s_pre[ai])];
                 const uint64_t pm = bytes_product(s_ab[ai], static_cast<uint32_t>(bm.z));
-                if (pm)
+                if (pm) {
                     atomicOr(&slab[bm.x], static_cast<unsigned long long>(pm));
+                    jlo = min(jlo, bm.x);
+                    jhi = max(jhi, bm.x);
+                }
             }
             __syncthreads();
+        }
+        if (jhi >= 0) {
+            atomicMin(&s_lo, jlo);
+            atomicMax(&s_hi, jhi);
         }
         __threadfence();
         __syncthreads();

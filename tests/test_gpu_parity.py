@@ -224,6 +224,33 @@ def test_wide_rows_take_the_slab_paths(gpu, aat):
     assert stats["heavy_rows"] > 0
 
 
+@pytest.mark.parametrize("scale,aat", [(12, 0), (14, 0), (12, 1)])
+def test_rmat_power_law(gpu, scale, aat):
+    # R-MAT rows follow a power law: hub rows with thousands of entries give
+    # C tile rows wider than the LDS tables, so many rows take the slab
+    # paths, and B rows mix heavy and light tiles of every class.
+    a = gpu.gen_rmat(scale, 16, 0.57, 0.19, 0.19, 20260101)
+    _, _, _, stats, _ = run(gpu, a, aat)
+    assert stats["heavy_rows"] > 0
+
+
+def test_repeat_calls_are_deterministic_on_slab_rows(gpu):
+    # Slab rows share per-workgroup slabs across the rows a workgroup takes;
+    # the structure of C must not depend on which rows those are.
+    a = gpu.gen_rmat(14, 16, 0.57, 0.19, 0.19, 7)
+    pa = gpu.prepare_identity_a(a)
+    pb = gpu.rebuild_b(pa, 0)
+    tc1, s1 = gpu.tilespgemm(pa, pb)
+    pb = gpu.rebuild_b(pa, 0, pb)
+    tc2, s2 = gpu.tilespgemm(pa, pb)
+    assert s1["heavy_rows"] > 0
+    assert (s1["output_tiles"], s1["output_nnz"]) == (s2["output_tiles"], s2["output_nnz"])
+    for k in ("tile_ptr", "tile_columnidx", "tile_nnz", "tile_csr_Ptr", "tile_csr_Col"):
+        assert np.array_equal(getattr(tc1, k), getattr(tc2, k)), k
+    # FP64 sums in another order: equal to rounding.
+    np.testing.assert_allclose(tc1.tile_csr_Value, tc2.tile_csr_Value, rtol=1e-13, atol=1e-13)
+
+
 def test_rectangular_aat_and_empty_shapes(gpu):
     a = gpu.gen_random(37, 300, 5, 9)
     run(gpu, a, 1)
