@@ -765,7 +765,7 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
                     uint32_t rk = 0;
                     const double *ap = acol;
                     double av[kColRegs];
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < kColRegs; ++i)
                         av[i] = 0.0;
                     // Products of B entry (k, cl) with the rows of A's column k.
@@ -775,12 +775,12 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
                             kcur = k;
                             rk = byte_of(ab, k);
                             ap = acol + byte_of(apf, k);
-    #pragma unroll
+#pragma unroll
                             for (int i = 0; i < kColRegs; ++i)
                                 av[i] = ap[i];
                         }
                         uint32_t rr = rk;
-    #pragma unroll
+#pragma unroll
                         for (int i = 0; i < kColRegs; ++i) {
                             if (rr) {
                                 const int r = __builtin_ctz(rr);
@@ -798,7 +798,7 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
                         }
                     };
                     uint32_t mz = static_cast<uint32_t>(rec.meta.z);
-    #pragma unroll
+#pragma unroll
                     for (int j = 0; j < kRecVals; ++j) {
                         if (mz) {
                             const int pos = __builtin_ctz(mz);
