@@ -273,9 +273,9 @@ __global__ void row_ptr_from_sorted_kernel(const int *__restrict__ keys, int64_t
         ptr[k] = static_cast<int>(i);
 }
 
-// One 80-byte record per B tile (meta + the first kRecVals values of the
-// tile's k-major payload: row k's entries contiguous, columns ascending),
-// written as five 16-byte parts by five consecutive threads.
+// One record per B tile (meta + the first kRecVals values of the tile's
+// k-major payload: row k's entries contiguous, columns ascending), written
+// as kRecParts 16-byte parts by as many consecutive threads.
 __device__ __forceinline__ void write_record_part(BRecord *recs, int64_t t, int part,
                                                   const int4 &m, const double *kmajor)
 {
@@ -299,10 +299,10 @@ __global__ void records_kernel(int64_t n, const int4 *__restrict__ meta,
                                const double *__restrict__ kmajor, BRecord *__restrict__ recs)
 {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t t = gid / 5;
+    const int64_t t = gid / kRecParts;
     if (t >= n)
         return;
-    write_record_part(recs, t, static_cast<int>(gid % 5), meta[t], kmajor);
+    write_record_part(recs, t, static_cast<int>(gid % kRecParts), meta[t], kmajor);
 }
 
 // Offline, per A tile: the record a B tile built as A^T takes over, its meta
@@ -314,12 +314,12 @@ __global__ void transpose_source_kernel(int64_t n, const int *__restrict__ a_row
                                         BRecord *__restrict__ out)
 {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t t = gid / 5;
+    const int64_t t = gid / kRecParts;
     if (t >= n)
         return;
     int4 m = a_meta[t];
     m.x = a_row_of[t];
-    write_record_part(out, t, static_cast<int>(gid % 5), m, a_vals_cm);
+    write_record_part(out, t, static_cast<int>(gid % kRecParts), m, a_vals_cm);
 }
 
 // B = A^T: B tile b is A tile ids[b], so its record is a copy of that A
@@ -329,10 +329,10 @@ __global__ void transposed_records_kernel(int64_t n, const int *__restrict__ ids
                                           int4 *__restrict__ b_meta, BRecord *__restrict__ recs)
 {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t t = gid / 5;
+    const int64_t t = gid / kRecParts;
     if (t >= n)
         return;
-    const int part = static_cast<int>(gid % 5);
+    const int part = static_cast<int>(gid % kRecParts);
     const int4 v = reinterpret_cast<const int4 *>(src + ids[t])[part];
     reinterpret_cast<int4 *>(recs + t)[part] = v;
     if (part == 0)
@@ -409,10 +409,10 @@ __global__ void gather_records_kernel(int64_t n, const int *__restrict__ perm,
                                       int4 *__restrict__ meta_out, BRecord *__restrict__ rec_out)
 {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t t = gid / 5;
+    const int64_t t = gid / kRecParts;
     if (t >= n)
         return;
-    const int part = static_cast<int>(gid % 5);
+    const int part = static_cast<int>(gid % kRecParts);
     const int src = perm[t];
     reinterpret_cast<int4 *>(rec_out + t)[part] = reinterpret_cast<const int4 *>(rec_in + src)[part];
     if (part == 0)
@@ -641,7 +641,7 @@ int build_records(DevTiles *t)
     RTT_TRY(build_colmajor_values(t));
     RTT_TRY(t->records.ensure(t->num_tiles * sizeof(BRecord)));
     if (t->num_tiles > 0)
-        hipLaunchKernelGGL(records_kernel, dim3(blocks_for(t->num_tiles * 5, kThreads)),
+        hipLaunchKernelGGL(records_kernel, dim3(blocks_for(t->num_tiles * kRecParts, kThreads)),
                            dim3(kThreads), 0, ctx().stream, t->num_tiles, t->meta.as<int4>(),
                            t->values_cm.as<double>(), t->records.as<BRecord>());
     RTT_HIP(hipGetLastError());
@@ -682,7 +682,7 @@ int order_by_class(DevTiles *b, BuildWork *w)
                            w->ids_in.as<int>(), n);
         RTT_TRY(sort_pairs(w->keys_in.as<int>(), w->keys_out.as<int>(), w->ids_in.as<int>(),
                            w->ids_out.as<int>(), n, bits_for(slots), &w->scan_tmp, s));
-        hipLaunchKernelGGL(gather_records_kernel, dim3(blocks_for(n * 5, kThreads)),
+        hipLaunchKernelGGL(gather_records_kernel, dim3(blocks_for(n * kRecParts, kThreads)),
                            dim3(kThreads), 0, s, n, w->ids_out.as<int>(), b->meta.as<int4>(),
                            b->records.as<BRecord>(), w->meta2.as<int4>(),
                            w->records2.as<BRecord>());
@@ -748,7 +748,7 @@ int build_transpose_source(DevTiles *t)
     RTT_TRY(t->transpose_src.ensure(static_cast<size_t>(t->num_tiles > 0 ? t->num_tiles : 1) *
                                     sizeof(BRecord)));
     if (t->num_tiles > 0)
-        hipLaunchKernelGGL(transpose_source_kernel, dim3(blocks_for(t->num_tiles * 5, kThreads)),
+        hipLaunchKernelGGL(transpose_source_kernel, dim3(blocks_for(t->num_tiles * kRecParts, kThreads)),
                            dim3(kThreads), 0, ctx().stream, t->num_tiles,
                            t->tile_row_of.as<int>(), t->meta.as<int4>(), t->values_cm.as<double>(),
                            t->transpose_src.as<BRecord>());
@@ -923,7 +923,7 @@ int transpose_tiles(const DevTiles &a, int b_cols, DevTiles *b, BuildWork *w)
         RTT_TRY(sort_pairs(w->keys_in.as<int>(), w->keys_out.as<int>(), w->ids_in.as<int>(),
                            b->sorted_ids.as<int>(), n, bits_for(slots), &w->scan_tmp, s));
         RTT_TRY(b->records.ensure(n * sizeof(BRecord)));
-        hipLaunchKernelGGL(transposed_records_kernel, dim3(blocks_for(n * 5, kThreads)),
+        hipLaunchKernelGGL(transposed_records_kernel, dim3(blocks_for(n * kRecParts, kThreads)),
                            dim3(kThreads), 0, s, n, b->sorted_ids.as<int>(),
                            a.transpose_src.as<BRecord>(), b->meta.as<int4>(),
                            b->records.as<BRecord>());

@@ -25,12 +25,14 @@ constexpr int kClassSlots = 8;   // class slots per B row: heavy, light k = 0..3
 
 // Per-B-tile record read by the numeric kernel: the packed meta and the
 // first kRecVals payload values, so that one (A tile, B tile) pair costs one
-// contiguous 80-byte load.
+// contiguous load of kRecParts 16-byte parts.
 constexpr int kRecVals = 8;
 struct alignas(16) BRecord {
     int4 meta;
     double v[kRecVals];
 };
+constexpr int kRecParts = 1 + kRecVals / 2;
+static_assert(kRecVals % 2 == 0 && sizeof(BRecord) == 16 * kRecParts, "record layout");
 
 // Checks a HIP call; on failure records the error and returns RTT_ERR_DEVICE.
 #define RTT_HIP(call)                                                            \

@@ -48,7 +48,8 @@ constexpr int kSymProbe = 32;      // longest probe before a row goes to the sla
 constexpr int kSymBatch = 4;       // pairs whose loads a thread issues together
 
 constexpr int kNumChunk = 64;      // A tiles per numeric chunk
-constexpr int kColRegs = 4;        // values of an A column a numeric lane holds in registers
+constexpr int kColRegs = 3;         // values of an A column a numeric lane holds in registers
+constexpr int kExtVals = 3;         // B values past the record a numeric lane prefetches
 static_assert(kThreads == 4 * kNumChunk, "one numeric thread per (pair run, chunk tile)");
 
 // LDS budgets of the two numeric instances: accumulator slots, staged A
@@ -596,19 +597,74 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
 // ---------------------------------------------------------------------------
 // Numeric, LDS path: one workgroup per row, LDS accumulator.
 // ---------------------------------------------------------------------------
-// The row's exact output segment is an FP64 accumulator in LDS, every C tile
-// of the row is in a hash table, and each thread takes the pairs p,
+// The row's output segment is an FP64 accumulator in LDS, every C tile of
+// the row is in a hash table, and each thread takes the pairs p,
 // p + kThreads, ... of a window.  A pair walks its B tile's entries in
 // k-major order (its record holds them so): when the walk reaches row k of
 // B, the lane loads the first values of column k of the A tile (staged
-// column-major in LDS) into registers, and every product of the row-k
-// entries with that column is one FMA into the accumulator slot given by a
-// popcount of the C tile mask.
+// column-major in LDS) into registers, and B entry (k, cl) adds
+// A(r, k) * B(k, cl) into C(r, cl) for every row r of that column.
+//
+// The accumulator has the output's layout: a tile's entries row by row,
+// columns ascending.  When every row of the C tile is one contiguous run of
+// columns and every column of the A tile one contiguous run of rows, the
+// slot of C(r, cl) is (first slot of row r - its first column) + cl: the
+// lane takes those bases for the rows of A column k once per k, and each
+// product costs one add.  Other tiles take the slot from a popcount of the
+// tile mask.
 struct alignas(16) CTileInfo {
-    unsigned long long mask;
-    int base;      // first value of the tile within the row's segment
-    int pad;
+    unsigned long long runs;    // row_runs of the mask
+    unsigned long long mask;    // bit r * 8 + cl
 };
+
+// Byte value of a row whose columns are not one contiguous run.
+constexpr uint32_t kNoRun = 0x80u;
+
+__device__ __forceinline__ bool is_run(uint32_t bits)
+{
+    const uint32_t x = bits >> __builtin_ctz(bits);
+    return (x & (x + 1)) == 0;
+}
+
+// Byte r: (slot of row r's first entry within the tile) - (its first
+// column) when the row's columns are contiguous, kNoRun when they are not,
+// 0 for an empty row (never addressed).
+__device__ __forceinline__ uint64_t row_runs(uint64_t m)
+{
+    uint64_t out = 0;
+    int first = 0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const uint32_t bits = static_cast<uint32_t>(m >> (8 * r)) & 0xFFu;
+        uint32_t b = 0;
+        if (bits)
+            b = is_run(bits) ? static_cast<uint32_t>(first - __builtin_ctz(bits)) & 0xFFu : kNoRun;
+        out |= static_cast<uint64_t>(b) << (8 * r);
+        first += __popc(bits);
+    }
+    return out;
+}
+
+__device__ __forceinline__ int run_base(uint64_t runs, int r)
+{
+    return static_cast<int8_t>((runs >> (8 * r)) & 0xFFu);
+}
+
+// Byte k: rows of A column k as count | first row << 4, with bit 7 set when
+// they are not one contiguous run of at most kColRegs rows.
+__device__ __forceinline__ uint32_t a_column_runs(uint32_t ab)
+{
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t rk = byte_of(ab, k);
+        if (rk) {
+            const bool regs = is_run(rk) && __popc(rk) <= kColRegs;
+            out |= ((__popc(rk) | (__builtin_ctz(rk) << 4) | (regs ? 0u : kNoRun)) << (8 * k));
+        }
+    }
+    return out;
+}
 
 template <class Cfg>
 __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
@@ -619,6 +675,7 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     __shared__ double s_acc[Cfg::kAcc];
     __shared__ double s_acol[Cfg::kAStage + kColRegs];
     __shared__ CTileInfo s_ct[Cfg::kTiles];
+    __shared__ int s_cbase[Cfg::kTiles];   // first slot of each tile within the row's segment
     __shared__ int s_hk[Cfg::kSlots];
     __shared__ QT s_hq[Cfg::kSlots];
     // Per chunk A tile: its staged info and B runs (see stage_tile), and per
@@ -646,12 +703,12 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     const int a_len = g.a_row_off[row + 1] - a_base;
     if (nt > Cfg::kTiles || nnz_row > Cfg::kAcc || a_len > Cfg::kAStage)
         return;
-    int j0 = 0;
-    CTileInfo ct0{};
+    int j0 = 0, base0 = 0;
+    uint64_t m0 = 0;
     if (tid < nt) {
         j0 = c.col[t0 + tid];
-        ct0.mask = c.mask[t0 + tid];
-        ct0.base = static_cast<int>(c.tile_nnz[t0 + tid] - zb);
+        m0 = c.mask[t0 + tid];
+        base0 = static_cast<int>(c.tile_nnz[t0 + tid] - zb);
     }
     double av0 = 0.0;
     if (tid < a_len)
@@ -675,32 +732,37 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
         s_acol[a_len + tid] = 0.0;   // the register loads of a column may run past it
     __syncthreads();
     for (int q = tid; q < nt; q += kThreads) {
-        int j = j0;
-        CTileInfo ct = ct0;
+        int j = j0, base = base0;
+        uint64_t m = m0;
         if (q != tid) {
             j = c.col[t0 + q];
-            ct.mask = c.mask[t0 + q];
-            ct.base = static_cast<int>(c.tile_nnz[t0 + q] - zb);
+            m = c.mask[t0 + q];
+            base = static_cast<int>(c.tile_nnz[t0 + q] - zb);
         }
-        ct.pad = 0;
+        CTileInfo ct;
+        ct.runs = row_runs(m);
+        ct.mask = m;
         s_ct[q] = ct;
+        s_cbase[q] = base;
         uint32_t h = slot_of(j, Cfg::kSlotBits);
         while (atomicCAS(&s_hk[h], kEmpty, j) != kEmpty)
             h = (h + 1) & (Cfg::kSlots - 1);
         s_hq[h] = static_cast<QT>(q);
     }
-    // Chunk tile info: {rows present in column k (byte k), first staged
-    // value of column k (byte k), first staged value, heavy} and its B runs
-    // {first heavy tile, heavy count, first light tile, light count}.
+    // Chunk tile info: {rows present in column k (byte k), heavy (bit 0) and
+    // first staged value of column k (byte k, k > 0), first staged value,
+    // row runs of the columns (a_column_runs)} and its B runs {first heavy
+    // tile, heavy count, first light tile, light count}.
     auto stage_tile = [&](const int4 &am, const int4 &br, const int2 &bl) {
         const uint32_t ab = static_cast<uint32_t>(am.z);
         const uint32_t p1 = __popc(byte_of(ab, 0));
         const uint32_t p2 = p1 + __popc(byte_of(ab, 1));
         const uint32_t p3 = p2 + __popc(byte_of(ab, 2));
+        const uint32_t heavy = __popc(static_cast<uint32_t>(am.y)) >= kHeavyMin ? 1u : 0u;
         s_at[tid] = make_int4(static_cast<int>(ab),
-                              static_cast<int>((p1 << 8) | (p2 << 16) | (p3 << 24)),
+                              static_cast<int>(heavy | (p1 << 8) | (p2 << 16) | (p3 << 24)),
                               static_cast<int>(static_cast<uint32_t>(am.w) & ~kDenseBit) - a_base,
-                              __popc(static_cast<uint32_t>(am.y)) >= kHeavyMin ? 1 : 0);
+                              static_cast<int>(a_column_runs(ab)));
         s_br[tid] = make_int4(br.x, br.y, bl.x, bl.y);
     };
     if (tid < min(kNumChunk, a1 - a0))
@@ -716,7 +778,7 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
         // stands for run v / kNumChunk of tile v % kNumChunk.
         const int run = tid / kNumChunk, vt = tid % kNumChunk;
         int nb = 0, bstart = 0;
-        if (vt < n && s_at[vt].w == ((run & 1) == 0 ? 1 : 0)) {
+        if (vt < n && (s_at[vt].y & 1) == ((run & 1) == 0 ? 1 : 0)) {
             const int4 br = s_br[vt];
             nb = run < 2 ? br.y : br.w;
             bstart = run < 2 ? br.x : br.z;
@@ -755,63 +817,95 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
                     }
                     if (q < 0)
                         continue;
-                    const CTileInfo ct = s_ct[q];
-                    double *acc = s_acc + ct.base;
-                    const uint64_t cm = ct.mask;
-                    const uint32_t apf = static_cast<uint32_t>(at.y);
+                    const unsigned long long cruns = s_ct[q].runs;
+                    double *acc = s_acc + s_cbase[q];
+                    const uint32_t apf = static_cast<uint32_t>(at.y) & ~0xFFu;
+                    const uint32_t aruns = static_cast<uint32_t>(at.w);
                     const double *acol = s_acol + at.z;
                     const uint32_t boff = static_cast<uint32_t>(rec.meta.w) & ~kDenseBit;
-                    int kcur = -1;
-                    uint32_t rk = 0;
-                    const double *ap = acol;
-                    double av[kColRegs];
+                    // B entries in k-major order: the record's values, then the
+                    // payload's.  The first kExtVals payload values are loaded once
+                    // kExtVals record values have been used, into the registers
+                    // those held, and arrive while the rest of the record is walked.
+                    auto walk = [&](auto &&entry) {
+                        uint32_t mz = static_cast<uint32_t>(rec.meta.z);
+                        const int nbv = __popc(mz);
+                        double ext[kExtVals > 0 ? kExtVals : 1];
 #pragma unroll
-                    for (int i = 0; i < kColRegs; ++i)
-                        av[i] = 0.0;
-                    // Products of B entry (k, cl) with the rows of A's column k.
-                    auto entry = [&](int pos, double vb) {
-                        const int k = pos >> 3, cl = pos & 7;
-                        if (k != kcur) {
-                            kcur = k;
-                            rk = byte_of(ab, k);
-                            ap = acol + byte_of(apf, k);
+                        for (int j = 0; j < kRecVals; ++j) {
+                            if (j == kExtVals) {
+#pragma unroll
+                                for (int i = 0; i < kExtVals; ++i)
+                                    ext[i] = kRecVals + i < nbv ? g.b_vals_km[boff + kRecVals + i]
+                                                                : 0.0;
+                            }
+                            if (mz) {
+                                const int pos = __builtin_ctz(mz);
+                                mz &= mz - 1;
+                                entry(pos, rec.v[j]);
+                            }
+                        }
+#pragma unroll
+                        for (int i = 0; i < kExtVals; ++i) {
+                            if (mz) {
+                                const int pos = __builtin_ctz(mz);
+                                mz &= mz - 1;
+                                entry(pos, ext[i]);
+                            }
+                        }
+                        int j = kRecVals + kExtVals;
+                        while (mz) {
+                            const int pos = __builtin_ctz(mz);
+                            mz &= mz - 1;
+                            entry(pos, g.b_vals_km[boff + j]);
+                            ++j;
+                        }
+                    };
+                    if (((cruns & 0x8080808080808080ull) | (aruns & 0x80808080u)) == 0) {
+                        // All runs, and no A column longer than kColRegs: at each
+                        // new k the lane takes the values and the row bases of the
+                        // rows r0, r0 + 1, ... of A column k into registers.
+                        static_assert(kColRegs <= 4, "row bases of one dword");
+                        int kcur = -1;
+                        uint32_t kd = 0;   // a_column_runs byte of column kcur
+                        uint32_t rb = 0;   // byte i: row base of row r0 + i
+                        double av[kColRegs];
+#pragma unroll
+                        for (int i = 0; i < kColRegs; ++i)
+                            av[i] = 0.0;
+                        walk([&](int pos, double vb) {
+                            const int k = pos >> 3, cl = pos & 7;
+                            if (k != kcur) {
+                                kcur = k;
+                                kd = byte_of(aruns, k);
+                                rb = static_cast<uint32_t>(cruns >> (8 * (kd >> 4)));
+                                const double *ap = acol + byte_of(apf, k);
+#pragma unroll
+                                for (int i = 0; i < kColRegs; ++i)
+                                    av[i] = ap[i];
+                            }
+                            double *pc = acc + cl;
+                            const int na = static_cast<int>(kd & 15u);
 #pragma unroll
                             for (int i = 0; i < kColRegs; ++i)
-                                av[i] = ap[i];
-                        }
-                        uint32_t rr = rk;
-#pragma unroll
-                        for (int i = 0; i < kColRegs; ++i) {
-                            if (rr) {
+                                if (i < na)
+                                    atomicAdd(pc + run_base(rb, i), av[i] * vb);
+                        });
+                    } else {
+                        // Slots from popcounts of the tile mask.
+                        const uint64_t cm = s_ct[q].mask;
+                        walk([&](int pos, double vb) {
+                            const int k = pos >> 3, cl = pos & 7;
+                            const double *ap = acol + byte_of(apf, k);
+                            uint32_t rr = byte_of(ab, k);
+                            int i = 0;
+                            while (rr) {
                                 const int r = __builtin_ctz(rr);
                                 rr &= rr - 1;
                                 atomicAdd(&acc[__popcll(cm & ((1ull << (r * 8 + cl)) - 1ull))],
-                                          av[i] * vb);
+                                          ap[i++] * vb);
                             }
-                        }
-                        int i = kColRegs;
-                        while (rr) {
-                            const int r = __builtin_ctz(rr);
-                            rr &= rr - 1;
-                            atomicAdd(&acc[__popcll(cm & ((1ull << (r * 8 + cl)) - 1ull))],
-                                      ap[i++] * vb);
-                        }
-                    };
-                    uint32_t mz = static_cast<uint32_t>(rec.meta.z);
-#pragma unroll
-                    for (int j = 0; j < kRecVals; ++j) {
-                        if (mz) {
-                            const int pos = __builtin_ctz(mz);
-                            mz &= mz - 1;
-                            entry(pos, rec.v[j]);
-                        }
-                    }
-                    int j = kRecVals;
-                    while (mz) {
-                        const int pos = __builtin_ctz(mz);
-                        mz &= mz - 1;
-                        entry(pos, g.b_vals_km[boff + j]);
-                        ++j;
+                        });
                     }
                 }
                 __syncthreads();
