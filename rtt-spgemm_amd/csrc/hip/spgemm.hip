@@ -47,6 +47,13 @@ constexpr int kSymPerThread = kSymSlots / kThreads;
 constexpr int kSymProbe = 32;      // longest probe before a row goes to the slab path
 constexpr int kSymBatch = 4;       // pairs whose loads a thread issues together
 
+constexpr int kWSlots = 256;       // hash slots of the one-wave symbolic table
+constexpr int kWSlotBits = 8;
+static_assert(kWSlots == 1 << kWSlotBits, "one-wave table size");
+constexpr int kWStage = 3 * kWSlots / 4;   // C tiles a row may have there
+constexpr int kWChunk = kWave / 4; // A tiles per one-wave enumeration step
+constexpr int kWSeg = 512;         // pairs per one-wave segment window
+
 constexpr int kNumChunk = 64;      // A tiles per numeric chunk
 constexpr int kColRegs = 3;         // values of an A column a numeric lane holds in registers
 constexpr int kExtVals = 3;         // B values past the record a numeric lane prefetches
@@ -68,7 +75,9 @@ constexpr int kWriteBuf = 2048;    // column bytes staged per row in the write p
 constexpr int kEmpty = -1;
 constexpr uint32_t kDenseBit = 0x80000000u;
 
-enum RowLevel : int { kLevelEmpty = 0, kLevelLds = 1, kLevelSlab = 3 };
+// kLevelLarge: the row outgrew the wave table and waits for the workgroup
+// table; kLevelSlab: it outgrew that too.
+enum RowLevel : int { kLevelEmpty = 0, kLevelLds = 1, kLevelLarge = 2, kLevelSlab = 3 };
 
 struct Operands {
     int a_tile_rows;
@@ -205,11 +214,187 @@ __device__ __forceinline__ void fill_segments(unsigned char *s_seg, int pre, int
 }
 
 // ---------------------------------------------------------------------------
+// Symbolic, one wave per row: the same hash -> compact -> rank -> stage as the
+// workgroup kernel below, with a kWSlots table, so that a CU holds four times
+// as many rows and their load latencies overlap.  Pairs come in the numeric
+// kernel's four runs (A class x B class) per chunk of kWChunk A tiles, one
+// lane per (run, tile), and a pair's mask sums only the k where both A's
+// column and B's row are nonempty.  A row that outgrows the table is marked
+// kLevelLarge for the workgroup kernel.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t nonzero_bytes(uint32_t v)
+{
+    const uint32_t nz = (((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u;
+    return ((nz >> 7) | (nz >> 14) | (nz >> 21) | (nz >> 28)) & 0xFu;
+}
+
+__global__ __launch_bounds__(kWave) void symbolic_wave_kernel(
+    Operands g, Stage st, int64_t *__restrict__ row_tiles, int64_t *__restrict__ row_nnz,
+    int *__restrict__ row_level)
+{
+    __shared__ __align__(16) int s_key[kWSlots];
+    __shared__ unsigned long long s_msk[kWSlots];
+    __shared__ unsigned long long s_spread[4][kWChunk];
+    __shared__ int4 s_br[kWChunk];
+    __shared__ int s_acls[kWChunk];   // nonempty columns (bits 0-3), heavy (bit 4)
+    __shared__ int s_pw[kWave];
+    __shared__ unsigned char s_seg[kWSeg];
+
+    const int row = xcd_block(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x;
+    const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
+    for (int i = lane; i < kWSlots; i += kWave) {
+        s_key[i] = kEmpty;
+        s_msk[i] = 0ull;
+    }
+    bool ovf = false;
+    for (int c0 = a0; c0 < a1 && !ovf; c0 += kWChunk) {
+        const int n = min(kWChunk, a1 - c0);
+        if (lane < n) {
+            const int4 am = g.a_meta[c0 + lane];
+            const int4 br = g.a_brange[c0 + lane];
+            const int2 bl = g.a_blight[c0 + lane];
+            const uint32_t ab = static_cast<uint32_t>(am.z);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                s_spread[k][lane] = rows_to_bytes(byte_of(ab, k));
+            s_br[lane] = make_int4(br.x, br.y, bl.x, bl.y);
+            s_acls[lane] = static_cast<int>(nonzero_bytes(ab)) |
+                           (__popc(static_cast<uint32_t>(am.y)) >= kHeavyMin ? 16 : 0);
+        }
+        __syncthreads();
+        const int run = lane / kWChunk, vt = lane % kWChunk;
+        int nb = 0, bstart = 0;
+        if (vt < n && ((s_acls[vt] >> 4) & 1) == ((run & 1) == 0 ? 1 : 0)) {
+            const int4 br = s_br[vt];
+            nb = run < 2 ? br.y : br.w;
+            bstart = run < 2 ? br.x : br.z;
+        }
+        const int incl = wave_inclusive_scan(nb);
+        const int total = __shfl(incl, kWave - 1, kWave);
+        const int pre = incl - nb;
+        s_pw[lane] = bstart - pre;
+        for (int r0 = 0; r0 < total && !ovf; r0 += kWSeg) {
+            const int r1 = min(total, r0 + kWSeg);
+            if (nb > 0)
+                fill_segments(s_seg, pre, nb, r0, r1);
+            __syncthreads();
+            bool lane_ovf = false;
+            for (int p0 = r0 + lane; p0 < r1; p0 += kSymBatch * kWave) {
+                int4 bm[kSymBatch];
+                int ai[kSymBatch];
+#pragma unroll
+                for (int u = 0; u < kSymBatch; ++u) {
+                    const int p = p0 + u * kWave;
+                    ai[u] = -1;
+                    bm[u] = make_int4(0, 0, 0, 0);
+                    if (p < r1) {
+                        const int v = s_seg[p - r0];
+                        ai[u] = v % kWChunk;
+                        bm[u] = g.b_meta[s_pw[v] + p];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kSymBatch; ++u) {
+                    if (ai[u] < 0)
+                        continue;
+                    const uint32_t bb = static_cast<uint32_t>(bm[u].z);
+                    uint32_t kk = static_cast<uint32_t>(s_acls[ai[u]]) & nonzero_bytes(bb);
+                    uint64_t pm = 0;
+                    while (kk) {
+                        const int k = __builtin_ctz(kk);
+                        kk &= kk - 1;
+                        const uint32_t b4 = byte_of(bb, k) * 0x01010101u;
+                        pm |= s_spread[k][ai[u]] & ((static_cast<uint64_t>(b4) << 32) | b4);
+                    }
+                    if (!pm)
+                        continue;
+                    const int j = bm[u].x;
+                    uint32_t h = slot_of(j, kWSlotBits);
+                    for (int probe = 0; probe < kSymProbe; ++probe) {
+                        int k = s_key[h];
+                        if (k == kEmpty) {
+                            k = atomicCAS(&s_key[h], kEmpty, j);
+                            if (k == kEmpty) {
+                                atomicOr(&s_msk[h], static_cast<unsigned long long>(pm));
+                                break;
+                            }
+                        }
+                        if (k == j) {
+                            atomicOr(&s_msk[h], static_cast<unsigned long long>(pm));
+                            break;
+                        }
+                        h = (h + 1) & (kWSlots - 1);
+                        if (probe == kSymProbe - 1)
+                            lane_ovf = true;
+                    }
+                }
+            }
+            ovf = __any(lane_ovf);
+            __syncthreads();
+        }
+    }
+    // Compact the occupied slots to the front; the scan total is the tile count.
+    constexpr int kPer = kWSlots / kWave;
+    int rk[kPer];
+    unsigned long long rm[kPer];
+    int mine = 0;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        rk[i] = s_key[lane * kPer + i];
+        rm[i] = s_msk[lane * kPer + i];
+        mine += rk[i] != kEmpty ? 1 : 0;
+    }
+    const int incl = wave_inclusive_scan(mine);
+    const int nt = __shfl(incl, kWave - 1, kWave);
+    if (ovf || nt > kWStage) {
+        if (lane == 0)
+            row_level[row] = kLevelLarge;
+        return;
+    }
+    __syncthreads();
+    int pos = incl - mine;
+    long long nz = 0;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i)
+        if (rk[i] != kEmpty) {
+            s_key[pos] = rk[i];
+            s_msk[pos] = rm[i];
+            nz += __popcll(rm[i]);
+            ++pos;
+        }
+    __syncthreads();
+    // Rank by counting: entry t goes to the number of smaller columns.
+    const int64_t sb = static_cast<int64_t>(row) * st.stride;
+    const int4 *k4 = reinterpret_cast<const int4 *>(s_key);
+    const int full = nt & ~3;
+    for (int t = lane; t < nt; t += kWave) {
+        const int mj = s_key[t];
+        int rank = 0;
+        for (int f = 0; f < full / 4; ++f) {
+            const int4 kk = k4[f];
+            rank += (kk.x < mj ? 1 : 0) + (kk.y < mj ? 1 : 0) + (kk.z < mj ? 1 : 0) +
+                    (kk.w < mj ? 1 : 0);
+        }
+        for (int f = full; f < nt; ++f)
+            rank += s_key[f] < mj ? 1 : 0;
+        st.j[sb + rank] = mj;
+        st.m[sb + rank] = s_msk[t];
+    }
+    nz = wave_sum(nz);
+    if (lane == 0) {
+        row_level[row] = nt > 0 ? kLevelLds : kLevelEmpty;
+        row_tiles[row] = nt;
+        row_nnz[row] = nz;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Symbolic: per-row hash of C tile masks, compaction, ranking, staging.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
-    Operands g, int stage_limit, Stage st, int64_t *__restrict__ row_tiles,
-    int64_t *__restrict__ row_nnz, int *__restrict__ row_level)
+    Operands g, int stage_limit, Stage st, const int *__restrict__ rows,
+    int64_t *__restrict__ row_tiles, int64_t *__restrict__ row_nnz, int *__restrict__ row_level)
 {
     __shared__ __align__(16) int s_key[kSymSlots];
     __shared__ unsigned long long s_msk[kSymSlots];
@@ -221,7 +406,7 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
     __shared__ long long s_tmp64[kWavesPerBlock];
     __shared__ int s_ovf;
 
-    const int row = xcd_block(blockIdx.x, gridDim.x);
+    const int row = rows ? rows[blockIdx.x] : xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
     for (int i = tid; i < kSymSlots; i += kThreads) {
         s_key[i] = kEmpty;
@@ -1026,8 +1211,8 @@ __global__ void collect_rows_kernel(int n, int mode, const int *__restrict__ lev
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    if (mode == 0) {
-        if (level[i] == kLevelSlab)
+    if (mode == 0 || mode == 2) {
+        if (level[i] == (mode == 0 ? kLevelSlab : kLevelLarge))
             out_slab[atomicAdd(n_slab, 1)] = i;
         return;
     }
@@ -1122,7 +1307,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     const int num_cus = ctx().num_cus > 0 ? ctx().num_cus : 256;
     // Counters read back by the host: [0] pairs, [1] symbolic slab rows,
     // [2] numeric slab rows, [3] numeric large-LDS rows, [4] C tiles,
-    // [5] nnz(C).
+    // [5] nnz(C), [6] rows for the workgroup symbolic table.
     RTT_TRY(w->counters.ensure(8 * sizeof(unsigned long long)));
     RTT_TRY(w->host_counters.ensure(8 * sizeof(unsigned long long)));
     unsigned long long *ctr = w->counters.as<unsigned long long>();
@@ -1130,6 +1315,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     int *slab_count = reinterpret_cast<int *>(ctr + 1);
     int *num_slab_count = reinterpret_cast<int *>(ctr + 2);
     int *num_lds_count = reinterpret_cast<int *>(ctr + 3);
+    int *large_count = reinterpret_cast<int *>(ctr + 6);
 
     auto t_sym = Clock::now();
     // ---- symbolic: per-row tile sets, staged ------------------------------
@@ -1156,24 +1342,40 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
         hipLaunchKernelGGL(pair_count_kernel,
                            dim3(static_cast<unsigned>(std::min<int64_t>(4 * num_cus, blocks_for(a.num_tiles, kThreads)))),
                            dim3(kThreads), 0, s, a.num_tiles, g.a_brange, ctr);
+    // One wave per row first; rows that outgrow its table go to the
+    // workgroup table, and rows that outgrow that to the slab.
     RTT_HIP(hipEventRecord(ev->ev[0], s));
     if (R > 0)
-        hipLaunchKernelGGL(symbolic_count_kernel, dim3(R), dim3(kThreads), 0, s, g, stage_limit,
-                           st, w->row_tiles.as<int64_t>(), w->row_nnz.as<int64_t>(),
+        hipLaunchKernelGGL(symbolic_wave_kernel, dim3(R), dim3(kWave), 0, s, g, st,
+                           w->row_tiles.as<int64_t>(), w->row_nnz.as<int64_t>(),
                            w->row_level.as<int>());
     RTT_HIP(hipGetLastError());
-    RTT_HIP(hipEventRecord(ev->ev[1], s));
     int n_slab = 0;
     COut none{};
     if (R > 0) {
         hipLaunchKernelGGL(collect_rows_kernel, dim3(blocks_for(R, kThreads)), dim3(kThreads), 0,
-                           s, R, 0, w->row_level.as<int>(), nullptr, nullptr, nullptr, nullptr,
-                           w->l3_rows.as<int>(), slab_count, nullptr, nullptr);
-        RTT_HIP(hipMemcpyAsync(hctr, ctr, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           s, R, 2, w->row_level.as<int>(), nullptr, nullptr, nullptr, nullptr,
+                           w->l3_rows.as<int>(), large_count, nullptr, nullptr);
+        RTT_HIP(hipMemcpyAsync(hctr, ctr, 7 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                s));
         RTT_HIP(hipStreamSynchronize(s));
-        n_slab = static_cast<int>(hctr[1] & 0xffffffffu);
+        const int n_large = static_cast<int>(hctr[6] & 0xffffffffu);
+        if (n_large > 0) {
+            hipLaunchKernelGGL(symbolic_count_kernel, dim3(n_large), dim3(kThreads), 0, s, g,
+                               stage_limit, st, w->l3_rows.as<int>(),
+                               w->row_tiles.as<int64_t>(), w->row_nnz.as<int64_t>(),
+                               w->row_level.as<int>());
+            RTT_HIP(hipGetLastError());
+            hipLaunchKernelGGL(collect_rows_kernel, dim3(blocks_for(R, kThreads)), dim3(kThreads),
+                               0, s, R, 0, w->row_level.as<int>(), nullptr, nullptr, nullptr,
+                               nullptr, w->l3_rows.as<int>(), slab_count, nullptr, nullptr);
+            RTT_HIP(hipMemcpyAsync(hctr, ctr, 2 * sizeof(unsigned long long),
+                                   hipMemcpyDeviceToHost, s));
+            RTT_HIP(hipStreamSynchronize(s));
+            n_slab = static_cast<int>(hctr[1] & 0xffffffffu);
+        }
     }
+    RTT_HIP(hipEventRecord(ev->ev[1], s));
     int slab_blocks = 0;
     if (n_slab > 0) {
         slab_blocks = std::min(n_slab, 2 * num_cus);
