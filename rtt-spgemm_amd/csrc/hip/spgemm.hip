@@ -591,7 +591,7 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
 // packed in increasing order, and each tile row ORs them into the zeroed
 // buffer with at most two 64-bit LDS ORs.
 __global__ __launch_bounds__(kThreads) void symbolic_write_kernel(
-    int rows, Stage st, const int *__restrict__ row_level, COut c)
+    const int *__restrict__ rows, int count, Stage st, const int *__restrict__ row_level, COut c)
 {
     __shared__ unsigned long long s_buf[kWavesPerBlock][kWriteBuf / 8];
     __shared__ unsigned long long s_cols[256];
@@ -608,8 +608,11 @@ __global__ __launch_bounds__(kThreads) void symbolic_write_kernel(
         s_cols[x] = v;
     }
     __syncthreads();
-    const int row = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + w;
-    if (row >= rows || row_level[row] != kLevelLds)
+    const int idx = blockIdx.x * kWavesPerBlock + w;
+    if (idx >= count)
+        return;
+    const int row = rows[idx];
+    if (row_level[row] != kLevelLds)
         return;
     const int64_t tb = c.tile_ptr[row];
     const int nt = static_cast<int>(c.tile_ptr[row + 1] - tb);
@@ -854,7 +857,7 @@ __device__ __forceinline__ uint32_t a_column_runs(uint32_t ab)
 template <class Cfg>
 __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     Operands g, const BRecord *__restrict__ recs, const int *__restrict__ rows,
-    const int *__restrict__ row_level, COut c)
+    const int *__restrict__ row_level, Stage st, COut c)
 {
     using QT = typename Cfg::QType;
     __shared__ double s_acc[Cfg::kAcc];
@@ -873,12 +876,11 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
 
     const int tid = threadIdx.x;
     const int row = rows ? rows[blockIdx.x] : xcd_block(blockIdx.x, gridDim.x);
-    if (row_level[row] != kLevelLds)
-        return;
-    // The row's inputs come in one round of independent loads: C's row
-    // extents, A's tile and payload extents (the payload base per tile row
-    // is precomputed), then the first C tile, A value and A tile of each
-    // thread, before the first barrier.
+    // The row's inputs come in two rounds of independent loads: its level,
+    // C's row extents, A's tile and payload extents (the payload base per
+    // tile row is precomputed); then the first staged C tile, A value and A
+    // tile of each thread, before the first barrier.
+    const int level = row_level[row];
     const int64_t t0 = c.tile_ptr[row];
     const int nt = static_cast<int>(c.tile_ptr[row + 1] - t0);
     const int64_t zb = c.row_nnz_off[row];
@@ -886,14 +888,15 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
     const int a_base = g.a_row_off[row];
     const int a_len = g.a_row_off[row + 1] - a_base;
-    if (nt > Cfg::kTiles || nnz_row > Cfg::kAcc || a_len > Cfg::kAStage)
+    if ((level != kLevelLds) | (nt > Cfg::kTiles) | (nnz_row > Cfg::kAcc) |
+        (a_len > Cfg::kAStage))
         return;
-    int j0 = 0, base0 = 0;
+    const int64_t sb = static_cast<int64_t>(row) * st.stride;
+    int j0 = 0;
     uint64_t m0 = 0;
     if (tid < nt) {
-        j0 = c.col[t0 + tid];
-        m0 = c.mask[t0 + tid];
-        base0 = static_cast<int>(c.tile_nnz[t0 + tid] - zb);
+        j0 = st.j[sb + tid];
+        m0 = st.m[sb + tid];
     }
     double av0 = 0.0;
     if (tid < a_len)
@@ -915,24 +918,38 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
         s_acol[i] = g.a_vals_cm[a_base + i];
     if (tid < kColRegs)
         s_acol[a_len + tid] = 0.0;   // the register loads of a column may run past it
-    __syncthreads();
-    for (int q = tid; q < nt; q += kThreads) {
-        int j = j0, base = base0;
+    // C's structure for the row from the staged (column, mask) list, in
+    // column order: value offsets by a scan of the tile popcounts.  The
+    // row's tiles also go into LDS and the hash table.  (The scan's barrier
+    // orders the table clear above before the inserts.)
+    int carry = 0;
+    for (int q0 = 0; q0 < nt; q0 += kThreads) {
+        const int q = q0 + tid;
+        int j = j0;
         uint64_t m = m0;
-        if (q != tid) {
-            j = c.col[t0 + q];
-            m = c.mask[t0 + q];
-            base = static_cast<int>(c.tile_nnz[t0 + q] - zb);
+        if (q0 != 0 && q < nt) {
+            j = st.j[sb + q];
+            m = st.m[sb + q];
         }
-        CTileInfo ct;
-        ct.runs = row_runs(m);
-        ct.mask = m;
-        s_ct[q] = ct;
-        s_cbase[q] = base;
-        uint32_t h = slot_of(j, Cfg::kSlotBits);
-        while (atomicCAS(&s_hk[h], kEmpty, j) != kEmpty)
-            h = (h + 1) & (Cfg::kSlots - 1);
-        s_hq[h] = static_cast<QT>(q);
+        int total = 0;
+        const int base = carry + block_exclusive_scan(q < nt ? __popcll(m) : 0, s_tmp, &total);
+        carry += total;
+        if (q < nt) {
+            const int64_t t = t0 + q;
+            c.col[t] = j;
+            c.mask[t] = m;
+            c.tile_nnz[t] = zb + base;
+            reinterpret_cast<uint64_t *>(c.csr_ptr)[t] = packed_row_offsets(m);
+            CTileInfo ct;
+            ct.runs = row_runs(m);
+            ct.mask = m;
+            s_ct[q] = ct;
+            s_cbase[q] = base;
+            uint32_t h = slot_of(j, Cfg::kSlotBits);
+            while (atomicCAS(&s_hk[h], kEmpty, j) != kEmpty)
+                h = (h + 1) & (Cfg::kSlots - 1);
+            s_hq[h] = static_cast<QT>(q);
+        }
     }
     // Chunk tile info: {rows present in column k (byte k), heavy (bit 0) and
     // first staged value of column k (byte k, k > 0), first staged value,
@@ -1098,8 +1115,23 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
         }
     }
     __syncthreads();
-    for (int64_t i = tid; i < nnz_row; i += kThreads)
+    // Column bytes of the row's entries, staged in the A buffer (free now),
+    // then values and column bytes stream out together.
+    unsigned char *s_colb = reinterpret_cast<unsigned char *>(s_acol);
+    static_assert(sizeof(s_acol) >= Cfg::kAcc, "column byte staging");
+    for (int q = tid; q < nt; q += kThreads) {
+        uint64_t m = s_ct[q].mask;
+        int o = s_cbase[q];
+        while (m) {
+            s_colb[o++] = static_cast<unsigned char>(__builtin_ctzll(m) & 7);
+            m &= m - 1;
+        }
+    }
+    __syncthreads();
+    for (int64_t i = tid; i < nnz_row; i += kThreads) {
         c.val[zb + i] = s_acc[i];
+        c.csr_col[zb + i] = s_colb[i];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1438,9 +1470,13 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     co.csr_col = c->tile_csr_col.as<uint8_t>();
     co.val = c->tile_csr_val.as<double>();
     RTT_HIP(hipEventRecord(ev->ev[2], s));
-    if (R > 0 && c->num_tiles > 0)
-        hipLaunchKernelGGL(symbolic_write_kernel, dim3(blocks_for(R, kWavesPerBlock)),
-                           dim3(kThreads), 0, s, R, st, w->row_level.as<int>(), co);
+    // The numeric LDS kernels write the structure of their own rows from the
+    // staged lists; this pass writes it for the staged rows they cannot hold
+    // (the numeric fallback list, whose slab rows it skips).
+    if (n_num_slab > 0 && c->num_tiles > 0)
+        hipLaunchKernelGGL(symbolic_write_kernel, dim3(blocks_for(n_num_slab, kWavesPerBlock)),
+                           dim3(kThreads), 0, s, w->num_rows.as<int>(), n_num_slab, st,
+                           w->row_level.as<int>(), co);
     RTT_HIP(hipGetLastError());
     RTT_HIP(hipEventRecord(ev->ev[3], s));
     if (n_slab > 0)
@@ -1456,13 +1492,13 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     RTT_HIP(hipEventRecord(ev->ev[4], s));
     if (R > 0 && c->nnz > 0)
         hipLaunchKernelGGL((numeric_row_kernel<SmallRow>), dim3(R), dim3(kThreads), 0, s, g,
-                           b.records.as<BRecord>(), nullptr, w->row_level.as<int>(), co);
+                           b.records.as<BRecord>(), nullptr, w->row_level.as<int>(), st, co);
     RTT_HIP(hipGetLastError());
     RTT_HIP(hipEventRecord(ev->ev[5], s));
     if (n_num_lds > 0)
         hipLaunchKernelGGL((numeric_row_kernel<LargeRow>), dim3(n_num_lds), dim3(kThreads), 0,
                            s, g, b.records.as<BRecord>(), w->lds_rows.as<int>(),
-                           w->row_level.as<int>(), co);
+                           w->row_level.as<int>(), st, co);
     if (n_num_slab > 0) {
         const int blocks = std::min(n_num_slab, 2 * num_cus);
         DevBuf &idx = w->slab;   // reused: the symbolic slab pass has been queued before
