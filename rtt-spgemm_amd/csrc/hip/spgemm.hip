@@ -168,6 +168,34 @@ __device__ __forceinline__ void write_tile_structure(const COut &c, int64_t t, i
     }
 }
 
+// Exclusive scans of two values over the workgroup with one barrier; s2
+// (2 * kWavesPerBlock ints) must not be reused without a barrier between.
+__device__ __forceinline__ void dual_exclusive_scan(int a, int b, int *s2, int *pa, int *pb,
+                                                    int *ta, int *tb)
+{
+    const int w = threadIdx.x / kWave;
+    const int ia = wave_inclusive_scan(a);
+    const int ib = wave_inclusive_scan(b);
+    if (lane_id() == kWave - 1) {
+        s2[w] = ia;
+        s2[kWavesPerBlock + w] = ib;
+    }
+    __syncthreads();
+    int ba = 0, bb = 0, sa = 0, sb = 0;
+#pragma unroll
+    for (int i = 0; i < kWavesPerBlock; ++i) {
+        const int x = s2[i], y = s2[kWavesPerBlock + i];
+        ba += i < w ? x : 0;
+        bb += i < w ? y : 0;
+        sa += x;
+        sb += y;
+    }
+    *pa = ba + ia - a;
+    *pb = bb + ib - b;
+    *ta = sa;
+    *tb = sb;
+}
+
 // Exclusive scan over the workgroup; *total receives the sum.
 __device__ __forceinline__ int block_exclusive_scan(int v, int *s_tmp, int *total)
 {
@@ -873,6 +901,7 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     __shared__ int s_pw[kThreads];
     __shared__ unsigned char s_seg[kSeg];
     __shared__ int s_tmp[kWavesPerBlock];
+    __shared__ int s_dual[2 * kWavesPerBlock];
 
     const int tid = threadIdx.x;
     const int row = rows ? rows[blockIdx.x] : xcd_block(blockIdx.x, gridDim.x);
@@ -918,39 +947,6 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
         s_acol[i] = g.a_vals_cm[a_base + i];
     if (tid < kColRegs)
         s_acol[a_len + tid] = 0.0;   // the register loads of a column may run past it
-    // C's structure for the row from the staged (column, mask) list, in
-    // column order: value offsets by a scan of the tile popcounts.  The
-    // row's tiles also go into LDS and the hash table.  (The scan's barrier
-    // orders the table clear above before the inserts.)
-    int carry = 0;
-    for (int q0 = 0; q0 < nt; q0 += kThreads) {
-        const int q = q0 + tid;
-        int j = j0;
-        uint64_t m = m0;
-        if (q0 != 0 && q < nt) {
-            j = st.j[sb + q];
-            m = st.m[sb + q];
-        }
-        int total = 0;
-        const int base = carry + block_exclusive_scan(q < nt ? __popcll(m) : 0, s_tmp, &total);
-        carry += total;
-        if (q < nt) {
-            const int64_t t = t0 + q;
-            c.col[t] = j;
-            c.mask[t] = m;
-            c.tile_nnz[t] = zb + base;
-            reinterpret_cast<uint64_t *>(c.csr_ptr)[t] = packed_row_offsets(m);
-            CTileInfo ct;
-            ct.runs = row_runs(m);
-            ct.mask = m;
-            s_ct[q] = ct;
-            s_cbase[q] = base;
-            uint32_t h = slot_of(j, Cfg::kSlotBits);
-            while (atomicCAS(&s_hk[h], kEmpty, j) != kEmpty)
-                h = (h + 1) & (Cfg::kSlots - 1);
-            s_hq[h] = static_cast<QT>(q);
-        }
-    }
     // Chunk tile info: {rows present in column k (byte k), heavy (bit 0) and
     // first staged value of column k (byte k, k > 0), first staged value,
     // row runs of the columns (a_column_runs)} and its B runs {first heavy
@@ -967,26 +963,72 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
                               static_cast<int>(a_column_runs(ab)));
         s_br[tid] = make_int4(br.x, br.y, bl.x, bl.y);
     };
-    if (tid < min(kNumChunk, a1 - a0))
-        stage_tile(am0, br0, bl0);
-    for (int c0 = a0; c0 < a1; c0 += kNumChunk) {
-        const int n = min(kNumChunk, a1 - c0);
-        if (c0 != a0 && tid < n)
-            stage_tile(g.a_meta[c0 + tid], g.a_brange[c0 + tid], g.a_blight[c0 + tid]);
-        __syncthreads();
-        // The chunk's pairs in four runs, one per (A class, B class): heavy
-        // and light A tiles times the heavy and light runs of their B rows,
-        // so that the lanes of a wave take pairs of like cost.  Thread v
-        // stands for run v / kNumChunk of tile v % kNumChunk.
-        const int run = tid / kNumChunk, vt = tid % kNumChunk;
-        int nb = 0, bstart = 0;
+    // The chunk's pairs come in four runs, one per (A class, B class): heavy
+    // and light A tiles times the heavy and light runs of their B rows, so
+    // that the lanes of a wave take pairs of like cost.  Thread v stands for
+    // run v / kNumChunk of tile v % kNumChunk.
+    const int run = tid / kNumChunk, vt = tid % kNumChunk;
+    auto chunk_runs = [&](int n, int &nb, int &bstart) {
+        nb = 0;
+        bstart = 0;
         if (vt < n && (s_at[vt].y & 1) == ((run & 1) == 0 ? 1 : 0)) {
             const int4 br = s_br[vt];
             nb = run < 2 ? br.y : br.w;
             bstart = run < 2 ? br.x : br.z;
         }
-        int total = 0;
-        const int pre = block_exclusive_scan(nb, s_tmp, &total);
+    };
+    // C's structure for the row from the staged (column, mask) list, in
+    // column order, with value offsets from a scan of the tile popcounts;
+    // the row's tiles also go into LDS and the hash table.
+    auto insert_tile = [&](int q, int j, uint64_t m, int base) {
+        const int64_t t = t0 + q;
+        c.col[t] = j;
+        c.mask[t] = m;
+        c.tile_nnz[t] = zb + base;
+        reinterpret_cast<uint64_t *>(c.csr_ptr)[t] = packed_row_offsets(m);
+        CTileInfo ct;
+        ct.runs = row_runs(m);
+        ct.mask = m;
+        s_ct[q] = ct;
+        s_cbase[q] = base;
+        uint32_t h = slot_of(j, Cfg::kSlotBits);
+        while (atomicCAS(&s_hk[h], kEmpty, j) != kEmpty)
+            h = (h + 1) & (Cfg::kSlots - 1);
+        s_hq[h] = static_cast<QT>(q);
+    };
+    if (tid < min(kNumChunk, a1 - a0))
+        stage_tile(am0, br0, bl0);
+    __syncthreads();   // chunk 0 staged, table cleared
+    // One scan gives the first kThreads tiles' value offsets and chunk 0's
+    // pair offsets.
+    int nb = 0, bstart = 0, pre = 0, total = 0, base = 0, carry = 0;
+    chunk_runs(min(kNumChunk, a1 - a0), nb, bstart);
+    dual_exclusive_scan(tid < nt ? __popcll(m0) : 0, nb, s_dual, &base, &pre, &carry, &total);
+    if (tid < nt)
+        insert_tile(tid, j0, m0, base);
+    for (int q0 = kThreads; q0 < nt; q0 += kThreads) {
+        const int q = q0 + tid;
+        int j = 0;
+        uint64_t m = 0;
+        if (q < nt) {
+            j = st.j[sb + q];
+            m = st.m[sb + q];
+        }
+        int tot = 0;
+        const int qb = carry + block_exclusive_scan(q < nt ? __popcll(m) : 0, s_tmp, &tot);
+        carry += tot;
+        if (q < nt)
+            insert_tile(q, j, m, qb);
+    }
+    for (int c0 = a0; c0 < a1; c0 += kNumChunk) {
+        const int n = min(kNumChunk, a1 - c0);
+        if (c0 != a0) {
+            if (tid < n)
+                stage_tile(g.a_meta[c0 + tid], g.a_brange[c0 + tid], g.a_blight[c0 + tid]);
+            __syncthreads();
+            chunk_runs(n, nb, bstart);
+            pre = block_exclusive_scan(nb, s_tmp, &total);
+        }
         s_pw[tid] = bstart - pre;
         {
             for (int r0 = 0; r0 < total; r0 += kSeg) {
@@ -1119,12 +1161,14 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     // then values and column bytes stream out together.
     unsigned char *s_colb = reinterpret_cast<unsigned char *>(s_acol);
     static_assert(sizeof(s_acol) >= Cfg::kAcc, "column byte staging");
-    for (int q = tid; q < nt; q += kThreads) {
-        uint64_t m = s_ct[q].mask;
-        int o = s_cbase[q];
-        while (m) {
-            s_colb[o++] = static_cast<unsigned char>(__builtin_ctzll(m) & 7);
-            m &= m - 1;
+    for (int it = tid; it < nt * 8; it += kThreads) {
+        const int q = it >> 3, r = it & 7;
+        const uint64_t m = s_ct[q].mask;
+        uint32_t x = static_cast<uint32_t>(m >> (8 * r)) & 0xFFu;
+        int o = s_cbase[q] + __popcll(m & ((1ull << (8 * r)) - 1ull));
+        while (x) {
+            s_colb[o++] = static_cast<unsigned char>(__builtin_ctz(x));
+            x &= x - 1;
         }
     }
     __syncthreads();
