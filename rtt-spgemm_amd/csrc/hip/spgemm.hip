@@ -1159,22 +1159,41 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     __syncthreads();
     // Column bytes of the row's entries, staged in the A buffer (free now),
     // then values and column bytes stream out together.
+    // They are staged at the output's byte alignment, zb mod 4, so that
+    // whole words go out as words.
+    const int shift = static_cast<int>(zb & 3);
     unsigned char *s_colb = reinterpret_cast<unsigned char *>(s_acol);
-    static_assert(sizeof(s_acol) >= Cfg::kAcc, "column byte staging");
+    static_assert(sizeof(s_acol) >= Cfg::kAcc + 3, "column byte staging");
     for (int it = tid; it < nt * 8; it += kThreads) {
         const int q = it >> 3, r = it & 7;
         const uint64_t m = s_ct[q].mask;
         uint32_t x = static_cast<uint32_t>(m >> (8 * r)) & 0xFFu;
-        int o = s_cbase[q] + __popcll(m & ((1ull << (8 * r)) - 1ull));
+        int o = shift + s_cbase[q] + __popcll(m & ((1ull << (8 * r)) - 1ull));
         while (x) {
             s_colb[o++] = static_cast<unsigned char>(__builtin_ctz(x));
             x &= x - 1;
         }
     }
     __syncthreads();
-    for (int64_t i = tid; i < nnz_row; i += kThreads) {
+    for (int64_t i = tid; i < nnz_row; i += kThreads)
         c.val[zb + i] = s_acc[i];
-        c.csr_col[zb + i] = s_colb[i];
+    // Words [0, nw) of the row's column bytes from the aligned word at zb -
+    // shift; the first and last may hold bytes of the neighbouring rows and
+    // are written byte by byte.
+    unsigned char *colw = c.csr_col + (zb - shift);
+    const int nw = static_cast<int>((shift + nnz_row + 3) >> 2);
+    for (int i = tid; i < nw; i += kThreads) {
+        const uint32_t v = reinterpret_cast<const uint32_t *>(s_colb)[i];
+        if (i > 0 && i < nw - 1) {
+            reinterpret_cast<uint32_t *>(colw)[i] = v;
+        } else {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int idx = 4 * i + b;
+                if (idx >= shift && idx < shift + nnz_row)
+                    colw[idx] = static_cast<unsigned char>(v >> (8 * b));
+            }
+        }
     }
 }
 
