@@ -51,8 +51,9 @@ constexpr int kWSlots = 256;       // hash slots of the one-wave symbolic table
 constexpr int kWSlotBits = 8;
 static_assert(kWSlots == 1 << kWSlotBits, "one-wave table size");
 constexpr int kWStage = 3 * kWSlots / 4;   // C tiles a row may have there
-constexpr int kWChunk = kWave / 4; // A tiles per one-wave enumeration step
+constexpr int kWChunk = kWave / 2; // A tiles per one-wave enumeration step
 constexpr int kWSeg = 512;         // pairs per one-wave segment window
+constexpr int kWBatch = 6;         // pairs whose loads a lane of that kernel issues together
 
 constexpr int kNumChunk = 64;      // A tiles per numeric chunk
 constexpr int kColRegs = 3;         // values of an A column a numeric lane holds in registers
@@ -244,8 +245,8 @@ __device__ __forceinline__ void fill_segments(unsigned char *s_seg, int pre, int
 // ---------------------------------------------------------------------------
 // Symbolic, one wave per row: the same hash -> compact -> rank -> stage as the
 // workgroup kernel below, with a kWSlots table, so that a CU holds four times
-// as many rows and their load latencies overlap.  Pairs come in the numeric
-// kernel's four runs (A class x B class) per chunk of kWChunk A tiles, one
+// as many rows and their load latencies overlap.  Pairs come in two runs per
+// chunk of kWChunk A tiles, the heavy and the light tiles of the B rows, one
 // lane per (run, tile), and a pair's mask sums only the k where both A's
 // column and B's row are nonempty.  A row that outgrows the table is marked
 // kLevelLarge for the workgroup kernel.
@@ -264,7 +265,7 @@ __global__ __launch_bounds__(kWave) void symbolic_wave_kernel(
     __shared__ unsigned long long s_msk[kWSlots];
     __shared__ unsigned long long s_spread[4][kWChunk];
     __shared__ int4 s_br[kWChunk];
-    __shared__ int s_acls[kWChunk];   // nonempty columns (bits 0-3), heavy (bit 4)
+    __shared__ int s_acls[kWChunk];   // nonempty columns (bits 0-3)
     __shared__ int s_pw[kWave];
     __shared__ unsigned char s_seg[kWSeg];
 
@@ -287,16 +288,15 @@ __global__ __launch_bounds__(kWave) void symbolic_wave_kernel(
             for (int k = 0; k < 4; ++k)
                 s_spread[k][lane] = rows_to_bytes(byte_of(ab, k));
             s_br[lane] = make_int4(br.x, br.y, bl.x, bl.y);
-            s_acls[lane] = static_cast<int>(nonzero_bytes(ab)) |
-                           (__popc(static_cast<uint32_t>(am.y)) >= kHeavyMin ? 16 : 0);
+            s_acls[lane] = static_cast<int>(nonzero_bytes(ab));
         }
         __syncthreads();
         const int run = lane / kWChunk, vt = lane % kWChunk;
         int nb = 0, bstart = 0;
-        if (vt < n && ((s_acls[vt] >> 4) & 1) == ((run & 1) == 0 ? 1 : 0)) {
+        if (vt < n) {
             const int4 br = s_br[vt];
-            nb = run < 2 ? br.y : br.w;
-            bstart = run < 2 ? br.x : br.z;
+            nb = run == 0 ? br.y : br.w;
+            bstart = run == 0 ? br.x : br.z;
         }
         const int incl = wave_inclusive_scan(nb);
         const int total = __shfl(incl, kWave - 1, kWave);
@@ -308,11 +308,11 @@ __global__ __launch_bounds__(kWave) void symbolic_wave_kernel(
                 fill_segments(s_seg, pre, nb, r0, r1);
             __syncthreads();
             bool lane_ovf = false;
-            for (int p0 = r0 + lane; p0 < r1; p0 += kSymBatch * kWave) {
-                int4 bm[kSymBatch];
-                int ai[kSymBatch];
+            for (int p0 = r0 + lane; p0 < r1; p0 += kWBatch * kWave) {
+                int4 bm[kWBatch];
+                int ai[kWBatch];
 #pragma unroll
-                for (int u = 0; u < kSymBatch; ++u) {
+                for (int u = 0; u < kWBatch; ++u) {
                     const int p = p0 + u * kWave;
                     ai[u] = -1;
                     bm[u] = make_int4(0, 0, 0, 0);
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(kWave) void symbolic_wave_kernel(
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < kSymBatch; ++u) {
+                for (int u = 0; u < kWBatch; ++u) {
                     if (ai[u] < 0)
                         continue;
                     const uint32_t bb = static_cast<uint32_t>(bm[u].z);
