@@ -94,7 +94,7 @@ def cpu_baseline(a, aat, nnz_cub, budget_s=12.0):
 # Kernel names in the rocprofv3 counter summaries, per breakdown key.
 PROFILE_KERNELS = {
     "numeric": "numeric_row_kernel<SmallRow",
-    "symbolic_count": "symbolic_count_kernel",
+    "symbolic_count": "symbolic_wave_kernel",
     "symbolic_fill": "symbolic_write_kernel",
 }
 
@@ -196,15 +196,17 @@ def main():
         "symbolic_fill": np.mean([s[2]["symbolic_fill_kernel_ms"] for s in steps]),
         "numeric": np.mean([s[2]["numeric_kernel_ms"] for s in steps]),
     }
-    # Algorithmic bytes of each kernel (per launch):
-    #  numeric        reads A and B once, writes the FP64 values of C
-    #  symbolic_fill  reads A and B structure, writes C structure
+    # Algorithmic bytes of each kernel (per launch), in CSR terms:
+    #  numeric        reads A and B once, writes C (values and structure;
+    #                 the LDS kernels write both for the rows they hold)
+    #  symbolic_fill  C structure of the rows the LDS kernels cannot hold
+    #                 (none on the structured workloads)
     #  symbolic_count reads A and B structure, writes per-row counts
     m_b = a.n if aat else a.m
     a_struct = 4 * (a.m + 1) + 4 * a.nnz
     b_struct = 4 * (m_b + 1) + 4 * a.nnz
     alg = {
-        "numeric": csr_bytes(a.m, a.nnz) + csr_bytes(m_b, a.nnz) + 8 * nnz_c,
+        "numeric": csr_bytes(a.m, a.nnz) + csr_bytes(m_b, a.nnz) + csr_bytes(a.m, nnz_c),
         "symbolic_fill": a_struct + b_struct + 4 * (a.m + 1) + 4 * nnz_c,
         "symbolic_count": a_struct + b_struct + 8 * (a.m + 1),
     }
