@@ -51,7 +51,7 @@ constexpr int kWSlots = 256;       // hash slots of the one-wave symbolic table
 constexpr int kWSlotBits = 8;
 static_assert(kWSlots == 1 << kWSlotBits, "one-wave table size");
 constexpr int kWStage = 3 * kWSlots / 4;   // C tiles a row may have there
-constexpr int kWChunk = kWave / 2; // A tiles per one-wave enumeration step
+constexpr int kWChunk = kWave;     // A tiles per one-wave enumeration step
 constexpr int kWSeg = 512;         // pairs per one-wave segment window
 constexpr int kWBatch = 6;         // pairs whose loads a lane of that kernel issues together
 
@@ -246,9 +246,9 @@ __device__ __forceinline__ void fill_segments(unsigned char *s_seg, int pre, int
 // Symbolic, one wave per row: the same hash -> compact -> rank -> stage as the
 // workgroup kernel below, with a kWSlots table, so that a CU holds four times
 // as many rows and their load latencies overlap.  Pairs come in two runs per
-// chunk of kWChunk A tiles, the heavy and the light tiles of the B rows, one
-// lane per (run, tile), and a pair's mask sums only the k where both A's
-// column and B's row are nonempty.  A row that outgrows the table is marked
+// chunk of kWChunk A tiles, the heavy and then the light tiles of the B
+// rows; lane t owns both runs of tile t.  A pair's mask sums only the k where
+// both A's column and B's row are nonempty.  A row that outgrows the table is marked
 // kLevelLarge for the workgroup kernel.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t nonzero_bytes(uint32_t v)
@@ -264,10 +264,10 @@ __global__ __launch_bounds__(kWave) void symbolic_wave_kernel(
     __shared__ __align__(16) int s_key[kWSlots];
     __shared__ unsigned long long s_msk[kWSlots];
     __shared__ unsigned long long s_spread[4][kWChunk];
-    __shared__ int4 s_br[kWChunk];
     __shared__ int s_acls[kWChunk];   // nonempty columns (bits 0-3)
-    __shared__ int s_pw[kWave];
+    __shared__ int s_pw[2 * kWChunk]; // per (run, tile): first B tile minus first pair
     __shared__ unsigned char s_seg[kWSeg];
+    static_assert(2 * kWChunk <= 256, "segment entries are bytes");
 
     const int row = xcd_block(blockIdx.x, gridDim.x);
     const int lane = threadIdx.x;
@@ -279,6 +279,7 @@ __global__ __launch_bounds__(kWave) void symbolic_wave_kernel(
     bool ovf = false;
     for (int c0 = a0; c0 < a1 && !ovf; c0 += kWChunk) {
         const int n = min(kWChunk, a1 - c0);
+        int nh = 0, bh = 0, nl = 0, bls = 0;
         if (lane < n) {
             const int4 am = g.a_meta[c0 + lane];
             const int4 br = g.a_brange[c0 + lane];
@@ -287,25 +288,25 @@ __global__ __launch_bounds__(kWave) void symbolic_wave_kernel(
 #pragma unroll
             for (int k = 0; k < 4; ++k)
                 s_spread[k][lane] = rows_to_bytes(byte_of(ab, k));
-            s_br[lane] = make_int4(br.x, br.y, bl.x, bl.y);
             s_acls[lane] = static_cast<int>(nonzero_bytes(ab));
+            nh = br.y;
+            bh = br.x;
+            nl = bl.y;
+            bls = bl.x;
         }
-        __syncthreads();
-        const int run = lane / kWChunk, vt = lane % kWChunk;
-        int nb = 0, bstart = 0;
-        if (vt < n) {
-            const int4 br = s_br[vt];
-            nb = run == 0 ? br.y : br.w;
-            bstart = run == 0 ? br.x : br.z;
-        }
-        const int incl = wave_inclusive_scan(nb);
-        const int total = __shfl(incl, kWave - 1, kWave);
-        const int pre = incl - nb;
-        s_pw[lane] = bstart - pre;
+        // Pair offsets: the heavy runs of all tiles, then the light runs.
+        const int ih = wave_inclusive_scan(nh), il = wave_inclusive_scan(nl);
+        const int heavy_total = __shfl(ih, kWave - 1, kWave);
+        const int total = heavy_total + __shfl(il, kWave - 1, kWave);
+        const int ph = ih - nh, pl = heavy_total + il - nl;
+        s_pw[lane] = bh - ph;
+        s_pw[kWChunk + lane] = bls - pl;
         for (int r0 = 0; r0 < total && !ovf; r0 += kWSeg) {
             const int r1 = min(total, r0 + kWSeg);
-            if (nb > 0)
-                fill_segments(s_seg, pre, nb, r0, r1);
+            for (int p = max(ph, r0); p < min(ph + nh, r1); ++p)
+                s_seg[p - r0] = static_cast<unsigned char>(lane);
+            for (int p = max(pl, r0); p < min(pl + nl, r1); ++p)
+                s_seg[p - r0] = static_cast<unsigned char>(kWChunk + lane);
             __syncthreads();
             bool lane_ovf = false;
             for (int p0 = r0 + lane; p0 < r1; p0 += kWBatch * kWave) {
