@@ -64,12 +64,11 @@ static_assert(kThreads == 4 * kNumChunk, "one numeric thread per (pair run, chun
 // values, C tiles, hash slots (at least twice the tiles).
 struct SmallRow {
     static constexpr int kAcc = 1024, kAStage = 256, kTiles = 128, kSlots = 256, kSlotBits = 8;
-    using QType = unsigned char;
 };
 struct LargeRow {
     static constexpr int kAcc = 4096, kAStage = 2048, kTiles = 512, kSlots = 1024, kSlotBits = 10;
-    using QType = unsigned short;
 };
+static_assert(LargeRow::kAcc <= 65536 && LargeRow::kTiles <= 32768, "packed tile slots");
 
 constexpr int kWriteBuf = 2048;    // column bytes staged per row in the write pass
 
@@ -829,10 +828,14 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
 // lane takes those bases for the rows of A column k once per k, and each
 // product costs one add.  Other tiles take the slot from a popcount of the
 // tile mask.
-struct alignas(16) CTileInfo {
-    unsigned long long runs;    // row_runs of the mask
-    unsigned long long mask;    // bit r * 8 + cl
-};
+// Hash slot of a C tile: {tile column or kEmpty, first value slot | local
+// tile index << 16, row_runs low word, row_runs high word}, so that one
+// LDS read gives a pair everything the walk needs of its C tile.
+__device__ __forceinline__ int4 make_tile_slot(int j, int base, int q, uint64_t runs)
+{
+    return make_int4(j, base | (q << 16), static_cast<int>(static_cast<uint32_t>(runs)),
+                     static_cast<int>(static_cast<uint32_t>(runs >> 32)));
+}
 
 // Byte value of a row whose columns are not one contiguous run.
 constexpr uint32_t kNoRun = 0x80u;
@@ -888,13 +891,11 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     Operands g, const BRecord *__restrict__ recs, const int *__restrict__ rows,
     const int *__restrict__ row_level, Stage st, COut c)
 {
-    using QT = typename Cfg::QType;
     __shared__ double s_acc[Cfg::kAcc];
     __shared__ double s_acol[Cfg::kAStage + kColRegs];
-    __shared__ CTileInfo s_ct[Cfg::kTiles];
+    __shared__ unsigned long long s_cmask[Cfg::kTiles];   // C tile masks, bit r * 8 + cl
     __shared__ int s_cbase[Cfg::kTiles];   // first slot of each tile within the row's segment
-    __shared__ int s_hk[Cfg::kSlots];
-    __shared__ QT s_hq[Cfg::kSlots];
+    __shared__ int4 s_slot[Cfg::kSlots];   // hash of the row's C tiles (make_tile_slot)
     // Per chunk A tile: its staged info and B runs (see stage_tile), and per
     // (run, tile) the first B record minus the run's first pair.
     __shared__ int4 s_at[kNumChunk];
@@ -939,7 +940,7 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
         bl0 = g.a_blight[a0 + tid];
     }
     for (int i = tid; i < Cfg::kSlots; i += kThreads)
-        s_hk[i] = kEmpty;
+        s_slot[i] = make_int4(kEmpty, 0, 0, 0);
     for (int i = tid; i < nnz_row; i += kThreads)
         s_acc[i] = 0.0;
     if (tid < a_len)
@@ -987,15 +988,12 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
         c.mask[t] = m;
         c.tile_nnz[t] = zb + base;
         reinterpret_cast<uint64_t *>(c.csr_ptr)[t] = packed_row_offsets(m);
-        CTileInfo ct;
-        ct.runs = row_runs(m);
-        ct.mask = m;
-        s_ct[q] = ct;
+        s_cmask[q] = m;
         s_cbase[q] = base;
         uint32_t h = slot_of(j, Cfg::kSlotBits);
-        while (atomicCAS(&s_hk[h], kEmpty, j) != kEmpty)
+        while (atomicCAS(&s_slot[h].x, kEmpty, j) != kEmpty)
             h = (h + 1) & (Cfg::kSlots - 1);
-        s_hq[h] = static_cast<QT>(q);
+        s_slot[h] = make_tile_slot(j, base, q, row_runs(m));
     };
     if (tid < min(kNumChunk, a1 - a0))
         stage_tile(am0, br0, bl0);
@@ -1049,21 +1047,20 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
                     // C tile of the pair; every tile of the row is in the table and
                     // at least half of its slots are empty, so the probe ends.
                     uint32_t h = slot_of(rec.meta.x, Cfg::kSlotBits);
-                    int q = -1;
+                    int4 sl;
                     for (;;) {
-                        const int k = s_hk[h];
-                        if (k == rec.meta.x) {
-                            q = s_hq[h];
-                            break;
-                        }
-                        if (k == kEmpty)
+                        sl = s_slot[h];
+                        if (sl.x == rec.meta.x || sl.x == kEmpty)
                             break;
                         h = (h + 1) & (Cfg::kSlots - 1);
                     }
-                    if (q < 0)
+                    if (sl.x != rec.meta.x)
                         continue;
-                    const unsigned long long cruns = s_ct[q].runs;
-                    double *acc = s_acc + s_cbase[q];
+                    const int q = sl.y >> 16;
+                    const unsigned long long cruns =
+                        (static_cast<unsigned long long>(static_cast<uint32_t>(sl.w)) << 32) |
+                        static_cast<uint32_t>(sl.z);
+                    double *acc = s_acc + (sl.y & 0xFFFF);
                     const uint32_t apf = static_cast<uint32_t>(at.y) & ~0xFFu;
                     const uint32_t aruns = static_cast<uint32_t>(at.w);
                     const double *acol = s_acol + at.z;
@@ -1138,7 +1135,7 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
                         });
                     } else {
                         // Slots from popcounts of the tile mask.
-                        const uint64_t cm = s_ct[q].mask;
+                        const uint64_t cm = s_cmask[q];
                         walk([&](int pos, double vb) {
                             const int k = pos >> 3, cl = pos & 7;
                             const double *ap = acol + byte_of(apf, k);
@@ -1167,7 +1164,7 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     static_assert(sizeof(s_acol) >= Cfg::kAcc + 3, "column byte staging");
     for (int it = tid; it < nt * 8; it += kThreads) {
         const int q = it >> 3, r = it & 7;
-        const uint64_t m = s_ct[q].mask;
+        const uint64_t m = s_cmask[q];
         uint32_t x = static_cast<uint32_t>(m >> (8 * r)) & 0xFFu;
         int o = shift + s_cbase[q] + __popcll(m & ((1ull << (8 * r)) - 1ull));
         while (x) {
