@@ -273,36 +273,31 @@ __global__ void row_ptr_from_sorted_kernel(const int *__restrict__ keys, int64_t
         ptr[k] = static_cast<int>(i);
 }
 
-// One record per B tile (meta + the first kRecVals values of the tile's
-// k-major payload: row k's entries contiguous, columns ascending), written
-// as kRecParts 16-byte parts by as many consecutive threads.
-__device__ __forceinline__ void write_record_part(BRecord *recs, int64_t t, int part,
-                                                  const int4 &m, const double *kmajor)
+// Part `part` of a tile's record (meta, then the first kRecVals values of
+// the tile's k-major payload: row k's entries contiguous, columns ascending).
+__device__ __forceinline__ int4 record_part(int part, const int4 &m, const double *kmajor)
 {
-    int4 *dst = reinterpret_cast<int4 *>(recs + t) + part;
-    if (part == 0) {
-        *dst = m;
-        return;
-    }
-    const uint32_t w = static_cast<uint32_t>(m.w);
-    const uint32_t off = w & 0x7FFFFFFFu;
+    if (part == 0)
+        return m;
+    const uint32_t off = static_cast<uint32_t>(m.w) & 0x7FFFFFFFu;
     const int nv = min(kRecVals, __popc(static_cast<uint32_t>(m.y)));
-    const double *payload = kmajor;
     const int j = 2 * (part - 1);
     double2 v;
-    v.x = j < nv ? payload[off + j] : 0.0;
-    v.y = j + 1 < nv ? payload[off + j + 1] : 0.0;
-    *reinterpret_cast<double2 *>(dst) = v;
+    v.x = j < nv ? kmajor[off + j] : 0.0;
+    v.y = j + 1 < nv ? kmajor[off + j + 1] : 0.0;
+    return *reinterpret_cast<const int4 *>(&v);
 }
 
+// B's record planes, one part per thread.
 __global__ void records_kernel(int64_t n, const int4 *__restrict__ meta,
-                               const double *__restrict__ kmajor, BRecord *__restrict__ recs)
+                               const double *__restrict__ kmajor, int4 *__restrict__ planes)
 {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t t = gid / kRecParts;
-    if (t >= n)
+    if (gid >= n * kRecParts)
         return;
-    write_record_part(recs, t, static_cast<int>(gid % kRecParts), meta[t], kmajor);
+    const int part = static_cast<int>(gid / n);
+    const int64_t t = gid - part * n;
+    planes[gid] = record_part(part, meta[t], kmajor);
 }
 
 // Offline, per A tile: the record a B tile built as A^T takes over, its meta
@@ -319,14 +314,17 @@ __global__ void transpose_source_kernel(int64_t n, const int *__restrict__ a_row
         return;
     int4 m = a_meta[t];
     m.x = a_row_of[t];
-    write_record_part(out, t, static_cast<int>(gid % kRecParts), m, a_vals_cm);
+    const int part = static_cast<int>(gid % kRecParts);
+    reinterpret_cast<int4 *>(out + t)[part] = record_part(part, m, a_vals_cm);
 }
 
 // B = A^T: B tile b is A tile ids[b], so its record is a copy of that A
-// tile's transpose record, one 16-byte part per thread.
+// tile's transpose record, one 16-byte part per thread (the kRecParts
+// threads of a tile read its record contiguously and write one entry of
+// each plane).
 __global__ void transposed_records_kernel(int64_t n, const int *__restrict__ ids,
                                           const BRecord *__restrict__ src,
-                                          int4 *__restrict__ b_meta, BRecord *__restrict__ recs)
+                                          int4 *__restrict__ b_meta, int4 *__restrict__ planes)
 {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const int64_t t = gid / kRecParts;
@@ -334,7 +332,7 @@ __global__ void transposed_records_kernel(int64_t n, const int *__restrict__ ids
         return;
     const int part = static_cast<int>(gid % kRecParts);
     const int4 v = reinterpret_cast<const int4 *>(src + ids[t])[part];
-    reinterpret_cast<int4 *>(recs + t)[part] = v;
+    planes[part * n + t] = v;
     if (part == 0)
         b_meta[t] = v;
 }
@@ -402,19 +400,19 @@ __global__ void group_row_ptr_kernel(int rows, const int *__restrict__ groups,
         trp[r] = groups[static_cast<int64_t>(kClassSlots) * r];
 }
 
-// Gathers meta and records of a B built from CSR into class order.
+// Gathers meta and record planes of a B built from CSR into class order.
 __global__ void gather_records_kernel(int64_t n, const int *__restrict__ perm,
                                       const int4 *__restrict__ meta_in,
-                                      const BRecord *__restrict__ rec_in,
-                                      int4 *__restrict__ meta_out, BRecord *__restrict__ rec_out)
+                                      const int4 *__restrict__ rec_in,
+                                      int4 *__restrict__ meta_out, int4 *__restrict__ rec_out)
 {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t t = gid / kRecParts;
-    if (t >= n)
+    if (gid >= n * kRecParts)
         return;
-    const int part = static_cast<int>(gid % kRecParts);
+    const int part = static_cast<int>(gid / n);
+    const int64_t t = gid - part * n;
     const int src = perm[t];
-    reinterpret_cast<int4 *>(rec_out + t)[part] = reinterpret_cast<const int4 *>(rec_in + src)[part];
+    rec_out[gid] = rec_in[part * n + src];
     if (part == 0)
         meta_out[t] = meta_in[src];
 }
@@ -643,7 +641,7 @@ int build_records(DevTiles *t)
     if (t->num_tiles > 0)
         hipLaunchKernelGGL(records_kernel, dim3(blocks_for(t->num_tiles * kRecParts, kThreads)),
                            dim3(kThreads), 0, ctx().stream, t->num_tiles, t->meta.as<int4>(),
-                           t->values_cm.as<double>(), t->records.as<BRecord>());
+                           t->values_cm.as<double>(), t->records.as<int4>());
     RTT_HIP(hipGetLastError());
     return RTT_OK;
 }
@@ -684,8 +682,8 @@ int order_by_class(DevTiles *b, BuildWork *w)
                            w->ids_out.as<int>(), n, bits_for(slots), &w->scan_tmp, s));
         hipLaunchKernelGGL(gather_records_kernel, dim3(blocks_for(n * kRecParts, kThreads)),
                            dim3(kThreads), 0, s, n, w->ids_out.as<int>(), b->meta.as<int4>(),
-                           b->records.as<BRecord>(), w->meta2.as<int4>(),
-                           w->records2.as<BRecord>());
+                           b->records.as<int4>(), w->meta2.as<int4>(),
+                           w->records2.as<int4>());
         RTT_HIP(hipMemcpyAsync(b->meta.ptr, w->meta2.ptr, n * sizeof(int4),
                                hipMemcpyDeviceToDevice, s));
         RTT_HIP(hipMemcpyAsync(b->records.ptr, w->records2.ptr, n * sizeof(BRecord),
@@ -926,7 +924,7 @@ int transpose_tiles(const DevTiles &a, int b_cols, DevTiles *b, BuildWork *w)
         hipLaunchKernelGGL(transposed_records_kernel, dim3(blocks_for(n * kRecParts, kThreads)),
                            dim3(kThreads), 0, s, n, b->sorted_ids.as<int>(),
                            a.transpose_src.as<BRecord>(), b->meta.as<int4>(),
-                           b->records.as<BRecord>());
+                           b->records.as<int4>());
     }
     hipLaunchKernelGGL(row_ptr_from_sorted_kernel, dim3(blocks_for(n + 1, kThreads)),
                        dim3(kThreads), 0, s, w->keys_out.as<int>(), n, static_cast<int>(slots),

@@ -24,8 +24,11 @@ constexpr int kHeavyMin = 4;
 constexpr int kClassSlots = 8;   // class slots per B row: heavy, light k = 0..3, unused
 
 // Per-B-tile record read by the numeric kernel: the packed meta and the
-// first kRecVals payload values, so that one (A tile, B tile) pair costs one
-// contiguous load of kRecParts 16-byte parts.
+// first kRecVals payload values, kRecParts 16-byte parts.  B keeps its
+// records as part planes (part i of tile t at planes[i * n + t], n tiles), so
+// that the lanes of a wave, which take consecutive tiles, read each part as
+// one contiguous stretch.  A's offline transpose source keeps whole records
+// (one tile's parts contiguous).
 constexpr int kRecVals = 8;
 struct alignas(16) BRecord {
     int4 meta;
@@ -99,7 +102,7 @@ struct DevTiles {
     DevBuf tile_col_ptr, tile_row_idx, csc_tile_ids;
     DevBuf tile_row_of;   // tile -> tile row (A: used by the B transpose)
     DevBuf meta;          // int4 per tile
-    DevBuf records;       // BRecord per tile (B only)
+    DevBuf records;       // B only: kRecParts planes of int4, num_tiles entries each
     DevBuf a_brange;      // B only: per A tile, {first B tile, heavy, count} of B row K
     DevBuf a_blight;      // B only: per A tile, {first, count} of the light tiles it can meet
     DevBuf row_groups;    // B only: start of each (row, class) run, kClassSlots per row

@@ -888,7 +888,7 @@ __device__ __forceinline__ uint32_t a_column_runs(uint32_t ab)
 
 template <class Cfg>
 __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
-    Operands g, const BRecord *__restrict__ recs, const int *__restrict__ rows,
+    Operands g, const int4 *__restrict__ recs, int64_t rec_n, const int *__restrict__ rows,
     const int *__restrict__ row_level, Stage st, COut c)
 {
     __shared__ double s_acc[Cfg::kAcc];
@@ -1040,7 +1040,17 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
                 for (int p = r0 + tid; p < r1; p += kThreads) {
                     const int v = s_seg[p - r0];
                     const int4 at = s_at[v % kNumChunk];
-                    const BRecord rec = recs[s_pw[v] + p];
+                    // The pair's B record, part by part from the planes.
+                    const int64_t rb_idx = s_pw[v] + p;
+                    BRecord rec;
+                    rec.meta = recs[rb_idx];
+#pragma unroll
+                    for (int i = 0; i < kRecVals / 2; ++i) {
+                        const double2 d =
+                            reinterpret_cast<const double2 *>(recs + (i + 1) * rec_n)[rb_idx];
+                        rec.v[2 * i] = d.x;
+                        rec.v[2 * i + 1] = d.y;
+                    }
                     const uint32_t ab = static_cast<uint32_t>(at.x);
                     if (!product_count(ab, static_cast<uint32_t>(rec.meta.z)))
                         continue;
@@ -1553,12 +1563,13 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     RTT_HIP(hipEventRecord(ev->ev[4], s));
     if (R > 0 && c->nnz > 0)
         hipLaunchKernelGGL((numeric_row_kernel<SmallRow>), dim3(R), dim3(kThreads), 0, s, g,
-                           b.records.as<BRecord>(), nullptr, w->row_level.as<int>(), st, co);
+                           b.records.as<int4>(), b.num_tiles, nullptr, w->row_level.as<int>(),
+                           st, co);
     RTT_HIP(hipGetLastError());
     RTT_HIP(hipEventRecord(ev->ev[5], s));
     if (n_num_lds > 0)
         hipLaunchKernelGGL((numeric_row_kernel<LargeRow>), dim3(n_num_lds), dim3(kThreads), 0,
-                           s, g, b.records.as<BRecord>(), w->lds_rows.as<int>(),
+                           s, g, b.records.as<int4>(), b.num_tiles, w->lds_rows.as<int>(),
                            w->row_level.as<int>(), st, co);
     if (n_num_slab > 0) {
         const int blocks = std::min(n_num_slab, 2 * num_cus);
