@@ -1206,17 +1206,25 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Numeric for rows beyond the LDS budgets: one workgroup per row,
-// direct-mapped (tile column -> local tile) slab, FP64 atomics in HBM.
+// Numeric for rows beyond the LDS budgets, one workgroup per row at a time.
+// A direct-mapped HBM table, one int4 per B tile column {row, first value of
+// the C tile within the row, mask low word, mask high word}, gives a pair its
+// C tile in one load.  A's values are staged per chunk of A tiles in LDS
+// when they fit, B's come from the record planes, and every product is an
+// FP64 atomic on C's values in HBM.
 // ---------------------------------------------------------------------------
+constexpr int kSlabStage = 4096;   // A values a slab chunk stages in LDS
+
 __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
-    Operands g, const int *__restrict__ rows, int nrows, int *__restrict__ slabs, COut c)
+    Operands g, const int4 *__restrict__ recs, int64_t rec_n, const int *__restrict__ rows,
+    int nrows, int4 *__restrict__ slabs, COut c)
 {
-    __shared__ uint32_t s_ab[kThreads];
-    __shared__ int s_b0[kThreads];
-    __shared__ int s_pre[kThreads];
+    __shared__ double s_acol[kSlabStage + 8];
+    __shared__ int4 s_at[kThreads];   // {rows of column k (byte k), first value of column k (byte k), first value, unused}
+    __shared__ int s_pw[kThreads];
+    __shared__ unsigned char s_seg[kSeg];
     __shared__ int s_tmp[kWavesPerBlock];
-    int *slab = slabs + static_cast<int64_t>(blockIdx.x) * g.b_tile_cols;
+    int4 *slab = slabs + static_cast<int64_t>(blockIdx.x) * g.b_tile_cols;
     const int tid = threadIdx.x;
     for (int ri = blockIdx.x; ri < nrows; ri += gridDim.x) {
         const int row = rows[ri];
@@ -1224,77 +1232,111 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
         const int nt = static_cast<int>(c.tile_ptr[row + 1] - t0);
         const int64_t zb = c.row_nnz_off[row];
         const int64_t nnz_row = c.row_nnz_off[row + 1] - zb;
-        // Columns are checked against the slab's extent, so that a damaged
+        // Columns are checked against the table's extent, so that a damaged
         // structure cannot send a store outside it.
         for (int q = tid; q < nt; q += kThreads) {
             const int j = c.col[t0 + q];
+            const uint64_t m = c.mask[t0 + q];
             if (j >= 0 && j < g.b_tile_cols)
-                slab[j] = q;
+                slab[j] = make_int4(row, static_cast<int>(c.tile_nnz[t0 + q] - zb),
+                                    static_cast<int>(static_cast<uint32_t>(m)),
+                                    static_cast<int>(static_cast<uint32_t>(m >> 32)));
         }
         for (int64_t i = tid; i < nnz_row; i += kThreads)
             c.val[zb + i] = 0.0;
         __threadfence();
         __syncthreads();
+        double *cval = c.val + zb;
         const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
         for (int c0 = a0; c0 < a1; c0 += kThreads) {
             const int n = min(kThreads, a1 - c0);
-            const int total = load_slab_chunk(g, c0, n, s_ab, s_b0, s_pre, s_tmp);
-            for (int p = tid; p < total; p += kThreads) {
-                const int ai = upper_index(s_pre, n, p);
-                const int4 bm = g.b_meta[s_b0[ai] + (p - s_pre[ai])];
-                const uint32_t ab = s_ab[ai], bb = static_cast<uint32_t>(bm.z);
-                if (!product_count(ab, bb))
-                    continue;
-                const int4 am4 = g.a_meta[c0 + ai];
-                const uint32_t am = static_cast<uint32_t>(am4.y);
-                const uint32_t aw = static_cast<uint32_t>(am4.w);
-                const bool adense = (aw & kDenseBit) != 0;
-                const uint32_t aoff = aw & ~kDenseBit;
-                const uint32_t bmask = static_cast<uint32_t>(bm.y);
-                const uint32_t bw = static_cast<uint32_t>(bm.w);
-                const bool bdense = (bw & kDenseBit) != 0;
-                const uint32_t boff = bw & ~kDenseBit;
-                const int q = __hip_atomic_load(&slab[bm.x], __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT);
-                if (q < 0 || q >= nt)
-                    continue;
-                const int64_t tq = t0 + q;
-                const uint64_t cm = c.mask[tq];
-                const int64_t cb = c.tile_nnz[tq];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    uint32_t rws = byte_of(ab, k);
-                    const uint32_t cols = byte_of(bb, k);
-                    if (!cols)
+            const int v0 = g.a_off[c0];
+            const int span = g.a_off[c0 + n] - v0;
+            const bool staged = span <= kSlabStage;
+            int nb = 0, bstart = 0;
+            if (tid < n) {
+                const int4 am = g.a_meta[c0 + tid];
+                const int4 br = g.a_brange[c0 + tid];
+                const uint32_t ab = static_cast<uint32_t>(am.z);
+                const uint32_t p1 = __popc(byte_of(ab, 0));
+                const uint32_t p2 = p1 + __popc(byte_of(ab, 1));
+                const uint32_t p3 = p2 + __popc(byte_of(ab, 2));
+                s_at[tid] = make_int4(static_cast<int>(ab),
+                                      static_cast<int>((p1 << 8) | (p2 << 16) | (p3 << 24)),
+                                      static_cast<int>(static_cast<uint32_t>(am.w) & ~kDenseBit) - v0,
+                                      0);
+                nb = br.z;
+                bstart = br.x;
+            }
+            if (staged)
+                for (int i = tid; i < span; i += kThreads)
+                    s_acol[i] = g.a_vals_cm[v0 + i];
+            const double *abase = staged ? s_acol : g.a_vals_cm + v0;
+            int total = 0;
+            const int pre = block_exclusive_scan(nb, s_tmp, &total);
+            s_pw[tid] = bstart - pre;
+            for (int r0 = 0; r0 < total; r0 += kSeg) {
+                const int r1 = min(total, r0 + kSeg);
+                if (nb > 0)
+                    fill_segments(s_seg, pre, nb, r0, r1);
+                __syncthreads();
+                for (int p = r0 + tid; p < r1; p += kThreads) {
+                    const int v = s_seg[p - r0];
+                    const int4 at = s_at[v];
+                    const int64_t b = s_pw[v] + p;
+                    const int4 meta = recs[b];
+                    const uint32_t ab = static_cast<uint32_t>(at.x);
+                    if (!product_count(ab, static_cast<uint32_t>(meta.z)))
                         continue;
-                    while (rws) {
-                        const int r = __builtin_ctz(rws);
-                        rws &= rws - 1;
-                        const int pa = r * 4 + k;
-                        const double va =
-                            g.a_vals[aoff + (adense ? pa : __popc(am & ((1u << pa) - 1u)))];
-                        uint32_t cc = cols;
-                        while (cc) {
-                            const int cl = __builtin_ctz(cc);
-                            cc &= cc - 1;
-                            const int pb = cl * 4 + k;
-                            const double vb =
-                                g.b_vals[boff + (bdense ? pb : __popc(bmask & ((1u << pb) - 1u)))];
-                            const int pos = r * 8 + cl;
-                            unsafeAtomicAdd(&c.val[cb + __popcll(cm & ((1ull << pos) - 1ull))],
-                                            va * vb);
+                    if (meta.x < 0 || meta.x >= g.b_tile_cols)
+                        continue;
+                    const int4 sl = slab[meta.x];
+                    if (sl.x != row)
+                        continue;
+                    double rv[kRecVals];
+#pragma unroll
+                    for (int i = 0; i < kRecVals / 2; ++i) {
+                        const double2 d = reinterpret_cast<const double2 *>(recs + (i + 1) * rec_n)[b];
+                        rv[2 * i] = d.x;
+                        rv[2 * i + 1] = d.y;
+                    }
+                    const uint64_t cm = (static_cast<uint64_t>(static_cast<uint32_t>(sl.w)) << 32) |
+                                        static_cast<uint32_t>(sl.z);
+                    double *acc = cval + sl.y;
+                    const double *acol = abase + at.z;
+                    const uint32_t apf = static_cast<uint32_t>(at.y);
+                    const uint32_t boff = static_cast<uint32_t>(meta.w) & ~kDenseBit;
+                    auto entry = [&](int pos, double vb) {
+                        const int k = pos >> 3, cl = pos & 7;
+                        const double *ap = acol + byte_of(apf, k);
+                        uint32_t rr = byte_of(ab, k);
+                        int i = 0;
+                        while (rr) {
+                            const int r = __builtin_ctz(rr);
+                            rr &= rr - 1;
+                            unsafeAtomicAdd(&acc[__popcll(cm & ((1ull << (r * 8 + cl)) - 1ull))],
+                                            ap[i++] * vb);
+                        }
+                    };
+                    uint32_t mz = static_cast<uint32_t>(meta.z);
+#pragma unroll
+                    for (int j = 0; j < kRecVals; ++j) {
+                        if (mz) {
+                            const int pos = __builtin_ctz(mz);
+                            mz &= mz - 1;
+                            entry(pos, rv[j]);
                         }
                     }
+                    int j = kRecVals;
+                    while (mz) {
+                        const int pos = __builtin_ctz(mz);
+                        mz &= mz - 1;
+                        entry(pos, g.b_vals_km[boff + j]);
+                        ++j;
+                    }
                 }
+                __syncthreads();
             }
-            __syncthreads();
-        }
-        __threadfence();
-        __syncthreads();
-        for (int q = tid; q < nt; q += kThreads) {
-            const int j = c.col[t0 + q];
-            if (j >= 0 && j < g.b_tile_cols)
-                slab[j] = -1;
         }
         __threadfence();
         __syncthreads();
@@ -1481,7 +1523,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     RTT_HIP(hipEventRecord(ev->ev[1], s));
     int slab_blocks = 0;
     if (n_slab > 0) {
-        slab_blocks = std::min(n_slab, 2 * num_cus);
+        slab_blocks = std::min(n_slab, 4 * num_cus);
         RTT_TRY(w->slab.ensure(static_cast<size_t>(slab_blocks) * BC * sizeof(uint64_t)));
         hipLaunchKernelGGL(fill_u64_kernel,
                            dim3(blocks_for(static_cast<int64_t>(slab_blocks) * BC, kThreads)),
@@ -1572,15 +1614,15 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
                            s, g, b.records.as<int4>(), b.num_tiles, w->lds_rows.as<int>(),
                            w->row_level.as<int>(), st, co);
     if (n_num_slab > 0) {
-        const int blocks = std::min(n_num_slab, 2 * num_cus);
-        DevBuf &idx = w->slab;   // reused: the symbolic slab pass has been queued before
-        RTT_TRY(idx.ensure(static_cast<size_t>(blocks) * BC * sizeof(int)));
-        hipLaunchKernelGGL(fill_i32_kernel,
-                           dim3(blocks_for(static_cast<int64_t>(blocks) * BC, kThreads)),
-                           dim3(kThreads), 0, s, idx.as<int>(), static_cast<int64_t>(blocks) * BC,
-                           -1);
+        const int blocks = std::min(n_num_slab, 4 * num_cus);
+        DevBuf &tab = w->slab;   // reused: the symbolic slab pass has been queued before
+        const int64_t ents = static_cast<int64_t>(blocks) * BC;
+        RTT_TRY(tab.ensure(static_cast<size_t>(ents) * sizeof(int4)));
+        hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(4 * ents, kThreads)), dim3(kThreads), 0,
+                           s, tab.as<int>(), 4 * ents, -1);
         hipLaunchKernelGGL(numeric_slab_kernel, dim3(blocks), dim3(kThreads), 0, s, g,
-                           w->num_rows.as<int>(), n_num_slab, idx.as<int>(), co);
+                           b.records.as<int4>(), b.num_tiles, w->num_rows.as<int>(), n_num_slab,
+                           tab.as<int4>(), co);
         RTT_HIP(hipGetLastError());
     }
     RTT_HIP(hipEventRecord(ev->ev[7], s));
