@@ -16,6 +16,7 @@ timed on this host).
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -125,6 +126,57 @@ def profile_traffic(workload, kernel):
     return None, None
 
 
+def rocsparse_control(workload):
+    """rocSPARSE's SpGEMM on the same workload (the reference's vendor
+    control, src/cusparse_core_benchmark.h:4-30): the buffer-size and nnz
+    stages, C's allocation and the compute stage per sample, scratch reused;
+    median of three after one warm-up.  A reported comparison, not the
+    measured value.  It runs in a child process (this script with
+    --control-only), so that a failure inside the vendor library is reported
+    here instead of ending the benchmark."""
+    aat = WORKLOADS[workload]["aat"]
+    try:
+        p = subprocess.run([sys.executable, os.path.abspath(__file__), "--workload", workload,
+                            "--control-only"], capture_output=True, text=True, timeout=300)
+    except subprocess.TimeoutExpired:
+        return {"library": "rocSPARSE", "error": "timed out after 300 s"}
+    lines = [l for l in p.stdout.splitlines() if l.startswith("{")]
+    if p.returncode != 0 or not lines:
+        return {"library": "rocSPARSE", "error": f"exit {p.returncode}: {p.stderr[-200:]}"}
+    r = json.loads(lines[-1])
+    if "error" in r:
+        return {"library": "rocSPARSE", "error": r["error"][:200]}
+    return {"library": "rocSPARSE rocsparse_spgemm (CSR, FP64, int32 indices)",
+            "ms": round(r["ms"], 4), "nnz_c": r["nnz_c"],
+            "boundary": "buffer-size and nnz stages + C allocation + compute stage, scratch reused"
+                        + ("; B = A^T built by csr2csc before timing" if aat else "")}
+
+
+def finish_control(control, nnz_cub, nnz_c):
+    """GFLOP/s of the control on the same flop count, and whether its nnz(C)
+    equals this library's."""
+    if control is None or "error" in control:
+        return control
+    out = dict(control)
+    out["value"] = round(2.0 * nnz_cub / (control["ms"] * 1e6), 3)
+    out["unit"] = "GFLOP/s"
+    out["nnz_c_matches"] = control["nnz_c"] == nnz_c
+    return out
+
+
+def control_only(workload):
+    """Child side of rocsparse_control: build the workload's A, run the
+    control, print one JSON line."""
+    import rtt_amd as rt
+    spec = WORKLOADS[workload]
+    a, _ = make_matrix(rt, spec)
+    try:
+        r = rt.control_rocsparse(a, spec["aat"], warmup=1, iterations=3)
+    except Exception as e:  # a missing library or a vendor error status
+        r = {"error": str(e)}
+    print(json.dumps(r), flush=True)
+
+
 def copy_calibration(nbytes):
     """One device-to-device copy of nbytes through the HIP runtime: the plain
     copy kernel whose FETCH_SIZE / WRITE_SIZE counters scale the traffic
@@ -152,11 +204,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="stencil27-128-aat", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-control", action="store_true",
+                    help="skip the rocSPARSE SpGEMM control on the same workload")
+    ap.add_argument("--control-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (traffic-counter scale)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.control_only:
+        control_only(args.workload)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -166,6 +224,8 @@ def main():
     import rtt_amd as rt
     spec = WORKLOADS[args.workload]
     aat = spec["aat"]
+    # The control's child process runs before this process touches the GPU.
+    control = None if args.no_control else rocsparse_control(args.workload)
     a, desc = make_matrix(rt, spec)
     rt.init(0)
     pa = rt.prepare_identity_a(a)
@@ -224,6 +284,7 @@ def main():
     print(json.dumps({"breakdown": breakdown, "nnz_cub": nnz_cub, "nnz_c": nnz_c,
                       "tiles_c": st["output_tiles"], "pairs": st["pair_count"]}),
           file=sys.stderr, flush=True)
+    control = finish_control(control, nnz_cub, nnz_c)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(a, aat, nnz_cub)
@@ -252,6 +313,7 @@ def main():
                      "algorithmic_bytes": int(alg[dom]),
                      "traffic_source": traffic_src},
         "cpu_baseline": cpu,
+        "control": control,
     }
     print(json.dumps(line), flush=True)
 

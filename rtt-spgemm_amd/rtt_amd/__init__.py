@@ -520,3 +520,30 @@ def abi_symbols(header: str = HEADER_PATH) -> list[str]:
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"\b(rtt_[a-z0-9_]+)\s*\(", text)
     return sorted(set(names))
+
+
+# ---- vendor-library control ------------------------------------------------
+
+CONTROL_PATH = os.path.join(os.path.dirname(_HERE), "lib", "librtt_control.so")
+
+
+def control_rocsparse(a: CsrMatrix, aat: int, warmup: int = 1, iterations: int = 3) -> dict:
+    """C = A*A (aat 0) or A*A^T (aat 1) with rocsparse_spgemm, timed as the
+    reference's cuSPARSE control (src/cusparse_core_benchmark.h:4-30).  A
+    separate library (lib/librtt_control.so); raises when it is missing."""
+    if not os.path.exists(CONTROL_PATH):
+        raise RuntimeError(f"{CONTROL_PATH} is missing: build it with __graft_entry__.build()")
+    lib = C.CDLL(CONTROL_PATH)
+    lib.rtt_control_rocsparse.restype = C.c_int
+    lib.rtt_control_rocsparse.argtypes = [C.POINTER(SMatrix), C.c_int, C.c_int, C.c_int,
+                                          C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+                                          C.POINTER(C.c_double)]
+    lib.rtt_control_last_error.restype = C.c_char_p
+    s = a.as_smatrix()
+    ms = C.c_double()
+    nnz = C.c_longlong()
+    chk = C.c_double()
+    if lib.rtt_control_rocsparse(C.byref(s), aat, warmup, iterations, C.byref(ms), C.byref(nnz),
+                                 C.byref(chk)) != 0:
+        raise RuntimeError("rocSPARSE control: " + lib.rtt_control_last_error().decode())
+    return {"ms": ms.value, "nnz_c": int(nnz.value), "checksum": chk.value}

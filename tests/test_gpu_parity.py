@@ -303,6 +303,20 @@ def test_design_doc_tile_example_through_external_order(gpu):
     assert list(t["values"]) == [a_, c_, b_, d_]
 
 
+@pytest.mark.parametrize("aat", [0, 1])
+def test_rocsparse_control_agrees_with_oracle(gpu, aat):
+    """The vendor control computes the same product: rocSPARSE's nnz(C)
+    equals the oracle's and the sum of its values matches the oracle's
+    double-double sum (positive inputs, so no cancellation) within 1e-12."""
+    a = gpu.gen_stencil27(12, 0.5, 1.5, 3)
+    ref = oracle_product(a, aat)
+    r = gpu.control_rocsparse(a, aat, warmup=1, iterations=2)
+    assert r["nnz_c"] == ref["nnz"]
+    want = float(np.sum(ref["hi"]) + np.sum(ref["lo"]))
+    assert abs(r["checksum"] - want) <= 1e-12 * abs(want)
+    assert r["ms"] > 0
+
+
 def test_repeat_calls_reuse_workspace(gpu):
     a = gpu.gen_stencil27(10, 0.5, 1.5, 2)
     pa = gpu.prepare_identity_a(a)

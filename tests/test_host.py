@@ -24,6 +24,16 @@ def test_library_exports_every_header_symbol():
     assert set(names) <= exported
 
 
+def test_control_library_exports():
+    """The rocSPARSE control library is built beside the SpGEMM library and
+    exports its two entry points (loading it needs no GPU)."""
+    assert os.path.exists(rtt_amd.CONTROL_PATH)
+    out = subprocess.run(["nm", "-D", "--defined-only", rtt_amd.CONTROL_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    assert {"rtt_control_rocsparse", "rtt_control_last_error"} <= exported
+
+
 def test_smatrix_layout_matches_header(tmp_path):
     src = tmp_path / "layout.c"
     src.write_text(
