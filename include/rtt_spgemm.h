@@ -179,7 +179,8 @@ typedef struct rtt_b_update_stats {             /* DmmaBUpdateStats */
     double csr_row_permute_ms;
     double csr2tile_ms;
     double csc_ms;
-    int path;            /* 0 ordered-CSR merge, 1 tile transpose, 2 CSR transpose */
+    int path;            /* 0 ordered-CSR merge, 1 tile transpose, 2 CSR transpose,
+                            3 independent B CSR (general A*B), 4 values-only update */
 } rtt_b_update_stats;
 
 /* gpu_rebuild_dynamic_b_via_permuted_csr (aat = 0, src/gpu_dmma_tiles.h:2275)
@@ -195,6 +196,91 @@ void rtt_destroy_dynamic_b(rtt_dynamic_b *b);
  * scalar products, AA: sum_e rowlen(col e); AAT: sum_e coldeg(col e). */
 int rtt_compute_nnz_cub(const rtt_prepared_a *a, int aat,
                         unsigned long long *nnz_cub, double *elapsed_ms);
+
+/* ------------------------------------------------------------------------ */
+/* General A * B (--b B.mtx, src/main.cu:1554-1566,1960-1972) and the        */
+/* values-only B update (docs/BGRF_V1_PPOPP_ALGORITHM_AND_DATA_FORMAT_CN.md  */
+/* section 9.10; policy src/dmma_b_values_clear_policy.h)                    */
+/* ------------------------------------------------------------------------ */
+typedef struct rtt_device_csr rtt_device_csr;   /* DmmaOwnedDeviceCsr
+                                                   (src/gpu_dmma_tiles.h:32) */
+
+/* gpu_upload_csr (src/gpu_dmma_tiles.h:2259): H2D copy plus structural
+ * validation of an independent B.  Rows with unsorted columns are sorted
+ * (stably) on the device; the library keeps where each input entry went, so
+ * rtt_device_csr_set_values takes values in the caller's entry order. */
+int rtt_upload_csr(const rtt_smatrix *host, rtt_device_csr **out, double *h2d_ms,
+                   double *validation_ms);
+/* New values for an uploaded CSR whose structure is unchanged: nnz doubles
+ * in the entry order of the host CSR passed to rtt_upload_csr. */
+int rtt_device_csr_set_values(rtt_device_csr *csr, const double *host_values);
+void rtt_destroy_device_csr(rtt_device_csr *csr);
+
+/* gpu_rebuild_dynamic_b_via_permuted_csr with an independent B CSR
+ * (src/gpu_dmma_tiles.h:2275, called at src/main.cu:1417-1420): B's rows
+ * are taken through A's inner permutation, its columns are kept.  Needs
+ * b->rows == A's column count.  The build also records, per B source entry,
+ * where its value lands in the tile payload (the source-to-payload map). */
+int rtt_rebuild_b_general(const rtt_prepared_a *a, const rtt_device_csr *b,
+                          rtt_dynamic_b **inout, rtt_b_update_stats *stats);
+
+/* gpu_compute_nnz_cub_ab (src/gpu_dmma_tiles.h:2295): sum over A's entries
+ * of the length of the B row its column selects. */
+int rtt_compute_nnz_cub_ab(const rtt_prepared_a *a, const rtt_device_csr *b,
+                           unsigned long long *nnz_cub, double *elapsed_ms);
+
+enum rtt_b_values_clear_policy {        /* DmmaBValuesClearPolicy */
+    RTT_B_VALUES_ALWAYS_CLEAR = 0,
+    RTT_B_VALUES_NOCLEAR = 1,
+    RTT_B_VALUES_SAFE_AUTO = 2
+};
+
+enum rtt_b_values_clear_reason {        /* DmmaBValuesClearReason */
+    RTT_B_VALUES_CLEAR_POLICY_ALWAYS = 0,
+    RTT_B_VALUES_NOCLEAR_ELIGIBLE = 1,
+    RTT_B_VALUES_AUTO_ELIGIBLE = 2,
+    RTT_B_VALUES_FALLBACK_STRUCTURE_UPDATE = 3,
+    RTT_B_VALUES_FALLBACK_INVALID_STATE = 4,
+    RTT_B_VALUES_FALLBACK_DUPLICATES = 5,
+    RTT_B_VALUES_FALLBACK_NONUNIQUE_MAPPING = 6,
+    RTT_B_VALUES_FALLBACK_UNINITIALIZED_PAYLOAD = 7,
+    RTT_B_VALUES_FALLBACK_NUMERIC_WRITE_CONTRACT = 8,
+    RTT_B_VALUES_FALLBACK_INVALID_POLICY = 9,
+    RTT_B_VALUES_FALLBACK_INCOMPLETE_MAPPING = 10,
+    RTT_B_VALUES_FALLBACK_DENSE_HOLES = 11
+};
+
+typedef struct rtt_b_values_decision {  /* DmmaBValuesClearDecision */
+    int clear_payload;
+    int skip_clear;
+    int fallback;
+    int reason;                         /* enum rtt_b_values_clear_reason */
+} rtt_b_values_decision;
+
+/* dmma_choose_b_values_clear (src/dmma_b_values_clear_policy.h:91-160):
+ * host-only gate deciding whether a values-only update may skip clearing
+ * the payload.  Returns RTT_ERR_INVALID only for a NULL out. */
+int rtt_choose_b_values_clear(int policy, int values_only_update, int dynamic_b_valid,
+                              int has_active_duplicates, int active_source_mapping_complete,
+                              int active_source_to_payload_injective,
+                              int payload_fully_initialized, int every_payload_slot_overwritten,
+                              int numeric_b_read_only, rtt_b_values_decision *out);
+/* dmma_b_values_clear_policy_name / _reason_name (:43-88). */
+const char *rtt_b_values_clear_policy_name(int policy);
+const char *rtt_b_values_clear_reason_name(int reason);
+
+/* Values-only B update (section 9.10 of the design document above): the
+ * structure of *b was built by rtt_rebuild_b_general from this same
+ * rtt_device_csr, and only its values changed (rtt_device_csr_set_values).
+ * Clears the payload when the policy gate says so, then writes every mapped
+ * slot with the in-order sum of its source entries (duplicates included), so
+ * the payload equals what a rebuild computes, bit for bit.  When *b was not
+ * built from this CSR's structure, falls back to rtt_rebuild_b_general
+ * (decision->reason = RTT_B_VALUES_FALLBACK_INVALID_STATE, stats->path 3).
+ * decision may be NULL. */
+int rtt_update_b_values(const rtt_prepared_a *a, const rtt_device_csr *b_src, int policy,
+                        rtt_dynamic_b *b, rtt_b_update_stats *stats,
+                        rtt_b_values_decision *decision);
 
 /* ------------------------------------------------------------------------ */
 /* SpGEMM core: symbolic -> allocate -> numeric                              */

@@ -77,7 +77,25 @@ def ref_lib() -> C.CDLL | None:
         _ref.ref_mmio_info.argtypes = [C.c_char_p, C.POINTER(i32), C.POINTER(i32),
                                        C.POINTER(i32), C.POINTER(i32)]
         _ref.ref_mmio_data.argtypes = [C.c_char_p, vp, vp, vp]
+        _ref.ref_choose_b_values_clear.argtypes = [i32] * 9 + [vp]
+        _ref.ref_choose_b_values_clear.restype = None
+        _ref.ref_b_values_clear_policy_name.argtypes = [i32]
+        _ref.ref_b_values_clear_policy_name.restype = C.c_char_p
+        _ref.ref_b_values_clear_reason_name.argtypes = [i32]
+        _ref.ref_b_values_clear_reason_name.restype = C.c_char_p
     return _ref
+
+
+def ref_choose_b_values_clear(*args) -> tuple | None:
+    """The reference's dmma_choose_b_values_clear
+    (src/dmma_b_values_clear_policy.h:91-160) compiled from its header:
+    (clear_payload, skip_clear, fallback, reason), or None without _ref/."""
+    r = ref_lib()
+    if r is None:
+        return None
+    out = np.zeros(4, np.int32)
+    r.ref_choose_b_values_clear(*[int(x) for x in args], _p(out))
+    return tuple(int(x) for x in out)
 
 
 def _p(a: np.ndarray | None):

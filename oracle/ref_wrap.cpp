@@ -1,13 +1,15 @@
 /*
  * C-ABI wrapper around reference sources that compile with g++ as they lie
  * under the reference tree (no stand-in headers): the tail-queue admission
- * planner (src/dmma_tile_tail_queue.h) and the Matrix Market reader of the
+ * planner (src/dmma_tile_tail_queue.h), the values-only B update's clear gate
+ * (src/dmma_b_values_clear_policy.h) and the Matrix Market reader of the
  * standalone tool (src/external/cusparse/mmio_highlevel.h, whose mmio_data
  * follows the same placement as src/mmio_highlevel.h:593-759).
  *
  * TEST INFRASTRUCTURE ONLY: built into oracle/_ref/ by oracle/Makefile and
  * used by tests to pin the oracle restatement.
  */
+#include "dmma_b_values_clear_policy.h"
 #include "dmma_tile_tail_queue.h"
 #include "mmio_highlevel.h"
 
@@ -15,6 +17,33 @@
 #include <vector>
 
 extern "C" {
+
+// out = {clear_payload, skip_clear, fallback, reason}
+void ref_choose_b_values_clear(int policy, int values_only_update, int dynamic_b_valid,
+                               int has_active_duplicates, int mapping_complete, int injective,
+                               int payload_fully_initialized, int every_slot_overwritten,
+                               int numeric_b_read_only, int *out)
+{
+    const DmmaBValuesClearDecision d = dmma_choose_b_values_clear(
+        static_cast<DmmaBValuesClearPolicy>(policy), values_only_update != 0,
+        dynamic_b_valid != 0, has_active_duplicates != 0, mapping_complete != 0,
+        injective != 0, payload_fully_initialized != 0, every_slot_overwritten != 0,
+        numeric_b_read_only != 0);
+    out[0] = d.clear_payload ? 1 : 0;
+    out[1] = d.skip_clear ? 1 : 0;
+    out[2] = d.fallback ? 1 : 0;
+    out[3] = static_cast<int>(d.reason);
+}
+
+const char *ref_b_values_clear_policy_name(int policy)
+{
+    return dmma_b_values_clear_policy_name(static_cast<DmmaBValuesClearPolicy>(policy));
+}
+
+const char *ref_b_values_clear_reason_name(int reason)
+{
+    return dmma_b_values_clear_reason_name(static_cast<DmmaBValuesClearReason>(reason));
+}
 
 int ref_tail_chunk_count(int scans, int matches, double admission_threshold,
                          double chunk_target, int max_chunks)

@@ -35,6 +35,11 @@ struct rtt_dynamic_b {
     int aat = 0;
 };
 
+struct rtt_device_csr {
+    rtt::DevCsr csr;
+    rtt::BuildWork build;
+};
+
 namespace {
 
 using rtt::fail;
@@ -391,6 +396,133 @@ int rtt_rebuild_b(const rtt_prepared_a *a_const, int aat, rtt_dynamic_b **inout,
     RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
     st.total_ms = rtt::ms_since(t0);
     st.csr2tile_ms = st.total_ms - st.csr_row_permute_ms;
+    if (stats)
+        *stats = st;
+    return RTT_OK;
+}
+
+int rtt_upload_csr(const rtt_smatrix *host, rtt_device_csr **out, double *h2d_ms,
+                   double *validation_ms)
+{
+    rtt::clear_error();
+    RTT_TRY(rtt::require_ctx());
+    if (!host || !out)
+        return fail(RTT_ERR_INVALID, "rtt_upload_csr: NULL argument");
+    std::unique_ptr<rtt_device_csr> d(new (std::nothrow) rtt_device_csr());
+    if (!d)
+        return fail(RTT_ERR_OOM, "host allocation failed");
+    double h2d = 0, val = 0, sort = 0;
+    int sorted = 1;
+    RTT_TRY(rtt::upload_csr(*host, &d->csr, &d->build, &h2d, &val, &sort, &sorted));
+    if (h2d_ms)
+        *h2d_ms = h2d;
+    if (validation_ms)
+        *validation_ms = val + sort;
+    *out = d.release();
+    return RTT_OK;
+}
+
+int rtt_device_csr_set_values(rtt_device_csr *csr, const double *host_values)
+{
+    rtt::clear_error();
+    RTT_TRY(rtt::require_ctx());
+    if (!csr)
+        return fail(RTT_ERR_INVALID, "rtt_device_csr_set_values: NULL handle");
+    return rtt::set_csr_values(&csr->csr, host_values, &csr->build);
+}
+
+void rtt_destroy_device_csr(rtt_device_csr *csr)
+{
+    if (csr && rtt::ctx().stream)
+        (void)hipStreamSynchronize(rtt::ctx().stream);
+    delete csr;
+}
+
+int rtt_rebuild_b_general(const rtt_prepared_a *a_const, const rtt_device_csr *src,
+                          rtt_dynamic_b **inout, rtt_b_update_stats *stats)
+{
+    rtt::clear_error();
+    RTT_TRY(rtt::require_ctx());
+    if (!a_const || !src || !inout)
+        return fail(RTT_ERR_INVALID, "rtt_rebuild_b_general: NULL argument");
+    if (src->csr.rows != a_const->cols)
+        return fail(RTT_ERR_INVALID, "A is %d x %d but B has %d rows", a_const->rows,
+                    a_const->cols, src->csr.rows);
+    if (!*inout) {
+        *inout = new (std::nothrow) rtt_dynamic_b();
+        if (!*inout)
+            return fail(RTT_ERR_OOM, "host allocation failed");
+    }
+    rtt_dynamic_b *b = *inout;
+    b->aat = 0;
+    rtt_b_update_stats st;
+    std::memset(&st, 0, sizeof(st));
+    st.path = 3;
+    auto t0 = rtt::Clock::now();
+    const int *inner_map = a_const->identity ? nullptr : a_const->d_inner_new_to_old.as<int>();
+    // B's rows through A's inner permutation, all of B's columns, and the
+    // source-to-payload map for later values-only updates.
+    RTT_TRY(rtt::build_tiles(src->csr, inner_map, a_const->active_inner, src->csr.cols,
+                             a_const->cols, src->csr.cols, true, a_const->dense_threshold, false,
+                             &b->tiles, &b->build, true));
+    RTT_TRY(rtt::build_records(&b->tiles));
+    RTT_TRY(rtt::order_by_class(&b->tiles, &b->build));
+    RTT_TRY(rtt::build_pair_extents(a_const->tiles, &b->tiles));
+    RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
+    st.total_ms = rtt::ms_since(t0);
+    st.csr2tile_ms = st.total_ms;
+    if (stats)
+        *stats = st;
+    return RTT_OK;
+}
+
+int rtt_compute_nnz_cub_ab(const rtt_prepared_a *a_const, const rtt_device_csr *b,
+                           unsigned long long *nnz_cub, double *elapsed_ms)
+{
+    rtt::clear_error();
+    RTT_TRY(rtt::require_ctx());
+    if (!a_const || !b || !nnz_cub)
+        return fail(RTT_ERR_INVALID, "rtt_compute_nnz_cub_ab: bad argument");
+    rtt_prepared_a *a = const_cast<rtt_prepared_a *>(a_const);
+    auto t0 = rtt::Clock::now();
+    RTT_TRY(rtt::nnz_cub_ab(a->csr, b->csr, nnz_cub, &a->build));
+    if (elapsed_ms)
+        *elapsed_ms = rtt::ms_since(t0);
+    return RTT_OK;
+}
+
+int rtt_update_b_values(const rtt_prepared_a *a_const, const rtt_device_csr *src, int policy,
+                        rtt_dynamic_b *b, rtt_b_update_stats *stats,
+                        rtt_b_values_decision *decision)
+{
+    rtt::clear_error();
+    RTT_TRY(rtt::require_ctx());
+    if (!a_const || !src || !b)
+        return fail(RTT_ERR_INVALID, "rtt_update_b_values: NULL argument");
+    const rtt::DevTiles &t = b->tiles;
+    const bool valid = t.map_source_id != 0 && t.map_source_id == src->csr.structure_id &&
+                       t.map_nnz == src->csr.nnz && !t.transpose_of &&
+                       t.tile_row_count == a_const->tiles.tile_col_count;
+    // Every active source entry has a slot by construction, duplicates share
+    // one (so the map is injective exactly when there are none), the build
+    // writes every payload slot (dense padding as zeros the update never
+    // touches), and the numeric kernels only read B.
+    rtt_b_values_decision d;
+    RTT_TRY(rtt_choose_b_values_clear(policy, 1, valid ? 1 : 0, t.map_dups ? 1 : 0, 1,
+                                      t.map_dups ? 0 : 1, 1, 1, 1, &d));
+    if (decision)
+        *decision = d;
+    if (d.reason == RTT_B_VALUES_FALLBACK_INVALID_POLICY)
+        return fail(RTT_ERR_INVALID, "unknown B values clear policy %d", policy);
+    if (!valid)
+        return rtt_rebuild_b_general(a_const, src, &b, stats);
+    rtt_b_update_stats st;
+    std::memset(&st, 0, sizeof(st));
+    st.path = 4;
+    auto t0 = rtt::Clock::now();
+    RTT_TRY(rtt::update_b_values(src->csr, d.clear_payload != 0, &b->tiles));
+    RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
+    st.total_ms = rtt::ms_since(t0);
     if (stats)
         *stats = st;
     return RTT_OK;

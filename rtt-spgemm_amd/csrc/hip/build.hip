@@ -8,6 +8,7 @@
 // into the tile mask and writes the tile.  This replaces the reference's
 // 64-bit key radix sort + reduce_by_key (src/gpu_dmma_tiles.h:1117-1395) and
 // its 4-lane ordered-warp B path (src/gpu_dmma_tiles.h:313-560).
+#include <atomic>
 #include <hipcub/hipcub.hpp>
 
 #include <climits>
@@ -40,6 +41,33 @@ __global__ void iota_kernel(int *p, int64_t n)
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n)
         p[i] = static_cast<int>(i);
+}
+
+__global__ void fill_f64_kernel(double *p, int64_t n, double v)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+// out[x] = in[idx[x]]
+__global__ void gather_f64_kernel(int64_t n, const int *__restrict__ idx,
+                                  const double *__restrict__ in, double *__restrict__ out)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = in[idx[i]];
+}
+
+// Position of physical slot `phys` in the tile's values_cm order: stride-4
+// group k = phys & 3 first, then by phys >> 2 within the group.
+__device__ __forceinline__ int kmajor_slot(uint32_t mask, int phys)
+{
+    const int k = phys & 3;
+    const uint32_t lower_groups = k == 0 ? 0u : (k == 1 ? 0x11111111u
+                                                : (k == 2 ? 0x33333333u : 0x77777777u));
+    const uint32_t group = mask & (0x11111111u << k);
+    return __popc(mask & lower_groups) + __popc(group & ((1u << phys) - 1u));
 }
 
 // flags: 1 row_ptr not monotone / wrong ends, 2 column out of range,
@@ -83,7 +111,8 @@ __global__ __launch_bounds__(kThreads) void tile_build_kernel(
     int64_t *__restrict__ row_payload, int64_t *__restrict__ row_dense,
     const int64_t *__restrict__ tile_off, const int64_t *__restrict__ payload_off,
     int *__restrict__ tile_col_idx, uint32_t *__restrict__ masks,
-    int *__restrict__ value_offsets, double *__restrict__ values, int4 *__restrict__ meta)
+    int *__restrict__ value_offsets, double *__restrict__ values, int4 *__restrict__ meta,
+    int2 *__restrict__ src_slot, int *__restrict__ dup_flag)
 {
     static_assert(TR * TC == 32, "tiles hold 32 elements");
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -152,6 +181,7 @@ __global__ __launch_bounds__(kThreads) void tile_build_kernel(
             int x = p;
             while (x < q) {
                 const int c = ci[x];
+                const int x0 = x;
                 double s = val[x];
                 ++x;
                 while (x < q && ci[x] == c) {
@@ -160,7 +190,16 @@ __global__ __launch_bounds__(kThreads) void tile_build_kernel(
                 }
                 const int lc = c % TC;
                 const int phys = COL_MAJOR ? lc * TR + r : r * TC + lc;
-                values[p_out + payload_slot(mask, dense ? 32 : cnt, phys)] = s;
+                const int slot = static_cast<int>(p_out) + payload_slot(mask, dense ? 32 : cnt, phys);
+                values[slot] = s;
+                if (src_slot) {
+                    // Source entries x0 .. x-1 (duplicates of one coordinate) feed this slot.
+                    const int2 sl = make_int2(slot, static_cast<int>(p_out) + kmajor_slot(mask, phys));
+                    for (int y = x0; y < x; ++y)
+                        src_slot[y] = sl;
+                    if (x - x0 > 1)
+                        *dup_flag = 1;
+                }
             }
             ++t_out;
             p_out += dense ? 32 : cnt;
@@ -298,6 +337,41 @@ __global__ void records_kernel(int64_t n, const int4 *__restrict__ meta,
     const int part = static_cast<int>(gid / n);
     const int64_t t = gid - part * n;
     planes[gid] = record_part(part, meta[t], kmajor);
+}
+
+// Value parts (1 .. kRecParts-1) of B's record planes; part 0 (meta) is
+// left as it is.
+__global__ void record_values_kernel(int64_t n, const int4 *__restrict__ meta,
+                                     const double *__restrict__ kmajor,
+                                     int4 *__restrict__ planes)
+{
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= n * (kRecParts - 1))
+        return;
+    const int part = 1 + static_cast<int>(gid / n);
+    const int64_t t = gid - (part - 1) * n;
+    planes[part * n + t] = record_part(part, meta[t], kmajor);
+}
+
+// Values-only update: the first source entry of each slot's run sums the
+// run in entry order (the order the tile build sums duplicates in) and
+// stores the sum into both payload layouts.  Runs are adjacent because
+// duplicates of a coordinate are adjacent in a row-sorted CSR.
+__global__ void scatter_runs_kernel(int64_t nnz, const int2 *__restrict__ slot,
+                                    const double *__restrict__ v, double *__restrict__ values,
+                                    double *__restrict__ kmajor)
+{
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= nnz)
+        return;
+    const int2 s = slot[e];
+    if (s.x < 0 || (e > 0 && slot[e - 1].x == s.x))
+        return;
+    double sum = v[e];
+    for (int64_t f = e + 1; f < nnz && slot[f].x == s.x; ++f)
+        sum += v[f];
+    values[s.x] = sum;
+    kmajor[s.y] = sum;
 }
 
 // Offline, per A tile: the record a B tile built as A^T takes over, its meta
@@ -539,7 +613,7 @@ int build_csc(DevTiles *t, BuildWork *w)
 template <int TR, int TC, bool CM>
 int run_tile_build(const DevCsr &src, const int *row_map, int active_rows,
                    int active_cols, int dense_threshold, DevTiles *out,
-                   BuildWork *w)
+                   BuildWork *w, int2 *src_slot, int *dup_flag)
 {
     hipStream_t s = ctx().stream;
     const int R = out->tile_row_count;
@@ -569,7 +643,8 @@ int run_tile_build(const DevCsr &src, const int *row_map, int active_rows,
                            src.col_idx.as<int>(), src.values.as<double>(), row_map,
                            dense_threshold, w->row_tiles.as<int64_t>(),
                            w->row_payload.as<int64_t>(), w->row_dense.as<int64_t>(),
-                           nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+                           nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                           nullptr, nullptr);
     RTT_HIP(hipGetLastError());
     RTT_TRY(exclusive_scan_i64(w->row_tiles.as<int64_t>(), tile_pre, R, &w->scan_tmp, s));
     RTT_TRY(exclusive_scan_i64(w->row_payload.as<int64_t>(), pay_pre, R, &w->scan_tmp, s));
@@ -605,7 +680,7 @@ int run_tile_build(const DevCsr &src, const int *row_map, int active_rows,
                            dense_threshold, nullptr, nullptr, nullptr, tile_pre, pay_pre,
                            out->tile_col_idx.as<int>(), out->masks.as<uint32_t>(),
                            out->value_offsets.as<int>(), out->values.as<double>(),
-                           out->meta.as<int4>());
+                           out->meta.as<int4>(), src_slot, dup_flag);
     RTT_HIP(hipGetLastError());
     return RTT_OK;
 }
@@ -829,21 +904,30 @@ int upload_csr(const rtt_smatrix &host, DevCsr *out, BuildWork *w, double *h2d_m
     *rows_were_sorted = (hflags & 4) ? 0 : 1;
 
     t0 = Clock::now();
+    out->reordered = false;
     if (hflags & 4) {
         // Stable per-row sort by column; duplicates keep their input order.
-        DevBuf keys, vals;
+        // input_of[x] = input position of sorted entry x, kept for value updates.
+        DevBuf keys, ids_in, vals;
         RTT_TRY(keys.ensure(static_cast<size_t>(host.nnz) * sizeof(int)));
+        RTT_TRY(ids_in.ensure(static_cast<size_t>(host.nnz) * sizeof(int)));
         RTT_TRY(vals.ensure(static_cast<size_t>(host.nnz) * sizeof(double)));
+        RTT_TRY(out->input_of.ensure(static_cast<size_t>(host.nnz) * sizeof(int)));
+        hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(host.nnz, kThreads)), dim3(kThreads), 0,
+                           s, ids_in.as<int>(), static_cast<int64_t>(host.nnz));
         size_t bytes = 0;
         const int *rp = out->row_ptr.as<int>();
         RTT_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
-            nullptr, bytes, out->col_idx.as<int>(), keys.as<int>(), out->values.as<double>(),
-            vals.as<double>(), host.nnz, host.m, rp, rp + 1, 0, bits_for(host.n), s));
+            nullptr, bytes, out->col_idx.as<int>(), keys.as<int>(), ids_in.as<int>(),
+            out->input_of.as<int>(), host.nnz, host.m, rp, rp + 1, 0, bits_for(host.n), s));
         RTT_TRY(w->scan_tmp.ensure(bytes));
         RTT_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
-            w->scan_tmp.ptr, bytes, out->col_idx.as<int>(), keys.as<int>(),
-            out->values.as<double>(), vals.as<double>(), host.nnz, host.m, rp, rp + 1, 0,
-            bits_for(host.n), s));
+            w->scan_tmp.ptr, bytes, out->col_idx.as<int>(), keys.as<int>(), ids_in.as<int>(),
+            out->input_of.as<int>(), host.nnz, host.m, rp, rp + 1, 0, bits_for(host.n), s));
+        hipLaunchKernelGGL(gather_f64_kernel, dim3(blocks_for(host.nnz, kThreads)),
+                           dim3(kThreads), 0, s, static_cast<int64_t>(host.nnz),
+                           out->input_of.as<int>(), out->values.as<double>(), vals.as<double>());
+        RTT_HIP(hipGetLastError());
         RTT_HIP(hipMemcpyAsync(out->col_idx.ptr, keys.ptr,
                                static_cast<size_t>(host.nnz) * sizeof(int),
                                hipMemcpyDeviceToDevice, s));
@@ -851,14 +935,17 @@ int upload_csr(const rtt_smatrix &host, DevCsr *out, BuildWork *w, double *h2d_m
                                static_cast<size_t>(host.nnz) * sizeof(double),
                                hipMemcpyDeviceToDevice, s));
         RTT_HIP(hipStreamSynchronize(s));
+        out->reordered = true;
     }
+    static std::atomic<uint64_t> next_structure_id{1};
+    out->structure_id = next_structure_id.fetch_add(1);
     *sort_ms = ms_since(t0);
     return RTT_OK;
 }
 
 int build_tiles(const DevCsr &src, const int *row_map, int active_rows, int active_cols,
                 int logical_rows, int logical_cols, bool col_major, int dense_threshold,
-                bool want_csc, DevTiles *out, BuildWork *w)
+                bool want_csc, DevTiles *out, BuildWork *w, bool want_source_map)
 {
     RTT_TRY(require_ctx());
     const int TR = col_major ? kTileK : kTileM;
@@ -872,12 +959,41 @@ int build_tiles(const DevCsr &src, const int *row_map, int active_rows, int acti
     out->has_csc = false;
     out->arrays_ready = true;
     out->transpose_of = nullptr;
+    hipStream_t s = ctx().stream;
+    int2 *src_slot = nullptr;
+    int *dup_flag = nullptr;
+    out->map_source_id = 0;
+    out->map_nnz = 0;
+    out->map_dups = false;
+    if (want_source_map) {
+        RTT_TRY(out->src_slot.ensure(static_cast<size_t>(src.nnz > 0 ? src.nnz : 1) *
+                                     sizeof(int2)));
+        RTT_TRY(w->flag.ensure(sizeof(int)));
+        src_slot = out->src_slot.as<int2>();
+        dup_flag = w->flag.as<int>();
+        hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(1), 0, s, dup_flag, int64_t(1), 0);
+        if (src.nnz > 0)
+            hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(2 * src.nnz, kThreads)),
+                               dim3(kThreads), 0, s, reinterpret_cast<int *>(src_slot),
+                               2 * src.nnz, -1);
+        RTT_HIP(hipGetLastError());
+    }
     if (col_major)
         RTT_TRY((run_tile_build<kTileK, kTileN, true>(src, row_map, active_rows, active_cols,
-                                                      dense_threshold, out, w)));
+                                                      dense_threshold, out, w, src_slot,
+                                                      dup_flag)));
     else
         RTT_TRY((run_tile_build<kTileM, kTileK, false>(src, row_map, active_rows, active_cols,
-                                                       dense_threshold, out, w)));
+                                                       dense_threshold, out, w, src_slot,
+                                                       dup_flag)));
+    if (want_source_map) {
+        int dups = 0;
+        RTT_HIP(hipMemcpyAsync(&dups, dup_flag, sizeof(int), hipMemcpyDeviceToHost, s));
+        RTT_HIP(hipStreamSynchronize(s));
+        out->map_source_id = src.structure_id;
+        out->map_nnz = src.nnz;
+        out->map_dups = dups != 0;
+    }
     if (want_csc)
         RTT_TRY(build_csc(out, w));
     return RTT_OK;
@@ -1079,6 +1195,77 @@ int nnz_cub(const DevCsr &csr, bool aat, unsigned long long *result, BuildWork *
                            dim3(kThreads),
                            0, s, csr.nnz, csr.col_idx.as<int>(), csr.row_ptr.as<int>(), deg,
                            acc);
+    RTT_HIP(hipGetLastError());
+    RTT_HIP(hipMemcpyAsync(result, acc, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    RTT_HIP(hipStreamSynchronize(s));
+    return RTT_OK;
+}
+
+int set_csr_values(DevCsr *csr, const double *host_values, BuildWork *w)
+{
+    RTT_TRY(require_ctx());
+    hipStream_t s = ctx().stream;
+    if (csr->nnz == 0)
+        return RTT_OK;
+    if (!host_values)
+        return fail(RTT_ERR_INVALID, "NULL values for a CSR with %lld entries",
+                    (long long)csr->nnz);
+    const size_t bytes = static_cast<size_t>(csr->nnz) * sizeof(double);
+    if (!csr->reordered) {
+        RTT_HIP(hipMemcpyAsync(csr->values.ptr, host_values, bytes, hipMemcpyHostToDevice, s));
+    } else {
+        RTT_TRY(w->staging.ensure(bytes));
+        RTT_HIP(hipMemcpyAsync(w->staging.ptr, host_values, bytes, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(gather_f64_kernel, dim3(blocks_for(csr->nnz, kThreads)),
+                           dim3(kThreads), 0, s, csr->nnz, csr->input_of.as<int>(),
+                           w->staging.as<double>(), csr->values.as<double>());
+        RTT_HIP(hipGetLastError());
+    }
+    RTT_HIP(hipStreamSynchronize(s));
+    return RTT_OK;
+}
+
+int update_b_values(const DevCsr &src, bool clear, DevTiles *b)
+{
+    RTT_TRY(require_ctx());
+    hipStream_t s = ctx().stream;
+    if (b->map_source_id == 0 || b->map_source_id != src.structure_id ||
+        b->map_nnz != src.nnz || b->transpose_of)
+        return fail(RTT_ERR_INVALID, "B has no source map for this CSR");
+    if (clear && b->payload > 0) {
+        hipLaunchKernelGGL(fill_f64_kernel, dim3(blocks_for(b->payload, kThreads)),
+                           dim3(kThreads), 0, s, b->values.as<double>(), b->payload, 0.0);
+        hipLaunchKernelGGL(fill_f64_kernel, dim3(blocks_for(b->payload, kThreads)),
+                           dim3(kThreads), 0, s, b->values_cm.as<double>(), b->payload, 0.0);
+    }
+    if (src.nnz > 0)
+        hipLaunchKernelGGL(scatter_runs_kernel, dim3(blocks_for(src.nnz, kThreads)),
+                           dim3(kThreads), 0, s, src.nnz, b->src_slot.as<int2>(),
+                           src.values.as<double>(), b->values.as<double>(),
+                           b->values_cm.as<double>());
+    if (b->num_tiles > 0)
+        hipLaunchKernelGGL(record_values_kernel,
+                           dim3(blocks_for(b->num_tiles * (kRecParts - 1), kThreads)),
+                           dim3(kThreads), 0, s, b->num_tiles, b->meta.as<int4>(),
+                           b->values_cm.as<double>(), b->records.as<int4>());
+    RTT_HIP(hipGetLastError());
+    return RTT_OK;
+}
+
+int nnz_cub_ab(const DevCsr &a, const DevCsr &b, unsigned long long *result, BuildWork *w)
+{
+    RTT_TRY(require_ctx());
+    if (a.cols != b.rows)
+        return fail(RTT_ERR_INVALID, "A has %d columns but B has %d rows", a.cols, b.rows);
+    hipStream_t s = ctx().stream;
+    RTT_TRY(w->counters.ensure(sizeof(unsigned long long)));
+    unsigned long long *acc = w->counters.as<unsigned long long>();
+    RTT_HIP(hipMemsetAsync(acc, 0, sizeof(unsigned long long), s));
+    if (a.nnz > 0)
+        hipLaunchKernelGGL(nnz_cub_kernel,
+                           dim3(static_cast<unsigned>(std::min<int64_t>(1024, blocks_for(a.nnz, kThreads)))),
+                           dim3(kThreads), 0, s, a.nnz, a.col_idx.as<int>(), b.row_ptr.as<int>(),
+                           static_cast<const int64_t *>(nullptr), acc);
     RTT_HIP(hipGetLastError());
     RTT_HIP(hipMemcpyAsync(result, acc, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     RTT_HIP(hipStreamSynchronize(s));

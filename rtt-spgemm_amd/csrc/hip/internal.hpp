@@ -86,6 +86,10 @@ struct DevCsr {
     int rows = 0, cols = 0;
     int64_t nnz = 0;
     DevBuf row_ptr, col_idx, values;
+    // When the upload sorted rows: input_of[x] = input position of sorted entry x.
+    bool reordered = false;
+    DevBuf input_of;
+    uint64_t structure_id = 0;   // unique per upload; kept by value updates
 };
 
 // Hybrid tile matrix on the device (the DmmaDeviceTiles fields) plus the
@@ -117,6 +121,13 @@ struct DevTiles {
     // sorted_ids[b] is the A tile behind B tile b.
     const DevTiles *transpose_of = nullptr;
     DevBuf sorted_ids;
+    // B built from a CSR with a source map: src_slot[x] = {values slot,
+    // values_cm slot} of source entry x, {-1, -1} outside the tiled extent.
+    // Duplicate entries share a slot and are adjacent.
+    DevBuf src_slot;
+    uint64_t map_source_id = 0;  // DevCsr::structure_id the map belongs to (0: none)
+    int64_t map_nnz = 0;
+    bool map_dups = false;       // some slot has more than one source entry
     const double *payload_values() const
     {
         return transpose_of ? transpose_of->values.as<double>() : values.as<double>();
@@ -151,6 +162,8 @@ struct BuildWork {
     DevBuf keys_in, keys_out, ids_in, ids_out, spans;
     DevBuf meta2, records2;                     // heavy-first reordering
     DevBuf tcsr_row_ptr, tcsr_col, tcsr_val;   // transposed CSR (reordered AAT)
+    DevBuf flag;                                // one int (duplicate detection)
+    DevBuf staging;                             // uploaded values before the gather
 };
 
 struct Events {
@@ -179,7 +192,14 @@ int upload_csr(const rtt_smatrix &host, DevCsr *out, BuildWork *work,
 int build_tiles(const DevCsr &src, const int *row_map, int active_rows,
                 int active_cols, int logical_rows, int logical_cols,
                 bool col_major, int dense_threshold, bool build_csc,
-                DevTiles *out, BuildWork *work);
+                DevTiles *out, BuildWork *work, bool want_source_map = false);
+// Values-only update of a B built with a source map from `src`: optional
+// clear, in-order sums of each slot's source entries into values and
+// values_cm, then the value parts of the records.
+int update_b_values(const DevCsr &src, bool clear, DevTiles *b);
+// Uploaded values in the caller's entry order -> src->values (sorted order).
+int set_csr_values(DevCsr *csr, const double *host_values, BuildWork *work);
+int nnz_cub_ab(const DevCsr &a, const DevCsr &b, unsigned long long *out, BuildWork *work);
 // B = A^T at tile level (A tile (J,K) row-major == B tile (K,J) col-major).
 // Builds B's tile row pointer and meta only; materialize_tiles() fills the
 // remaining DmmaDeviceTiles arrays when a caller asks for them.

@@ -83,6 +83,14 @@ class BUpdateStats(C.Structure):
                 ("csr2tile_ms", C.c_double), ("csc_ms", C.c_double), ("path", C.c_int)]
 
 
+class BValuesDecision(C.Structure):
+    _fields_ = [("clear_payload", C.c_int), ("skip_clear", C.c_int), ("fallback", C.c_int),
+                ("reason", C.c_int)]
+
+
+B_VALUES_POLICIES = {"always-clear": 0, "noclear": 1, "safe-auto": 2}
+
+
 class Schedule(C.Structure):
     _fields_ = [("lb_mode", C.c_int), ("materialize_output", C.c_int),
                 ("split_threshold", C.c_double), ("split_chunk_target", C.c_double),
@@ -142,6 +150,16 @@ def load() -> C.CDLL:
         "rtt_dynamic_b_view": (C.c_int, [vp, P(DeviceTiles)]),
         "rtt_destroy_dynamic_b": (None, [vp]),
         "rtt_compute_nnz_cub": (C.c_int, [vp, C.c_int, P(C.c_ulonglong), P(C.c_double)]),
+        "rtt_upload_csr": (C.c_int, [P(SMatrix), P(vp), P(C.c_double), P(C.c_double)]),
+        "rtt_device_csr_set_values": (C.c_int, [vp, vp]),
+        "rtt_destroy_device_csr": (None, [vp]),
+        "rtt_rebuild_b_general": (C.c_int, [vp, vp, P(vp), P(BUpdateStats)]),
+        "rtt_compute_nnz_cub_ab": (C.c_int, [vp, vp, P(C.c_ulonglong), P(C.c_double)]),
+        "rtt_choose_b_values_clear": (C.c_int, [C.c_int] * 9 + [P(BValuesDecision)]),
+        "rtt_b_values_clear_policy_name": (C.c_char_p, [C.c_int]),
+        "rtt_b_values_clear_reason_name": (C.c_char_p, [C.c_int]),
+        "rtt_update_b_values": (C.c_int, [vp, vp, C.c_int, vp, P(BUpdateStats),
+                                          P(BValuesDecision)]),
         "rtt_schedule_defaults": (None, [P(Schedule)]),
         "rtt_tilespgemm": (C.c_int, [vp, vp, P(SMatrix), P(Stats), P(Schedule)]),
         "rtt_last_c_view": (C.c_int, [vp, P(DeviceC)]),
@@ -431,6 +449,88 @@ def nnz_cub(a: PreparedA, aat: int) -> int:
     ms = C.c_double()
     _check(load().rtt_compute_nnz_cub(a.handle, int(aat), C.byref(v), C.byref(ms)), "nnz_cub")
     return int(v.value)
+
+
+class DeviceCsr:
+    """Owns an rtt_device_csr handle: an independent B uploaded to the GPU
+    (DmmaOwnedDeviceCsr, src/gpu_dmma_tiles.h:32)."""
+
+    def __init__(self, csr: CsrMatrix):
+        s = csr.as_smatrix()
+        self.handle = C.c_void_p()
+        h2d, val = C.c_double(), C.c_double()
+        _check(load().rtt_upload_csr(C.byref(s), C.byref(self.handle), C.byref(h2d),
+                                     C.byref(val)), "upload_csr")
+        self.m, self.n, self.nnz = csr.m, csr.n, csr.nnz
+        self.stats = {"h2d_ms": h2d.value, "validation_ms": val.value}
+
+    def __del__(self):
+        if getattr(self, "handle", None) and self.handle.value and _lib is not None:
+            _lib.rtt_destroy_device_csr(self.handle)
+            self.handle = C.c_void_p()
+
+    def set_values(self, values) -> None:
+        """New values, in the entry order of the CSR this handle was uploaded
+        from; the structure stays."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.shape != (self.nnz,):
+            raise ValueError(f"expected {self.nnz} values, got shape {v.shape}")
+        _check(load().rtt_device_csr_set_values(self.handle, _ptr(v)), "device_csr_set_values")
+
+
+def upload_csr(csr: CsrMatrix) -> DeviceCsr:
+    """gpu_upload_csr (src/gpu_dmma_tiles.h:2259) for an independent B."""
+    return DeviceCsr(csr)
+
+
+def rebuild_b_general(a: PreparedA, src: DeviceCsr, b: DynamicB | None = None) -> DynamicB:
+    """B for C = A*B from an independent CSR (--b, src/main.cu:1417-1420)."""
+    b = b or DynamicB()
+    st = BUpdateStats()
+    _check(load().rtt_rebuild_b_general(a.handle, src.handle, C.byref(b.handle), C.byref(st)),
+           "rebuild_b_general")
+    b.last = {f: getattr(st, f) for f, _ in BUpdateStats._fields_}
+    return b
+
+
+def nnz_cub_ab(a: PreparedA, src: DeviceCsr) -> int:
+    v = C.c_ulonglong()
+    ms = C.c_double()
+    _check(load().rtt_compute_nnz_cub_ab(a.handle, src.handle, C.byref(v), C.byref(ms)),
+           "nnz_cub_ab")
+    return int(v.value)
+
+
+def choose_b_values_clear(policy: int, values_only_update=1, dynamic_b_valid=1,
+                          has_active_duplicates=0, active_source_mapping_complete=1,
+                          active_source_to_payload_injective=1, payload_fully_initialized=1,
+                          every_payload_slot_overwritten=1, numeric_b_read_only=1) -> dict:
+    """dmma_choose_b_values_clear (src/dmma_b_values_clear_policy.h:91-160)."""
+    d = BValuesDecision()
+    lib = load()
+    _check(lib.rtt_choose_b_values_clear(
+        int(policy), int(values_only_update), int(dynamic_b_valid), int(has_active_duplicates),
+        int(active_source_mapping_complete), int(active_source_to_payload_injective),
+        int(payload_fully_initialized), int(every_payload_slot_overwritten),
+        int(numeric_b_read_only), C.byref(d)), "choose_b_values_clear")
+    out = {f: getattr(d, f) for f, _ in BValuesDecision._fields_}
+    out["reason_name"] = lib.rtt_b_values_clear_reason_name(d.reason).decode()
+    return out
+
+
+def update_b_values(a: PreparedA, src: DeviceCsr, b: DynamicB,
+                    policy: str = "safe-auto") -> dict:
+    """Values-only B update: same structure as the build from `src`, new
+    values (DeviceCsr.set_values).  Returns the clear decision."""
+    st = BUpdateStats()
+    d = BValuesDecision()
+    lib = load()
+    _check(lib.rtt_update_b_values(a.handle, src.handle, B_VALUES_POLICIES[policy], b.handle,
+                                   C.byref(st), C.byref(d)), "update_b_values")
+    b.last = {f: getattr(st, f) for f, _ in BUpdateStats._fields_}
+    out = {f: getattr(d, f) for f, _ in BValuesDecision._fields_}
+    out["reason_name"] = lib.rtt_b_values_clear_reason_name(d.reason).decode()
+    return out
 
 
 def schedule(lb_mode: str = "adaptive-tail", materialize_output: bool = True, **kw) -> Schedule:

@@ -173,3 +173,52 @@ def test_gpu_entry_points_fail_loudly_without_init():
     a = rtt_amd.gen_laplace2d(8, 0.1, 1)
     with pytest.raises(rtt_amd.RttError):
         rtt_amd.prepare_identity_a(a)
+
+
+def test_b_values_clear_policy_matches_reference():
+    """rtt_choose_b_values_clear against the reference's own
+    dmma_choose_b_values_clear (src/dmma_b_values_clear_policy.h:91-160),
+    compiled from its header into oracle/_ref: every policy value (valid and
+    invalid) times every combination of the eight preconditions, and every
+    reason and policy name."""
+    if oracle.ref_lib() is None:
+        pytest.skip("oracle/_ref/libref.so was not built (no reference tree)")
+    import itertools
+    lib = rtt_amd.load()
+    for policy in (-1, 0, 1, 2, 3):
+        for flags in itertools.product((0, 1), repeat=8):
+            got = rtt_amd.choose_b_values_clear(policy, *flags)
+            want = oracle.ref_choose_b_values_clear(policy, *flags)
+            assert (got["clear_payload"], got["skip_clear"], got["fallback"],
+                    got["reason"]) == want, (policy, flags)
+    ref = oracle.ref_lib()
+    for reason in range(-1, 13):
+        assert lib.rtt_b_values_clear_reason_name(reason) == \
+            ref.ref_b_values_clear_reason_name(reason)
+    for policy in range(-1, 4):
+        assert lib.rtt_b_values_clear_policy_name(policy) == \
+            ref.ref_b_values_clear_policy_name(policy)
+
+
+def test_b_values_clear_policy_known_cases():
+    """Hand cases: always-clear never skips; a valid values-only update of a
+    duplicate-free B skips the clear under noclear and safe-auto; the first
+    failing precondition is the reason."""
+    ok = dict(values_only_update=1, dynamic_b_valid=1, has_active_duplicates=0,
+              active_source_mapping_complete=1, active_source_to_payload_injective=1,
+              payload_fully_initialized=1, every_payload_slot_overwritten=1,
+              numeric_b_read_only=1)
+    d = rtt_amd.choose_b_values_clear(0, **ok)
+    assert (d["clear_payload"], d["skip_clear"], d["fallback"]) == (1, 0, 0)
+    assert d["reason_name"] == "policy-always-clear"
+    for pol, name in ((1, "eligible-explicit-noclear"), (2, "eligible-safe-auto")):
+        d = rtt_amd.choose_b_values_clear(pol, **ok)
+        assert (d["clear_payload"], d["skip_clear"], d["fallback"]) == (0, 1, 0)
+        assert d["reason_name"] == name
+    d = rtt_amd.choose_b_values_clear(2, **dict(ok, has_active_duplicates=1,
+                                                 numeric_b_read_only=0))
+    assert d["reason_name"] == "active-source-duplicates" and d["clear_payload"] == 1
+    d = rtt_amd.choose_b_values_clear(1, **dict(ok, values_only_update=0, dynamic_b_valid=0))
+    assert d["reason_name"] == "structure-update-legacy-rebuild"
+    d = rtt_amd.choose_b_values_clear(5, **ok)
+    assert d["reason_name"] == "invalid-policy"
