@@ -1,4 +1,5 @@
-// rtt_spgemm: drop-in driver for the C = A*A / C = A*A^T benchmark path.
+// rtt_spgemm: drop-in driver for the C = A*A / C = A*A^T / C = A*B benchmark
+// path.
 //
 // Mirrors the reference driver's flags (src/main.cu:168-309), its timed
 // boundary (B rebuild + symbolic + allocation + numeric, median over the
@@ -29,7 +30,12 @@ struct Options {
     const char *inner_order = nullptr;
     const char *reorder_name = nullptr;
     const char *a_file = nullptr;
+    const char *b_file = nullptr;      // --b: general A*B
+    int b_update = 0;                  // 0 none, 1 values, 2 permutation, 3 structure
+    int b_clear_policy = RTT_B_VALUES_SAFE_AUTO;
 };
+
+const char *const kBUpdateNames[] = {"none", "values", "permutation", "structure"};
 
 void usage(const char *prog)
 {
@@ -37,6 +43,11 @@ void usage(const char *prog)
         "Usage: %s [options] A.mtx\n"
         "  -d GPU                         HIP device (default: 0)\n"
         "  -aat 0|1                       C = A*A (0) or C = A*A^T (1)\n"
+        "  --b B.mtx                      compute general A*B; otherwise A*A\n"
+        "  --b-update none|values|permutation|structure (default: none)\n"
+        "                                 values with --b: timed values-only\n"
+        "                                 update of B; otherwise a full rebuild\n"
+        "  --b-values-clear always-clear|noclear|safe-auto (default: safe-auto)\n"
         "  --lb-mode static|dynamic|adaptive-tail (default: adaptive-tail)\n"
         "  --warmup-iterations N          warmups (default: 0)\n"
         "  --iterations N                 repeats (default: 5)\n"
@@ -100,6 +111,27 @@ bool parse(int argc, char **argv, Options *o)
             o->reference_values = true;
         } else if (!std::strcmp(a, "--prepare-only")) {
             o->prepare_only = true;
+        } else if (!std::strcmp(a, "--b")) {
+            if (!(o->b_file = next()))
+                return false;
+        } else if (!std::strcmp(a, "--b-update")) {
+            const char *v = next();
+            int mode = -1;
+            for (int k = 0; v && k < 4; ++k)
+                if (!std::strcmp(v, kBUpdateNames[k]))
+                    mode = k;
+            if (mode < 0)
+                return false;
+            o->b_update = mode;
+        } else if (!std::strcmp(a, "--b-values-clear")) {
+            const char *v = next();
+            int pol = -1;
+            for (int k = 0; v && k < 3; ++k)
+                if (!std::strcmp(v, rtt_b_values_clear_policy_name(k)))
+                    pol = k;
+            if (pol < 0)
+                return false;
+            o->b_clear_policy = pol;
         } else if (a[0] != '-' && !o->a_file) {
             o->a_file = a;
         } else {
@@ -143,13 +175,37 @@ int main(int argc, char **argv)
     std::memset(&A, 0, sizeof(A));
     if (check(rtt_read_mtx(o.a_file, o.reference_values ? 1 : 0, &A), "read"))
         return EXIT_FAILURE;
-    if (!o.aat && A.m != A.n) {
+    const bool general_ab = o.b_file != nullptr;
+    rtt_smatrix B;
+    std::memset(&B, 0, sizeof(B));
+    if (general_ab) {
+        if (check(rtt_read_mtx(o.b_file, o.reference_values ? 1 : 0, &B), "read B"))
+            return EXIT_FAILURE;
+        if (A.n != B.m) {
+            std::fprintf(stderr, "A*B needs A.n == B.m (got %d x %d times %d x %d).\n", A.m,
+                         A.n, B.m, B.n);
+            return EXIT_FAILURE;
+        }
+        if (o.aat)
+            std::printf("Note: --b selects general A*B; -aat is ignored.\n");
+        o.aat = 0;
+    } else if (!o.aat && A.m != A.n) {
         std::fprintf(stderr, "C = A*A needs a square matrix (got %d x %d).\n", A.m, A.n);
         return EXIT_FAILURE;
     }
     if (check(rtt_init(o.device), "rtt_init"))
         return EXIT_FAILURE;
     std::printf("A: %d x %d, nnz %d, aat %d, lb-mode %s\n", A.m, A.n, A.nnz, o.aat, o.lb_name);
+    rtt_device_csr *bsrc = nullptr;
+    if (general_ab) {
+        double h2d = 0, val = 0;
+        if (check(rtt_upload_csr(&B, &bsrc, &h2d, &val), "B upload"))
+            return EXIT_FAILURE;
+        std::printf("B: %d x %d, nnz %d; B CSR H2D = %.3f ms; validation = %.3f ms\n", B.m, B.n,
+                    B.nnz, h2d, val);
+    } else {
+        std::printf("B source reuses A device CSR; additional CSR H2D = 0 ms\n");
+    }
 
     rtt_prepared_a *pa = nullptr;
     rtt_offline_a_stats ast;
@@ -182,7 +238,9 @@ int main(int argc, char **argv)
     }
     unsigned long long nnz_cub = 0;
     double cub_ms = 0;
-    if (check(rtt_compute_nnz_cub(pa, o.aat, &nnz_cub, &cub_ms), "nnzCub"))
+    if (check(general_ab ? rtt_compute_nnz_cub_ab(pa, bsrc, &nnz_cub, &cub_ms)
+                         : rtt_compute_nnz_cub(pa, o.aat, &nnz_cub, &cub_ms),
+              "nnzCub"))
         return EXIT_FAILURE;
     std::printf("SpGEMM nnzCub = %llu; GPU nnzCub time = %.3f ms\n", nnz_cub, cub_ms);
 
@@ -192,6 +250,15 @@ int main(int argc, char **argv)
     rtt_dynamic_b *pb = nullptr;
     rtt_smatrix C;
     std::memset(&C, 0, sizeof(C));
+    // The initial (untimed) B build, as the reference does before its
+    // samples (src/main.cu:1996-2005).
+    rtt_b_update_stats bst0;
+    if (check(general_ab ? rtt_rebuild_b_general(pa, bsrc, &pb, &bst0)
+                         : rtt_rebuild_b(pa, o.aat, &pb, &bst0),
+              "initial B rebuild"))
+        return EXIT_FAILURE;
+    std::printf("Initial B rebuild %.3f ms (path %d)\n", bst0.total_ms, bst0.path);
+    const bool values_only = general_ab && o.b_update == 1;
     std::vector<double> b_ms, b_perm_ms, b_tile_ms, sym_ms, alloc_ms, num_ms, total_ms;
     for (int it = 0; it < o.warmup + o.iterations; ++it) {
         const bool timed = it >= o.warmup;
@@ -199,9 +266,20 @@ int main(int argc, char **argv)
         sched.materialize_output = last ? 1 : 0;
         rtt_b_update_stats bst;
         rtt_stats st;
-        if (check(rtt_sync(), "sync") || check(rtt_rebuild_b(pa, o.aat, &pb, &bst), "B rebuild") ||
-            check(rtt_tilespgemm(pa, pb, &C, &st, &sched), "SpGEMM"))
+        rtt_b_values_decision dec;
+        if (check(rtt_sync(), "sync"))
             return EXIT_FAILURE;
+        if (values_only)
+            rc = rtt_update_b_values(pa, bsrc, o.b_clear_policy, pb, &bst, &dec);
+        else if (general_ab)
+            rc = rtt_rebuild_b_general(pa, bsrc, &pb, &bst);
+        else
+            rc = rtt_rebuild_b(pa, o.aat, &pb, &bst);
+        if (check(rc, "B update") || check(rtt_tilespgemm(pa, pb, &C, &st, &sched), "SpGEMM"))
+            return EXIT_FAILURE;
+        if (values_only && it == 0)
+            std::printf("B_UPDATE mode=values path=%d clear=%d reason=%s\n", bst.path,
+                        dec.clear_payload, rtt_b_values_clear_reason_name(dec.reason));
         if (!timed)
             continue;
         b_ms.push_back(bst.total_ms);
@@ -219,7 +297,10 @@ int main(int argc, char **argv)
     size_t sel = 0;
     const double runtime = median_of(total_ms, &sel);
     const double gflops = runtime > 0 ? 2.0 * static_cast<double>(nnz_cub) / (runtime * 1e6) : 0.0;
-    std::printf("RTT-SpGEMM (%s), repeats = %d, statistic = median\n", o.lb_name, o.iterations);
+    std::printf("RTT-SpGEMM (%s), repeats = %d, statistic = median, b_update_mode = %s, "
+                "product = %s\n",
+                o.lb_name, o.iterations, kBUpdateNames[o.b_update],
+                general_ab ? "AB" : (o.aat ? "AAT" : "AA"));
     std::printf("B online preparation runtime is %.3f ms\n", b_ms[sel]);
     std::printf("  CSR row permutation is %.3f ms; CSR2Tile is %.3f ms\n", b_perm_ms[sel],
                 b_tile_ms[sel]);
@@ -246,8 +327,11 @@ int main(int argc, char **argv)
     std::printf("C: %d x %d, nnz %d, tiles %d\n", C.m, C.n, C.nnz, C.numtile);
     rtt_destroy_smatrix(&C);
     rtt_destroy_dynamic_b(pb);
+    rtt_destroy_device_csr(bsrc);
     rtt_destroy_prepared_a(pa);
     rtt_destroy_smatrix(&A);
+    if (general_ab)
+        rtt_destroy_smatrix(&B);
     std::printf("---------------------------------------------------------------\n");
     return EXIT_SUCCESS;
 }

@@ -359,8 +359,19 @@ def _download_tiles(view: DeviceTiles) -> dict:
     return out
 
 
+_initialised = False
+DRIVER_PATH = os.path.join(os.path.dirname(_HERE), "bin", "rtt_spgemm")
+
+
 def init(device: int = 0) -> None:
+    global _initialised
     _check(load().rtt_init(device), "rtt_init")
+    _initialised = True
+
+
+def initialised() -> bool:
+    """True once init() has opened a device in this process."""
+    return _initialised
 
 
 def sync() -> None:
