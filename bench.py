@@ -204,6 +204,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="stencil27-128-aat", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--lb-mode", default="adaptive-tail",
+                    choices=["static", "dynamic", "adaptive-tail"],
+                    help="row assignment of the slab kernels (the reference's --lb-mode)")
     ap.add_argument("--no-control", action="store_true",
                     help="skip the rocSPARSE SpGEMM control on the same workload")
     ap.add_argument("--control-only", action="store_true", help=argparse.SUPPRESS)
@@ -230,7 +233,7 @@ def main():
     rt.init(0)
     pa = rt.prepare_identity_a(a)
     nnz_cub = rt.nnz_cub(pa, aat)
-    sched = rt.schedule(materialize_output=False)
+    sched = rt.schedule(args.lb_mode, materialize_output=False)
     pb = None
     for _ in range(args.warmup):
         pb = rt.rebuild_b(pa, aat, pb)
@@ -303,7 +306,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: C = A*A{'^T' if aat else ''}, {desc}, "
                                f"{a.m} rows, {a.nnz} nnz, nnzCub {nnz_cub}, nnzC {nnz_c}",
-                   "precision": "plain FP64", "lb_mode": "adaptive-tail",
+                   "precision": "plain FP64", "lb_mode": args.lb_mode,
                    "pipeline_hbm_frac": round(b_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                               5)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),

@@ -603,7 +603,8 @@ int rtt_tilespgemm(const rtt_prepared_a *a_const, const rtt_dynamic_b *b, rtt_sm
     auto t0 = rtt::Clock::now();
     rtt::CoreTimes times;
     a->has_c = false;
-    RTT_TRY(rtt::spgemm_core(a->tiles, b->tiles, &a->last_c, &a->core, &a->events, &times));
+    RTT_TRY(rtt::spgemm_core(a->tiles, b->tiles, &a->last_c, &a->core, &a->events, &times,
+                            sc.lb_mode));
     a->has_c = true;
     stats->symbolic_ms = times.symbolic_ms;
     stats->allocation_ms = times.allocation_ms;

@@ -154,6 +154,8 @@ struct SpgemmWork {
     PinnedBuf host_counters;
     DevBuf stage_j, stage_m;                // staged sorted tile lists
     DevBuf c_row_nnz_off;                   // per tile row, int64
+    DevBuf queue;                           // work-queue heads of the slab kernels
+    DevBuf row_cost, row_cost2, rows_sorted;   // slab rows, heaviest first
 };
 
 // Scratch of the tile builders, reused across calls.
@@ -230,8 +232,12 @@ struct CoreTimes {
     double sym_count_kernel_ms = 0, sym_fill_kernel_ms = 0, numeric_kernel_ms = 0;
     unsigned long long pairs = 0, heavy_rows = 0;
 };
+// lb_mode (enum rtt_lb_mode) assigns the rows of the slab kernels, which
+// hold one workgroup per row for many rows: static hands row i to
+// workgroup i mod grid; dynamic takes rows from a work queue in list order;
+// adaptive-tail takes them from the queue heaviest first (most pairs).
 int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *work,
-                Events *events, CoreTimes *times);
+                Events *events, CoreTimes *times, int lb_mode = RTT_LB_ADAPTIVE_TAIL);
 
 // ---- scans (scan.hip) -------------------------------------------------------
 // Exclusive sum of n int64 values into out[0..n] (out[n] = total).

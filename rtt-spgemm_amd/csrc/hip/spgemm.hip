@@ -28,6 +28,8 @@
 //    workgroup path with FP64 atomics in HBM.
 #include <climits>
 
+#include <hipcub/hipcub.hpp>
+
 #include "device.hpp"
 #include "internal.hpp"
 
@@ -701,6 +703,39 @@ __global__ __launch_bounds__(kThreads) void symbolic_write_kernel(
     }
 }
 
+// Work queue of the slab kernels: the workgroup's next position in the row
+// list, taken by one lane with a global atomic and shared through LDS.  The
+// head only grows, so every workgroup leaves once it passes the list's end.
+// The row bodies end with a barrier, so the slot is not rewritten while a
+// lane still reads it.
+__device__ __forceinline__ int next_row(int *queue)
+{
+    __shared__ int s_next;
+    if (threadIdx.x == 0)
+        s_next = atomicAdd(queue, 1);
+    __syncthreads();
+    return s_next;
+}
+
+// Cost of each listed row: the (A tile, B tile) pairs it visits.  One wave
+// per row.
+__global__ __launch_bounds__(kThreads) void row_pairs_kernel(const int *__restrict__ rows, int n,
+                                                             const int *__restrict__ a_trp,
+                                                             const int4 *__restrict__ a_brange,
+                                                             unsigned int *__restrict__ cost)
+{
+    const int i = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x) / kWave;
+    if (i >= n)
+        return;
+    const int row = rows[i];
+    unsigned long long sum = 0;
+    for (int t = a_trp[row] + lane_id(); t < a_trp[row + 1]; t += kWave)
+        sum += static_cast<unsigned int>(a_brange[t].z);
+    sum = wave_sum(sum);
+    if (lane_id() == 0)
+        cost[i] = static_cast<unsigned int>(sum < 0xffffffffull ? sum : 0xffffffffull);
+}
+
 // Rows whose tile set outgrows the LDS table: one workgroup per row, one
 // 64-bit mask per C tile column in an HBM slab, read back in column order.
 __device__ __forceinline__ unsigned long long load_coherent(const unsigned long long *p)
@@ -730,8 +765,9 @@ __device__ __forceinline__ int load_slab_chunk(const Operands &g, int c0, int n,
 
 template <bool FILL>
 __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
-    Operands g, const int *__restrict__ rows, int nrows, unsigned long long *__restrict__ slabs,
-    int64_t *__restrict__ row_tiles, int64_t *__restrict__ row_nnz, COut c)
+    Operands g, const int *__restrict__ rows, int nrows, int *__restrict__ queue,
+    int static_rows, unsigned long long *__restrict__ slabs, int64_t *__restrict__ row_tiles,
+    int64_t *__restrict__ row_nnz, COut c)
 {
     __shared__ uint32_t s_ab[kThreads];
     __shared__ int s_b0[kThreads];
@@ -740,7 +776,11 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
     __shared__ int s_lo, s_hi;
     unsigned long long *slab = slabs + static_cast<int64_t>(blockIdx.x) * g.b_tile_cols;
     const int tid = threadIdx.x;
-    for (int ri = blockIdx.x; ri < nrows; ri += gridDim.x) {
+    for (int step = 0;; ++step) {
+        const int ri = static_rows ? static_cast<int>(blockIdx.x) + step * static_cast<int>(gridDim.x)
+                                   : next_row(queue);
+        if (ri >= nrows)
+            break;
         const int row = rows[ri];
         const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
         if (tid == 0) {
@@ -1217,7 +1257,7 @@ constexpr int kSlabStage = 4096;   // A values a slab chunk stages in LDS
 
 __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
     Operands g, const int4 *__restrict__ recs, int64_t rec_n, const int *__restrict__ rows,
-    int nrows, int4 *__restrict__ slabs, COut c)
+    int nrows, int *__restrict__ queue, int static_rows, int4 *__restrict__ slabs, COut c)
 {
     __shared__ double s_acol[kSlabStage + 8];
     __shared__ int4 s_at[kThreads];   // {rows of column k (byte k), first value of column k (byte k), first value, unused}
@@ -1226,7 +1266,11 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
     __shared__ int s_tmp[kWavesPerBlock];
     int4 *slab = slabs + static_cast<int64_t>(blockIdx.x) * g.b_tile_cols;
     const int tid = threadIdx.x;
-    for (int ri = blockIdx.x; ri < nrows; ri += gridDim.x) {
+    for (int step = 0;; ++step) {
+        const int ri = static_rows ? static_cast<int>(blockIdx.x) + step * static_cast<int>(gridDim.x)
+                                   : next_row(queue);
+        if (ri >= nrows)
+            break;
         const int row = rows[ri];
         const int64_t t0 = c.tile_ptr[row];
         const int nt = static_cast<int>(c.tile_ptr[row + 1] - t0);
@@ -1420,8 +1464,33 @@ double event_ms(hipEvent_t a, hipEvent_t b)
 
 } // namespace
 
+// Copies a slab row list into w->rows_sorted, heaviest row (most pairs)
+// first, so that the work queue hands out the long rows before the short
+// ones.  Stream-ordered: the previous user of rows_sorted has been queued.
+static int order_rows_by_cost(const int *rows, int n, const Operands &g, SpgemmWork *w, hipStream_t s,
+                       const int **sorted)
+{
+    RTT_TRY(w->row_cost.ensure(static_cast<size_t>(n) * sizeof(unsigned int)));
+    RTT_TRY(w->row_cost2.ensure(static_cast<size_t>(n) * sizeof(unsigned int)));
+    RTT_TRY(w->rows_sorted.ensure(static_cast<size_t>(n) * sizeof(int)));
+    unsigned int *cost = w->row_cost.as<unsigned int>();
+    unsigned int *cost2 = w->row_cost2.as<unsigned int>();
+    int *out = w->rows_sorted.as<int>();
+    hipLaunchKernelGGL(row_pairs_kernel, dim3(blocks_for(n, kWavesPerBlock)), dim3(kThreads), 0,
+                       s, rows, n, g.a_trp, g.a_brange, cost);
+    RTT_HIP(hipGetLastError());
+    size_t bytes = 0;
+    RTT_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, bytes, cost, cost2, rows, out,
+                                                         n, 0, 32, s));
+    RTT_TRY(w->scan_tmp.ensure(bytes));
+    RTT_HIP(hipcub::DeviceRadixSort::SortPairsDescending(w->scan_tmp.ptr, bytes, cost, cost2,
+                                                         rows, out, n, 0, 32, s));
+    *sorted = out;
+    return RTT_OK;
+}
+
 int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Events *ev,
-                CoreTimes *times)
+                CoreTimes *times, int lb_mode)
 {
     RTT_TRY(require_ctx());
     RTT_TRY(ev->create());
@@ -1522,16 +1591,26 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     }
     RTT_HIP(hipEventRecord(ev->ev[1], s));
     int slab_blocks = 0;
+    const int *slab_rows = nullptr;
+    // Queue heads: [0] symbolic slab count, [1] symbolic slab fill, [2] numeric slab.
+    RTT_TRY(w->queue.ensure(4 * sizeof(int)));
+    int *queue = w->queue.as<int>();
+    const int static_rows = lb_mode == RTT_LB_STATIC ? 1 : 0;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(4), 0, s, queue, int64_t(4), 0);
     if (n_slab > 0) {
         slab_blocks = std::min(n_slab, 4 * num_cus);
+        slab_rows = w->l3_rows.as<int>();
+        if (lb_mode == RTT_LB_ADAPTIVE_TAIL)
+            RTT_TRY(order_rows_by_cost(w->l3_rows.as<int>(), n_slab, g, w, s, &slab_rows));
         RTT_TRY(w->slab.ensure(static_cast<size_t>(slab_blocks) * BC * sizeof(uint64_t)));
         hipLaunchKernelGGL(fill_u64_kernel,
                            dim3(blocks_for(static_cast<int64_t>(slab_blocks) * BC, kThreads)),
                            dim3(kThreads), 0, s, w->slab.as<unsigned long long>(),
                            static_cast<int64_t>(slab_blocks) * BC);
         hipLaunchKernelGGL((symbolic_slab_kernel<false>), dim3(slab_blocks), dim3(kThreads), 0, s,
-                           g, w->l3_rows.as<int>(), n_slab, w->slab.as<unsigned long long>(),
-                           w->row_tiles.as<int64_t>(), w->row_nnz.as<int64_t>(), none);
+                           g, slab_rows, n_slab, queue, static_rows,
+                           w->slab.as<unsigned long long>(), w->row_tiles.as<int64_t>(),
+                           w->row_nnz.as<int64_t>(), none);
         RTT_HIP(hipGetLastError());
     }
     RTT_TRY(exclusive_scan_i64(w->row_tiles.as<int64_t>(), c->tile_ptr.as<int64_t>(), R,
@@ -1594,8 +1673,8 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     RTT_HIP(hipEventRecord(ev->ev[3], s));
     if (n_slab > 0)
         hipLaunchKernelGGL((symbolic_slab_kernel<true>), dim3(slab_blocks), dim3(kThreads), 0, s,
-                           g, w->l3_rows.as<int>(), n_slab, w->slab.as<unsigned long long>(),
-                           nullptr, nullptr, co);
+                           g, slab_rows, n_slab, queue + 1, static_rows,
+                           w->slab.as<unsigned long long>(), nullptr, nullptr, co);
     hipLaunchKernelGGL(set_last_kernel, dim3(1), dim3(1), 0, s,
                        c->tile_nnz.as<int64_t>() + c->num_tiles,
                        w->c_row_nnz_off.as<int64_t>() + R);
@@ -1620,9 +1699,13 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
         RTT_TRY(tab.ensure(static_cast<size_t>(ents) * sizeof(int4)));
         hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(4 * ents, kThreads)), dim3(kThreads), 0,
                            s, tab.as<int>(), 4 * ents, -1);
+        // The symbolic fill pass above is queued before this sort reuses rows_sorted.
+        const int *num_rows = w->num_rows.as<int>();
+        if (lb_mode == RTT_LB_ADAPTIVE_TAIL)
+            RTT_TRY(order_rows_by_cost(w->num_rows.as<int>(), n_num_slab, g, w, s, &num_rows));
         hipLaunchKernelGGL(numeric_slab_kernel, dim3(blocks), dim3(kThreads), 0, s, g,
-                           b.records.as<int4>(), b.num_tiles, w->num_rows.as<int>(), n_num_slab,
-                           tab.as<int4>(), co);
+                           b.records.as<int4>(), b.num_tiles, num_rows, n_num_slab, queue + 2,
+                           static_rows, tab.as<int4>(), co);
         RTT_HIP(hipGetLastError());
     }
     RTT_HIP(hipEventRecord(ev->ev[7], s));
