@@ -207,6 +207,8 @@ def main():
     ap.add_argument("--lb-mode", default="adaptive-tail",
                     choices=["static", "dynamic", "adaptive-tail"],
                     help="row assignment of the slab kernels (the reference's --lb-mode)")
+    ap.add_argument("--layout", default="identity", choices=["identity", "bgrf"],
+                    help="A's offline layout: identity (--no-reorder) or the BGRF-v1 plan")
     ap.add_argument("--no-control", action="store_true",
                     help="skip the rocSPARSE SpGEMM control on the same workload")
     ap.add_argument("--control-only", action="store_true", help=argparse.SUPPRESS)
@@ -231,7 +233,7 @@ def main():
     control = None if args.no_control else rocsparse_control(args.workload)
     a, desc = make_matrix(rt, spec)
     rt.init(0)
-    pa = rt.prepare_identity_a(a)
+    pa = rt.prepare_bgrf_a(a) if args.layout == "bgrf" else rt.prepare_identity_a(a)
     nnz_cub = rt.nnz_cub(pa, aat)
     sched = rt.schedule(args.lb_mode, materialize_output=False)
     pb = None
@@ -307,6 +309,7 @@ def main():
         "config": {"workload": f"{args.workload}: C = A*A{'^T' if aat else ''}, {desc}, "
                                f"{a.m} rows, {a.nnz} nnz, nnzCub {nnz_cub}, nnzC {nnz_c}",
                    "precision": "plain FP64", "lb_mode": args.lb_mode,
+                   "layout": args.layout,
                    "pipeline_hbm_frac": round(b_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                               5)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),

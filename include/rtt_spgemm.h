@@ -155,6 +155,34 @@ int rtt_prepare_external_a(const rtt_smatrix *host,
                            const int *inner_new_to_old, int dense_threshold,
                            rtt_prepared_a **out, rtt_offline_a_stats *stats);
 
+/* BGRF-v1 offline reorder (build_dmma_reorder_plan, src/dmma_reorder.h:2651;
+ * algorithm in docs/BGRF_V1_PPOPP_ALGORITHM_AND_DATA_FORMAT_CN.md sections
+ * 3-6), the reference driver's default layout (src/main.cu:1863-1874).
+ * Host-side, deterministic.  Statistics of one plan: */
+typedef struct rtt_bgrf_stats {               /* DmmaReorderPlan counters */
+    long long tiles_identity;      /* occupied 8x4 A tiles, identity layout */
+    long long tiles_coarse;        /* ... under the coarse proposal */
+    long long tiles_after_coarse;  /* ... after the coarse accept/reject */
+    long long tiles_after_rows;    /* ... after the 32/8 row windows */
+    long long tiles_final;         /* ... after the 16/4 K windows */
+    long long row_fanout_before, row_fanout_after;
+    long long inner_fanout_before, inner_fanout_after;
+    int coarse_accepted;
+    int coarse_levels;             /* frontier rounds of the traversal */
+    int row_windows, row_windows_accepted;
+    int inner_windows, inner_windows_accepted;
+    double plan_ms;
+} rtt_bgrf_stats;
+
+/* Computes the row and inner permutations (new -> old, host->m and host->n
+ * entries) for rtt_prepare_external_a.  stats may be NULL. */
+int rtt_bgrf_plan(const rtt_smatrix *host, int *row_new_to_old, int *inner_new_to_old,
+                  rtt_bgrf_stats *stats);
+/* rtt_bgrf_plan followed by rtt_prepare_external_a (gpu_prepare_reordered_a,
+ * src/gpu_dmma_tiles.h:2334). */
+int rtt_prepare_bgrf_a(const rtt_smatrix *host, int dense_threshold, rtt_prepared_a **out,
+                       rtt_offline_a_stats *stats, rtt_bgrf_stats *bgrf);
+
 /* Parses one order file (exactly `expected` 0-based decimal tokens, new_id
  * -> old_id, no duplicates or out-of-range ids: src/dmma_reorder.h:2048-2120)
  * into out[expected]. */

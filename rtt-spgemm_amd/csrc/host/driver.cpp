@@ -51,7 +51,7 @@ void usage(const char *prog)
         "  --lb-mode static|dynamic|adaptive-tail (default: adaptive-tail)\n"
         "  --warmup-iterations N          warmups (default: 0)\n"
         "  --iterations N                 repeats (default: 5)\n"
-        "  --no-reorder                   identity layout\n"
+        "  --no-reorder                   identity layout (default: the BGRF-v1 plan)\n"
         "  --row-order FILE --inner-order FILE --reorder-name NAME\n"
         "  --reference-values             replace values by i %% 10 as the\n"
         "                                 reference driver does\n"
@@ -221,10 +221,21 @@ int main(int argc, char **argv)
         rc = rtt_prepare_external_a(&A, row_order.data(), inner_order.data(), o.dense_threshold,
                                     &pa, &ast);
         layout = "external";
-    } else {
-        // The learned BGRF layout is not part of this build; the default is
-        // the identity layout with the empty suffix trimmed (--no-reorder).
+    } else if (o.no_reorder) {
+        // Identity layout with the empty suffix trimmed (--no-reorder).
         rc = rtt_prepare_identity_a(&A, o.dense_threshold, &pa, &ast);
+    } else {
+        // The reference's default: the BGRF-v1 plan (src/main.cu:1863-1874).
+        rtt_bgrf_stats bs;
+        rc = rtt_prepare_bgrf_a(&A, o.dense_threshold, &pa, &ast, &bs);
+        layout = "bgrf-v1";
+        if (rc == RTT_OK)
+            std::printf("BGRF plan %.3f ms: A tiles identity %lld, coarse %lld (%s), rows %lld "
+                        "(%d/%d windows), final %lld (%d/%d K windows)\n",
+                        bs.plan_ms, bs.tiles_identity, bs.tiles_coarse,
+                        bs.coarse_accepted ? "accepted" : "rejected", bs.tiles_after_rows,
+                        bs.row_windows_accepted, bs.row_windows, bs.tiles_final,
+                        bs.inner_windows_accepted, bs.inner_windows);
     }
     if (check(rc, "prepare A"))
         return EXIT_FAILURE;

@@ -83,6 +83,17 @@ class BUpdateStats(C.Structure):
                 ("csr2tile_ms", C.c_double), ("csc_ms", C.c_double), ("path", C.c_int)]
 
 
+class BgrfStats(C.Structure):
+    _fields_ = [("tiles_identity", C.c_longlong), ("tiles_coarse", C.c_longlong),
+                ("tiles_after_coarse", C.c_longlong), ("tiles_after_rows", C.c_longlong),
+                ("tiles_final", C.c_longlong), ("row_fanout_before", C.c_longlong),
+                ("row_fanout_after", C.c_longlong), ("inner_fanout_before", C.c_longlong),
+                ("inner_fanout_after", C.c_longlong), ("coarse_accepted", C.c_int),
+                ("coarse_levels", C.c_int), ("row_windows", C.c_int),
+                ("row_windows_accepted", C.c_int), ("inner_windows", C.c_int),
+                ("inner_windows_accepted", C.c_int), ("plan_ms", C.c_double)]
+
+
 class BValuesDecision(C.Structure):
     _fields_ = [("clear_payload", C.c_int), ("skip_clear", C.c_int), ("fallback", C.c_int),
                 ("reason", C.c_int)]
@@ -142,6 +153,9 @@ def load() -> C.CDLL:
         "rtt_prepare_external_a": (C.c_int, [P(SMatrix), vp, vp, C.c_int, P(vp),
                                              P(OfflineAStats)]),
         "rtt_read_order_file": (C.c_int, [C.c_char_p, C.c_int, vp]),
+        "rtt_bgrf_plan": (C.c_int, [P(SMatrix), vp, vp, P(BgrfStats)]),
+        "rtt_prepare_bgrf_a": (C.c_int, [P(SMatrix), C.c_int, P(vp), P(OfflineAStats),
+                                         P(BgrfStats)]),
         "rtt_prepared_a_view": (C.c_int, [vp, P(DeviceTiles)]),
         "rtt_prepared_a_info": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_int)]),
         "rtt_prepared_a_row_new_to_old": (vp, [vp]),
@@ -426,6 +440,30 @@ def prepare_external_a(a: CsrMatrix, row_new_to_old, inner_new_to_old,
     _check(load().rtt_prepare_external_a(C.byref(s), _ptr(r), _ptr(i), dense_threshold,
                                          C.byref(h), C.byref(st)), "prepare_external_a")
     return PreparedA(h, st, a)
+
+
+def bgrf_plan(a: CsrMatrix):
+    """BGRF-v1 plan (build_dmma_reorder_plan, src/dmma_reorder.h:2651):
+    (row_new_to_old, inner_new_to_old, stats).  Host only."""
+    s = a.as_smatrix()
+    r = np.zeros(max(a.m, 1), np.int32)
+    i = np.zeros(max(a.n, 1), np.int32)
+    st = BgrfStats()
+    _check(load().rtt_bgrf_plan(C.byref(s), _ptr(r), _ptr(i), C.byref(st)), "bgrf_plan")
+    return r[:a.m], i[:a.n], {f: getattr(st, f) for f, _ in BgrfStats._fields_}
+
+
+def prepare_bgrf_a(a: CsrMatrix, dense_threshold: int = 24) -> PreparedA:
+    """A in the BGRF-v1 layout (the reference driver's default)."""
+    s = a.as_smatrix()
+    h = C.c_void_p()
+    st = OfflineAStats()
+    bs = BgrfStats()
+    _check(load().rtt_prepare_bgrf_a(C.byref(s), dense_threshold, C.byref(h), C.byref(st),
+                                     C.byref(bs)), "prepare_bgrf_a")
+    pa = PreparedA(h, st, a)
+    pa.bgrf = {f: getattr(bs, f) for f, _ in BgrfStats._fields_}
+    return pa
 
 
 class DynamicB:

@@ -299,6 +299,29 @@ def test_external_orders_restore_rows(gpu):
         assert err["fwd_max"] <= FWD_TOL, err
 
 
+@pytest.mark.parametrize("kind", ["random", "rmat12"])
+def test_bgrf_layout_restores_to_the_oracle_product(gpu, kind):
+    """The BGRF-v1 layout changes only where rows and K indices sit: after
+    tile2csr_restore_rows, C equals the oracle's in both modes, and A's tile
+    count is the plan's."""
+    a = (gpu.gen_random(400, 400, 7, 13) if kind == "random"
+         else gpu.gen_rmat(12, 16, 0.57, 0.19, 0.19, 8))
+    for aat in (0, 1):
+        pa = gpu.prepare_bgrf_a(a)
+        assert pa.tiles()["num_tiles"] == pa.bgrf["tiles_final"]
+        assert pa.bgrf["tiles_final"] <= pa.bgrf["tiles_identity"]
+        pb = gpu.rebuild_b(pa, aat)
+        tc, _ = gpu.tilespgemm(pa, pb)
+        ar, _, _ = pa.info()
+        perm = pa.row_new_to_old()
+        csr = gpu.tile2csr(tc) if perm is None else gpu.tile2csr_restore_rows(tc, perm, ar)
+        ref = oracle_product(a, aat)
+        assert np.array_equal(csr.rowptr.astype(np.int64), ref["rowptr"])
+        assert np.array_equal(csr.col, ref["col"])
+        err = oracle.value_errors(csr.val, ref["hi"], ref["lo"], ref["absprod"])
+        assert err["fwd_max"] <= FWD_TOL, err
+
+
 def test_design_doc_tile_example_through_external_order(gpu):
     # 16x8 A with entries (0,0)=a, (1,4)=b, (8,1)=c, (9,5)=d and the row
     # permutation 0->0, 8->1, 1->2, 9->3 (docs/...CN.md section 10).

@@ -222,3 +222,84 @@ def test_b_values_clear_policy_known_cases():
     assert d["reason_name"] == "structure-update-legacy-rebuild"
     d = rtt_amd.choose_b_values_clear(5, **ok)
     assert d["reason_name"] == "invalid-policy"
+
+
+def _permuted_tile_count(a, row_new_to_old, inner_new_to_old):
+    """Occupied 8x4 tiles of A laid out by the two maps, from the oracle's
+    tile builder on the explicitly permuted CSR."""
+    inner_old_to_new = np.empty(a.n, np.int64)
+    inner_old_to_new[inner_new_to_old] = np.arange(a.n)
+    rows, cols = [], []
+    for new, old in enumerate(row_new_to_old):
+        cs = a.col[a.rowptr[old]:a.rowptr[old + 1]]
+        rows += [new] * len(cs)
+        cols += list(inner_old_to_new[cs])
+    order = np.lexsort((np.asarray(cols), np.asarray(rows)))
+    r = np.asarray(rows, np.int64)[order]
+    c = np.asarray(cols, np.int32)[order]
+    rp = np.zeros(a.m + 1, np.int64)
+    np.add.at(rp, r + 1, 1)
+    t = oracle.hybrid_tiles(a.m, a.n, np.cumsum(rp).astype(np.int32), c,
+                            np.ones(len(c)), 8, 4, False, 24)
+    return t["num_tiles"]
+
+
+@pytest.mark.parametrize("kind", ["rmat12", "stencil8", "random"])
+def test_bgrf_plan_bijections_monotone_and_exact(kind):
+    """BGRF-v1: both maps are bijections, the plan is deterministic, no
+    stage raises the exact tile count, and the reported tile counts equal
+    the oracle's tile builder on the permuted matrix."""
+    a = {"rmat12": lambda: rtt_amd.gen_rmat(12, 16, 0.57, 0.19, 0.19, 5),
+         "stencil8": lambda: rtt_amd.gen_stencil27(8, 0.5, 1.5, 1),
+         "random": lambda: rtt_amd.gen_random(700, 500, 6, 3)}[kind]()
+    r, i, st = rtt_amd.bgrf_plan(a)
+    assert np.array_equal(np.sort(r), np.arange(a.m))
+    assert np.array_equal(np.sort(i), np.arange(a.n))
+    r2, i2, _ = rtt_amd.bgrf_plan(a)
+    assert np.array_equal(r, r2) and np.array_equal(i, i2)
+    assert st["tiles_after_coarse"] <= st["tiles_identity"]
+    assert st["tiles_after_rows"] <= st["tiles_after_coarse"]
+    assert st["tiles_final"] <= st["tiles_after_rows"]
+    assert st["tiles_after_coarse"] == (st["tiles_coarse"] if st["coarse_accepted"]
+                                        else st["tiles_identity"])
+    assert st["tiles_identity"] == _permuted_tile_count(a, np.arange(a.m), np.arange(a.n))
+    assert st["tiles_final"] == _permuted_tile_count(a, r, i)
+
+
+def test_bgrf_recovers_a_scrambled_band():
+    """A tridiagonal matrix under a random symmetric permutation: the
+    reversed Cuthill-McKee traversal of its bipartite graph recovers a banded
+    order, so the coarse stage is accepted and the tile count falls to the
+    band's: 8 consecutive rows of a tridiagonal matrix touch 10 consecutive
+    columns, at most four 4-wide K tiles."""
+    n = 2000
+    rng = np.random.default_rng(4)
+    p = rng.permutation(n)
+    rows, cols = [], []
+    for i in range(n):
+        for j in (i - 1, i, i + 1):
+            if 0 <= j < n:
+                rows.append(p[i])
+                cols.append(p[j])
+    order = np.lexsort((cols, rows))
+    r = np.asarray(rows)[order]
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, r + 1, 1)
+    a = rtt_amd.CsrMatrix(n, n, np.cumsum(rp).astype(np.int32),
+                          np.asarray(cols, np.int32)[order], np.ones(len(rows)))
+    _, _, st = rtt_amd.bgrf_plan(a)
+    assert st["coarse_accepted"] == 1
+    assert st["tiles_final"] < 0.5 * st["tiles_identity"]
+    assert st["tiles_final"] <= 4 * ((n + 7) // 8)
+
+
+def test_bgrf_plan_edge_cases():
+    empty = rtt_amd.CsrMatrix(5, 7, np.zeros(6, np.int32), np.zeros(0, np.int32),
+                              np.zeros(0))
+    r, i, st = rtt_amd.bgrf_plan(empty)
+    assert sorted(r) == list(range(5)) and sorted(i) == list(range(7))
+    assert st["tiles_final"] == 0
+    bad = rtt_amd.CsrMatrix(2, 2, np.array([0, 1, 2], np.int32), np.array([0, 5], np.int32),
+                            np.ones(2))
+    with pytest.raises(rtt_amd.RttError):
+        rtt_amd.bgrf_plan(bad)
