@@ -169,9 +169,10 @@ typedef struct rtt_bgrf_stats {               /* DmmaReorderPlan counters */
     long long inner_fanout_before, inner_fanout_after;
     int coarse_accepted;
     int coarse_levels;             /* frontier rounds of the traversal */
+    int coarse_components;         /* traversals started from a seed */
     int row_windows, row_windows_accepted;
     int inner_windows, inner_windows_accepted;
-    double plan_ms;
+    double coarse_ms, fine_ms, plan_ms;
 } rtt_bgrf_stats;
 
 /* Computes the row and inner permutations (new -> old, host->m and host->n
@@ -182,6 +183,20 @@ int rtt_bgrf_plan(const rtt_smatrix *host, int *row_new_to_old, int *inner_new_t
  * src/gpu_dmma_tiles.h:2334). */
 int rtt_prepare_bgrf_a(const rtt_smatrix *host, int dense_threshold, rtt_prepared_a **out,
                        rtt_offline_a_stats *stats, rtt_bgrf_stats *bgrf);
+
+/* On-disk diagnostics of a layout (new -> old maps; NULL means identity and
+ * stats NULL the identity baseline), in the reference's formats
+ * (docs/BGRF_V1_PPOPP_ALGORITHM_AND_DATA_FORMAT_CN.md section 9.12):
+ * dump_dmma_reorder_plan (src/dmma_reorder.h:3086-3209) writes
+ * <prefix>_permutations.csv, <prefix>_unified_stats.csv and
+ * <prefix>_A_reordered.mtx; dump_reorder_heatmap (src/main.cu:1114-1246)
+ * writes <prefix>_heatmap.csv (bins x bins entry counts before and after)
+ * and <prefix>_heatmap_meta.csv. */
+int rtt_dump_reorder(const char *prefix, const rtt_smatrix *host, const int *row_new_to_old,
+                     const int *inner_new_to_old, const rtt_bgrf_stats *stats);
+int rtt_dump_reorder_heatmap(const char *prefix, const rtt_smatrix *host,
+                             const int *row_new_to_old, const int *inner_new_to_old, int bins,
+                             const rtt_bgrf_stats *stats);
 
 /* Parses one order file (exactly `expected` 0-based decimal tokens, new_id
  * -> old_id, no duplicates or out-of-range ids: src/dmma_reorder.h:2048-2120)

@@ -149,7 +149,7 @@ std::vector<int> invert(const std::vector<int> &perm)
 // Vertices 0..m-1 are rows, m..m+k-1 are K indices.  Returns the number of
 // frontier rounds processed.
 int coarse_proposal(const Pattern &p, std::vector<int> *row_new_to_old,
-                    std::vector<int> *inner_new_to_old)
+                    std::vector<int> *inner_new_to_old, int *components)
 {
     const int m = p.rows, k = p.cols;
     const int64_t nv = static_cast<int64_t>(m) + k;
@@ -203,6 +203,7 @@ int coarse_proposal(const Pattern &p, std::vector<int> *row_new_to_old,
         if (seed_pos == seeds.size())
             break;
         const int s = seeds[seed_pos];
+        ++*components;
         visited[s] = 1;
         order.push_back(s);
         frontier.assign(1, s);
@@ -503,7 +504,7 @@ int rtt_bgrf_plan(const rtt_smatrix *a, int *row_new_to_old, int *inner_new_to_o
 
     // I: coarse proposal, kept only when it lowers the exact tile count.
     std::vector<int> crow, cinner;
-    st.coarse_levels = coarse_proposal(p, &crow, &cinner);
+    st.coarse_levels = coarse_proposal(p, &crow, &cinner, &st.coarse_components);
     st.tiles_coarse = count_tiles(p, crow, invert(cinner));
     if (st.tiles_coarse < st.tiles_identity) {
         rows.swap(crow);
@@ -511,6 +512,8 @@ int rtt_bgrf_plan(const rtt_smatrix *a, int *row_new_to_old, int *inner_new_to_o
         st.coarse_accepted = 1;
     }
     st.tiles_after_coarse = st.coarse_accepted ? st.tiles_coarse : st.tiles_identity;
+    const auto t_fine = std::chrono::steady_clock::now();
+    st.coarse_ms = std::chrono::duration<double, std::milli>(t_fine - t0).count();
 
     // II: rows in windows of 32, groups of 8, over K panels of the inner order.
     {
@@ -535,8 +538,9 @@ int rtt_bgrf_plan(const rtt_smatrix *a, int *row_new_to_old, int *inner_new_to_o
     st.tiles_final = count_tiles(p, rows, invert(inner));
     std::copy(rows.begin(), rows.end(), row_new_to_old);
     std::copy(inner.begin(), inner.end(), inner_new_to_old);
-    st.plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
-                     .count();
+    const auto t_end = std::chrono::steady_clock::now();
+    st.fine_ms = std::chrono::duration<double, std::milli>(t_end - t_fine).count();
+    st.plan_ms = std::chrono::duration<double, std::milli>(t_end - t0).count();
     if (stats)
         *stats = st;
     return RTT_OK;

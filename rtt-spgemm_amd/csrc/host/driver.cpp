@@ -33,6 +33,9 @@ struct Options {
     const char *b_file = nullptr;      // --b: general A*B
     int b_update = 0;                  // 0 none, 1 values, 2 permutation, 3 structure
     int b_clear_policy = RTT_B_VALUES_SAFE_AUTO;
+    const char *dump_prefix = nullptr;    // --dump-reorder-prefix
+    const char *heatmap_prefix = nullptr; // --dump-reorder-heatmap
+    int heatmap_bins = 256;
 };
 
 const char *const kBUpdateNames[] = {"none", "values", "permutation", "structure"};
@@ -53,6 +56,7 @@ void usage(const char *prog)
         "  --iterations N                 repeats (default: 5)\n"
         "  --no-reorder                   identity layout (default: the BGRF-v1 plan)\n"
         "  --row-order FILE --inner-order FILE --reorder-name NAME\n"
+        "  --dump-reorder-prefix P | --dump-reorder-heatmap P [--heatmap-bins N (16..1024)]\n"
         "  --reference-values             replace values by i %% 10 as the\n"
         "                                 reference driver does\n"
         "  --prepare-only\n",
@@ -111,6 +115,17 @@ bool parse(int argc, char **argv, Options *o)
             o->reference_values = true;
         } else if (!std::strcmp(a, "--prepare-only")) {
             o->prepare_only = true;
+        } else if (!std::strcmp(a, "--dump-reorder-prefix")) {
+            if (!(o->dump_prefix = next()))
+                return false;
+        } else if (!std::strcmp(a, "--dump-reorder-heatmap")) {
+            if (!(o->heatmap_prefix = next()))
+                return false;
+        } else if (!std::strcmp(a, "--heatmap-bins")) {
+            const char *v = next();
+            if (!v)
+                return false;
+            o->heatmap_bins = std::atoi(v);
         } else if (!std::strcmp(a, "--b")) {
             if (!(o->b_file = next()))
                 return false;
@@ -142,7 +157,8 @@ bool parse(int argc, char **argv, Options *o)
     const bool any_ext = o->row_order || o->inner_order || o->reorder_name;
     const bool all_ext = o->row_order && o->inner_order && o->reorder_name;
     return o->a_file && (o->aat == 0 || o->aat == 1) && o->warmup >= 0 && o->iterations > 0 &&
-           (!any_ext || all_ext) && !(o->no_reorder && any_ext);
+           (!any_ext || all_ext) && !(o->no_reorder && any_ext) && o->heatmap_bins >= 16 &&
+           o->heatmap_bins <= 1024;
 }
 
 double median_of(const std::vector<double> &v, size_t *selected)
@@ -211,6 +227,8 @@ int main(int argc, char **argv)
     rtt_offline_a_stats ast;
     std::vector<int> row_order, inner_order;
     const char *layout = "identity";
+    rtt_bgrf_stats bgrf;
+    bool have_bgrf = false;
     int rc;
     if (o.row_order) {
         row_order.resize(A.m);
@@ -226,19 +244,41 @@ int main(int argc, char **argv)
         rc = rtt_prepare_identity_a(&A, o.dense_threshold, &pa, &ast);
     } else {
         // The reference's default: the BGRF-v1 plan (src/main.cu:1863-1874).
-        rtt_bgrf_stats bs;
-        rc = rtt_prepare_bgrf_a(&A, o.dense_threshold, &pa, &ast, &bs);
-        layout = "bgrf-v1";
+        row_order.resize(A.m > 0 ? A.m : 1);
+        inner_order.resize(A.n > 0 ? A.n : 1);
+        rc = rtt_bgrf_plan(&A, row_order.data(), inner_order.data(), &bgrf);
         if (rc == RTT_OK)
+            rc = rtt_prepare_external_a(&A, row_order.data(), inner_order.data(),
+                                        o.dense_threshold, &pa, &ast);
+        layout = "bgrf-v1";
+        have_bgrf = rc == RTT_OK;
+        if (have_bgrf)
             std::printf("BGRF plan %.3f ms: A tiles identity %lld, coarse %lld (%s), rows %lld "
                         "(%d/%d windows), final %lld (%d/%d K windows)\n",
-                        bs.plan_ms, bs.tiles_identity, bs.tiles_coarse,
-                        bs.coarse_accepted ? "accepted" : "rejected", bs.tiles_after_rows,
-                        bs.row_windows_accepted, bs.row_windows, bs.tiles_final,
-                        bs.inner_windows_accepted, bs.inner_windows);
+                        bgrf.plan_ms, bgrf.tiles_identity, bgrf.tiles_coarse,
+                        bgrf.coarse_accepted ? "accepted" : "rejected", bgrf.tiles_after_rows,
+                        bgrf.row_windows_accepted, bgrf.row_windows, bgrf.tiles_final,
+                        bgrf.inner_windows_accepted, bgrf.inner_windows);
     }
     if (check(rc, "prepare A"))
         return EXIT_FAILURE;
+    {
+        // Reorder diagnostics (src/main.cu:1930-1960): the layout in use.
+        const int *rmap = o.no_reorder ? nullptr : row_order.data();
+        const int *imap = o.no_reorder ? nullptr : inner_order.data();
+        const rtt_bgrf_stats *bs = have_bgrf ? &bgrf : nullptr;
+        if (o.dump_prefix &&
+            check(rtt_dump_reorder(o.dump_prefix, &A, rmap, imap, bs), "dump reorder"))
+            return EXIT_FAILURE;
+        if (o.heatmap_prefix &&
+            check(rtt_dump_reorder_heatmap(o.heatmap_prefix, &A, rmap, imap, o.heatmap_bins, bs),
+                  "dump heatmap"))
+            return EXIT_FAILURE;
+        if (o.dump_prefix)
+            std::printf("REORDER_DUMP prefix=%s\n", o.dump_prefix);
+        if (o.heatmap_prefix)
+            std::printf("REORDER_HEATMAP prefix=%s bins=%d\n", o.heatmap_prefix, o.heatmap_bins);
+    }
     std::printf("A prepared (%s): upload %.3f ms, validate %.3f ms, row sort %.3f ms, "
                 "tiles %.3f ms\n",
                 layout, ast.h2d_ms, ast.validation_ms, ast.canonicalize_ms, ast.tile_build_ms);

@@ -89,9 +89,10 @@ class BgrfStats(C.Structure):
                 ("tiles_final", C.c_longlong), ("row_fanout_before", C.c_longlong),
                 ("row_fanout_after", C.c_longlong), ("inner_fanout_before", C.c_longlong),
                 ("inner_fanout_after", C.c_longlong), ("coarse_accepted", C.c_int),
-                ("coarse_levels", C.c_int), ("row_windows", C.c_int),
-                ("row_windows_accepted", C.c_int), ("inner_windows", C.c_int),
-                ("inner_windows_accepted", C.c_int), ("plan_ms", C.c_double)]
+                ("coarse_levels", C.c_int), ("coarse_components", C.c_int),
+                ("row_windows", C.c_int), ("row_windows_accepted", C.c_int),
+                ("inner_windows", C.c_int), ("inner_windows_accepted", C.c_int),
+                ("coarse_ms", C.c_double), ("fine_ms", C.c_double), ("plan_ms", C.c_double)]
 
 
 class BValuesDecision(C.Structure):
@@ -154,6 +155,9 @@ def load() -> C.CDLL:
                                              P(OfflineAStats)]),
         "rtt_read_order_file": (C.c_int, [C.c_char_p, C.c_int, vp]),
         "rtt_bgrf_plan": (C.c_int, [P(SMatrix), vp, vp, P(BgrfStats)]),
+        "rtt_dump_reorder": (C.c_int, [C.c_char_p, P(SMatrix), vp, vp, P(BgrfStats)]),
+        "rtt_dump_reorder_heatmap": (C.c_int, [C.c_char_p, P(SMatrix), vp, vp, C.c_int,
+                                               P(BgrfStats)]),
         "rtt_prepare_bgrf_a": (C.c_int, [P(SMatrix), C.c_int, P(vp), P(OfflineAStats),
                                          P(BgrfStats)]),
         "rtt_prepared_a_view": (C.c_int, [vp, P(DeviceTiles)]),
@@ -451,6 +455,39 @@ def bgrf_plan(a: CsrMatrix):
     st = BgrfStats()
     _check(load().rtt_bgrf_plan(C.byref(s), _ptr(r), _ptr(i), C.byref(st)), "bgrf_plan")
     return r[:a.m], i[:a.n], {f: getattr(st, f) for f, _ in BgrfStats._fields_}
+
+
+def _stats_struct(stats: dict | None):
+    if stats is None:
+        return None
+    st = BgrfStats()
+    for k, v in stats.items():
+        setattr(st, k, v)
+    return C.byref(st)
+
+
+def dump_reorder(prefix: str, a: CsrMatrix, row_new_to_old=None, inner_new_to_old=None,
+                 stats: dict | None = None) -> None:
+    """<prefix>_permutations.csv, _unified_stats.csv, _A_reordered.mtx
+    (dump_dmma_reorder_plan, src/dmma_reorder.h:3086)."""
+    s = a.as_smatrix()
+    r = None if row_new_to_old is None else np.ascontiguousarray(row_new_to_old, np.int32)
+    i = None if inner_new_to_old is None else np.ascontiguousarray(inner_new_to_old, np.int32)
+    _check(load().rtt_dump_reorder(prefix.encode(), C.byref(s), None if r is None else _ptr(r),
+                                   None if i is None else _ptr(i), _stats_struct(stats)),
+           "dump_reorder")
+
+
+def dump_reorder_heatmap(prefix: str, a: CsrMatrix, row_new_to_old=None, inner_new_to_old=None,
+                         bins: int = 256, stats: dict | None = None) -> None:
+    """<prefix>_heatmap.csv and _heatmap_meta.csv (src/main.cu:1114-1246)."""
+    s = a.as_smatrix()
+    r = None if row_new_to_old is None else np.ascontiguousarray(row_new_to_old, np.int32)
+    i = None if inner_new_to_old is None else np.ascontiguousarray(inner_new_to_old, np.int32)
+    _check(load().rtt_dump_reorder_heatmap(prefix.encode(), C.byref(s),
+                                           None if r is None else _ptr(r),
+                                           None if i is None else _ptr(i), int(bins),
+                                           _stats_struct(stats)), "dump_reorder_heatmap")
 
 
 def prepare_bgrf_a(a: CsrMatrix, dense_threshold: int = 24) -> PreparedA:

@@ -67,3 +67,21 @@ def test_driver_general_ab_values_update(tmp_path):
     bad = subprocess.run([rtt_amd.DRIVER_PATH, "--b-update", "sideways", pa],
                          capture_output=True, text=True, timeout=60)
     assert bad.returncode != 0 and "Usage" in bad.stdout
+
+
+def test_driver_reorder_dumps(tmp_path):
+    a = rtt_amd.gen_rmat(9, 8, 0.57, 0.19, 0.19, 4)
+    path = str(tmp_path / "a.mtx")
+    rtt_amd.write_mtx(path, a)
+    prefix = str(tmp_path / "P")
+    out = run_driver(["--prepare-only", "--dump-reorder-prefix", prefix,
+                      "--dump-reorder-heatmap", prefix, "--heatmap-bins", "32", path])
+    assert "BGRF plan" in out and f"REORDER_DUMP prefix={prefix}" in out
+    lines = open(prefix + "_permutations.csv").read().splitlines()
+    assert len(lines) == 1 + a.m + a.n
+    stats = dict(l.split(",", 1) for l in open(prefix + "_unified_stats.csv").read().split()[1:])
+    assert stats["algorithm"] == "bgrf-v1"
+    grid = np.loadtxt(prefix + "_heatmap.csv", delimiter=",", skiprows=1, dtype=np.int64)
+    assert grid.shape == (32 * 32, 4) and grid[:, 3].sum() == a.nnz
+    r, i, st = rtt_amd.bgrf_plan(a)
+    assert int(stats["num_tiles"]) == st["tiles_final"]

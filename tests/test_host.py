@@ -303,3 +303,54 @@ def test_bgrf_plan_edge_cases():
                             np.ones(2))
     with pytest.raises(rtt_amd.RttError):
         rtt_amd.bgrf_plan(bad)
+
+
+def _read_kv(path):
+    with open(path) as f:
+        rows = [l.strip().split(",", 1) for l in f][1:]
+    return {k: v for k, v in rows}
+
+
+def test_reorder_dumps_match_the_plan(tmp_path):
+    """The on-disk formats of the reference's reorder diagnostics: the
+    permutation CSV (header, old->new, and the inverse check column equal to
+    old_id), the key/value stats, the pattern .mtx of the reordered A (every
+    entry, 1-based), and the heatmap grid (exactly bins^2 cells, both sides
+    summing to the entry count)."""
+    a = rtt_amd.gen_rmat(10, 8, 0.57, 0.19, 0.19, 3)
+    r, i, st = rtt_amd.bgrf_plan(a)
+    prefix = str(tmp_path / "P")
+    rtt_amd.dump_reorder(prefix, a, r, i, st)
+    lines = open(prefix + "_permutations.csv").read().splitlines()
+    assert lines[0] == "axis,old_id,new_id,new_to_old_at_new_id"
+    body = [l.split(",") for l in lines[1:]]
+    assert len(body) == a.m + a.n
+    rows = [b for b in body if b[0] == "row"]
+    inner = [b for b in body if b[0] == "inner"]
+    assert all(b[1] == b[3] for b in body)
+    assert [int(b[2]) for b in rows] == list(np.argsort(r))
+    assert [int(b[2]) for b in inner] == list(np.argsort(i))
+    kv = _read_kv(prefix + "_unified_stats.csv")
+    assert kv["algorithm"] == "bgrf-v1"
+    assert int(kv["num_tiles"]) == st["tiles_final"]
+    assert int(kv["coarse_candidate_accepted"]) == st["coarse_accepted"]
+    assert int(kv["sparse_tiles"]) + int(kv["dense_tiles"]) == st["tiles_final"]
+    mtx = open(prefix + "_A_reordered.mtx").read().splitlines()
+    assert mtx[0] == "%%MatrixMarket matrix coordinate pattern general"
+    assert mtx[2] == f"{a.m} {a.n} {a.nnz}"
+    pairs = sorted(tuple(int(x) for x in l.split()) for l in mtx[3:])
+    r_o2n, i_o2n = np.argsort(r), np.argsort(i)
+    want = sorted((int(r_o2n[row]) + 1, int(i_o2n[c]) + 1)
+                  for row in range(a.m) for c in a.col[a.rowptr[row]:a.rowptr[row + 1]])
+    assert pairs == want
+    rtt_amd.dump_reorder_heatmap(prefix, a, r, i, 16, st)
+    grid = np.loadtxt(prefix + "_heatmap.csv", delimiter=",", skiprows=1, dtype=np.int64)
+    assert grid.shape == (256, 4)
+    assert grid[:, 2].sum() == a.nnz and grid[:, 3].sum() == a.nnz
+    meta = _read_kv(prefix + "_heatmap_meta.csv")
+    assert int(meta["bins"]) == 16 and int(meta["nnz"]) == a.nnz
+    # The identity baseline (no maps, no stats) and a non-bijective map.
+    rtt_amd.dump_reorder(prefix + "_id", a)
+    assert _read_kv(prefix + "_id_unified_stats.csv")["algorithm"] == "identity_baseline"
+    with pytest.raises(rtt_amd.RttError):
+        rtt_amd.dump_reorder(prefix, a, np.zeros(a.m, np.int32), i, st)
