@@ -156,6 +156,7 @@ struct SpgemmWork {
     DevBuf c_row_nnz_off;                   // per tile row, int64
     DevBuf queue;                           // work-queue heads of the slab kernels
     DevBuf row_cost, row_cost2, rows_sorted;   // slab rows, heaviest first
+    DevBuf trace_start, trace_end, trace_cu;   // numeric timeline samples
 };
 
 // Scratch of the tile builders, reused across calls.

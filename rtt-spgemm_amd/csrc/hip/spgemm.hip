@@ -27,6 +27,9 @@
 //    streamed to HBM.  Rows that exceed the LDS budgets fall back to a
 //    workgroup path with FP64 atomics in HBM.
 #include <climits>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
 
 #include <hipcub/hipcub.hpp>
 
@@ -926,10 +929,38 @@ __device__ __forceinline__ uint32_t a_column_runs(uint32_t ab)
     return out;
 }
 
-template <class Cfg>
+// Sampled start/end times of the numeric kernel's waves (the timeline
+// trace): workgroups b with b mod 2^shift == phase record, per wave, the
+// compute unit and the device's constant-rate wall clock at entry and exit.
+// Null pointers switch it off.
+struct Timeline {
+    unsigned long long *start = nullptr, *end = nullptr;
+    unsigned int *cu = nullptr;
+    unsigned int shift = 0, phase = 0;
+};
+
+__device__ __forceinline__ void trace_mark(const Timeline &tl, bool at_end)
+{
+    if (!tl.start || lane_id() != 0)
+        return;
+    const unsigned int b = blockIdx.x;
+    if ((b & ((1u << tl.shift) - 1u)) != tl.phase)
+        return;
+    const size_t slot = static_cast<size_t>((b - tl.phase) >> tl.shift) * kWavesPerBlock +
+                        threadIdx.x / kWave;
+    const unsigned long long t = static_cast<unsigned long long>(wall_clock64());
+    if (at_end) {
+        tl.end[slot] = t;
+    } else {
+        tl.cu[slot] = __smid();
+        tl.start[slot] = t;
+    }
+}
+
+template <class Cfg, bool TRACE = false>
 __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     Operands g, const int4 *__restrict__ recs, int64_t rec_n, const int *__restrict__ rows,
-    const int *__restrict__ row_level, Stage st, COut c)
+    const int *__restrict__ row_level, Stage st, COut c, Timeline tl)
 {
     __shared__ double s_acc[Cfg::kAcc];
     __shared__ double s_acol[Cfg::kAStage + kColRegs];
@@ -946,6 +977,8 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     __shared__ int s_dual[2 * kWavesPerBlock];
 
     const int tid = threadIdx.x;
+    if constexpr (TRACE)
+        trace_mark(tl, false);
     const int row = rows ? rows[blockIdx.x] : xcd_block(blockIdx.x, gridDim.x);
     // The row's inputs come in two rounds of independent loads: its level,
     // C's row extents, A's tile and payload extents (the payload base per
@@ -960,8 +993,11 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     const int a_base = g.a_row_off[row];
     const int a_len = g.a_row_off[row + 1] - a_base;
     if ((level != kLevelLds) | (nt > Cfg::kTiles) | (nnz_row > Cfg::kAcc) |
-        (a_len > Cfg::kAStage))
+        (a_len > Cfg::kAStage)) {
+        if constexpr (TRACE)
+            trace_mark(tl, true);
         return;
+    }
     const int64_t sb = static_cast<int64_t>(row) * st.stride;
     int j0 = 0;
     uint64_t m0 = 0;
@@ -1243,6 +1279,8 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
             }
         }
     }
+    if constexpr (TRACE)
+        trace_mark(tl, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -1464,6 +1502,59 @@ double event_ms(hipEvent_t a, hipEvent_t b)
 
 } // namespace
 
+// Timeline trace of the main numeric launch, switched on by the reference's
+// environment variables: DMMA_TRACE_FILE (output path), DMMA_TRACE_SHIFT
+// (sample every 2^shift-th workgroup, default 4, at most 10) and
+// DMMA_TRACE_PHASE (which one, default min(7, 2^shift - 1)).
+static unsigned int trace_env(const char *name, unsigned int fallback, unsigned int maximum)
+{
+    const char *text = std::getenv(name);
+    if (!text || !*text)
+        return fallback;
+    char *end = nullptr;
+    const unsigned long v = std::strtoul(text, &end, 10);
+    return (end == text || *end || v > maximum) ? fallback : static_cast<unsigned int>(v);
+}
+
+// The reference's DMTL0001 layout (dmma_write_timeline_trace,
+// src/dmma_spgemm.h:95-164): magic, version, compute units, waves per
+// workgroup, shift, phase, output tiles, workgroups, slots, then the slot
+// start times, end times (ns) and compute unit ids.
+static int write_timeline(const char *path, const Timeline &tl, size_t slots, uint64_t blocks,
+                          uint64_t output_tiles, int num_cus)
+{
+    std::vector<unsigned long long> start(slots), end(slots);
+    std::vector<uint32_t> cu(slots);
+    RTT_HIP(hipMemcpy(start.data(), tl.start, slots * sizeof(start[0]), hipMemcpyDeviceToHost));
+    RTT_HIP(hipMemcpy(end.data(), tl.end, slots * sizeof(end[0]), hipMemcpyDeviceToHost));
+    RTT_HIP(hipMemcpy(cu.data(), tl.cu, slots * sizeof(cu[0]), hipMemcpyDeviceToHost));
+    int rate_khz = 0;
+    RTT_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx().device));
+    const double ns_per_tick = rate_khz > 0 ? 1e6 / rate_khz : 1.0;
+    for (size_t i = 0; i < slots; ++i) {
+        start[i] = static_cast<unsigned long long>(static_cast<double>(start[i]) * ns_per_tick);
+        end[i] = static_cast<unsigned long long>(static_cast<double>(end[i]) * ns_per_tick);
+    }
+    FILE *f = std::fopen(path, "wb");
+    if (!f)
+        return fail(RTT_ERR_INVALID, "cannot open timeline output %s", path);
+    const char magic[8] = {'D', 'M', 'T', 'L', '0', '0', '0', '1'};
+    const uint32_t head[5] = {1u, static_cast<uint32_t>(num_cus),
+                              static_cast<uint32_t>(kWavesPerBlock), tl.shift, tl.phase};
+    const uint64_t tail[3] = {output_tiles, blocks, static_cast<uint64_t>(slots)};
+    bool ok = std::fwrite(magic, sizeof(magic), 1, f) == 1 &&
+              std::fwrite(head, sizeof(head), 1, f) == 1 &&
+              std::fwrite(tail, sizeof(tail), 1, f) == 1;
+    if (slots)
+        ok = ok && std::fwrite(start.data(), sizeof(start[0]), slots, f) == slots &&
+             std::fwrite(end.data(), sizeof(end[0]), slots, f) == slots &&
+             std::fwrite(cu.data(), sizeof(cu[0]), slots, f) == slots;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok)
+        return fail(RTT_ERR_INVALID, "write of timeline %s failed", path);
+    return RTT_OK;
+}
+
 // Copies a slab row list into w->rows_sorted, heaviest row (most pairs)
 // first, so that the work queue hands out the long rows before the short
 // ones.  Stream-ordered: the previous user of rows_sorted has been queued.
@@ -1681,17 +1772,50 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     RTT_HIP(hipGetLastError());
     RTT_HIP(hipEventRecord(ev->ev[6], s));
     // ---- numeric ----------------------------------------------------------
+    Timeline tl;
+    size_t trace_slots = 0;
+    const char *trace_path = std::getenv("DMMA_TRACE_FILE");
+    if (trace_path && *trace_path && R > 0 && c->nnz > 0) {
+        tl.shift = trace_env("DMMA_TRACE_SHIFT", 4, 10);
+        const unsigned int stride = 1u << tl.shift;
+        tl.phase = trace_env("DMMA_TRACE_PHASE", std::min(7u, stride - 1u), stride - 1u);
+        const size_t sampled = static_cast<unsigned int>(R) <= tl.phase
+                                   ? 0
+                                   : (static_cast<size_t>(R) - 1 - tl.phase) / stride + 1;
+        trace_slots = sampled * kWavesPerBlock;
+        if (trace_slots) {
+            RTT_TRY(w->trace_start.ensure(trace_slots * sizeof(unsigned long long)));
+            RTT_TRY(w->trace_end.ensure(trace_slots * sizeof(unsigned long long)));
+            RTT_TRY(w->trace_cu.ensure(trace_slots * sizeof(unsigned int)));
+            tl.start = w->trace_start.as<unsigned long long>();
+            tl.end = w->trace_end.as<unsigned long long>();
+            tl.cu = w->trace_cu.as<unsigned int>();
+            hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(trace_slots, kThreads)),
+                               dim3(kThreads), 0, s, tl.start, static_cast<int64_t>(trace_slots));
+            hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(trace_slots, kThreads)),
+                               dim3(kThreads), 0, s, tl.end, static_cast<int64_t>(trace_slots));
+            hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(trace_slots, kThreads)),
+                               dim3(kThreads), 0, s, reinterpret_cast<int *>(tl.cu),
+                               static_cast<int64_t>(trace_slots), 0);
+        }
+    }
     RTT_HIP(hipEventRecord(ev->ev[4], s));
-    if (R > 0 && c->nnz > 0)
-        hipLaunchKernelGGL((numeric_row_kernel<SmallRow>), dim3(R), dim3(kThreads), 0, s, g,
-                           b.records.as<int4>(), b.num_tiles, nullptr, w->row_level.as<int>(),
-                           st, co);
+    if (R > 0 && c->nnz > 0) {
+        if (trace_slots)
+            hipLaunchKernelGGL((numeric_row_kernel<SmallRow, true>), dim3(R), dim3(kThreads), 0,
+                               s, g, b.records.as<int4>(), b.num_tiles, nullptr,
+                               w->row_level.as<int>(), st, co, tl);
+        else
+            hipLaunchKernelGGL((numeric_row_kernel<SmallRow>), dim3(R), dim3(kThreads), 0, s, g,
+                               b.records.as<int4>(), b.num_tiles, nullptr,
+                               w->row_level.as<int>(), st, co, tl);
+    }
     RTT_HIP(hipGetLastError());
     RTT_HIP(hipEventRecord(ev->ev[5], s));
     if (n_num_lds > 0)
         hipLaunchKernelGGL((numeric_row_kernel<LargeRow>), dim3(n_num_lds), dim3(kThreads), 0,
                            s, g, b.records.as<int4>(), b.num_tiles, w->lds_rows.as<int>(),
-                           w->row_level.as<int>(), st, co);
+                           w->row_level.as<int>(), st, co, Timeline{});
     if (n_num_slab > 0) {
         const int blocks = std::min(n_num_slab, 4 * num_cus);
         DevBuf &tab = w->slab;   // reused: the symbolic slab pass has been queued before
@@ -1710,6 +1834,9 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     }
     RTT_HIP(hipEventRecord(ev->ev[7], s));
     RTT_HIP(hipStreamSynchronize(s));
+    if (trace_slots)
+        RTT_TRY(write_timeline(trace_path, tl, trace_slots, static_cast<uint64_t>(R),
+                               static_cast<uint64_t>(c->num_tiles), num_cus));
     // Structure write and numeric share the stream without a host wait
     // between them; their split comes from the device events.
     const double fill_ms = event_ms(ev->ev[2], ev->ev[6]);

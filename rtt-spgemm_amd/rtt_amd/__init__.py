@@ -490,6 +490,28 @@ def dump_reorder_heatmap(prefix: str, a: CsrMatrix, row_new_to_old=None, inner_n
                                            _stats_struct(stats)), "dump_reorder_heatmap")
 
 
+def read_timeline(path: str) -> dict:
+    """Reads a numeric timeline trace (the DMTL0001 layout of
+    dmma_write_timeline_trace, src/dmma_spgemm.h:95-164), written by the
+    library when DMMA_TRACE_FILE is set: header fields and per-slot start and
+    end times (ns) and compute-unit ids."""
+    with open(path, "rb") as f:
+        data = f.read()
+    if data[:8] != b"DMTL0001":
+        raise ValueError(f"{path}: not a DMTL0001 timeline")
+    head = np.frombuffer(data, np.uint32, 5, 8)
+    tail = np.frombuffer(data, np.uint64, 3, 28)
+    slots = int(tail[2])
+    off = 52
+    start = np.frombuffer(data, np.uint64, slots, off)
+    end = np.frombuffer(data, np.uint64, slots, off + 8 * slots)
+    cu = np.frombuffer(data, np.uint32, slots, off + 16 * slots)
+    return {"version": int(head[0]), "compute_units": int(head[1]),
+            "waves_per_workgroup": int(head[2]), "shift": int(head[3]), "phase": int(head[4]),
+            "output_tiles": int(tail[0]), "workgroups": int(tail[1]), "slots": slots,
+            "start_ns": start, "end_ns": end, "cu": cu}
+
+
 def prepare_bgrf_a(a: CsrMatrix, dense_threshold: int = 24) -> PreparedA:
     """A in the BGRF-v1 layout (the reference driver's default)."""
     s = a.as_smatrix()

@@ -264,6 +264,33 @@ def test_lb_modes_agree_on_slab_rows(gpu, lb_mode):
     check_c(gpu, tc, oracle_product(a, 0))
 
 
+def test_numeric_timeline_trace(gpu, tmp_path, monkeypatch):
+    """DMMA_TRACE_FILE turns on the sampled timeline of the numeric kernel:
+    the file has the DMTL0001 header, one slot per wave of every 2^shift-th
+    workgroup from the phase on, each with end >= start > 0; the product
+    is unchanged, and without the variable no file is written."""
+    a = gpu.gen_stencil27(12, 0.5, 1.5, 3)
+    path = tmp_path / "trace.bin"
+    monkeypatch.setenv("DMMA_TRACE_FILE", str(path))
+    monkeypatch.setenv("DMMA_TRACE_SHIFT", "2")
+    monkeypatch.setenv("DMMA_TRACE_PHASE", "1")
+    pa = gpu.prepare_identity_a(a)
+    pb = gpu.rebuild_b(pa, 1)
+    tc, _ = gpu.tilespgemm(pa, pb)
+    t = gpu.read_timeline(str(path))
+    R = (a.m + 7) // 8
+    assert (t["version"], t["waves_per_workgroup"], t["shift"], t["phase"]) == (1, 4, 2, 1)
+    assert t["workgroups"] == R and t["output_tiles"] == tc.numtile
+    assert t["slots"] == ((R - 1 - 1) // 4 + 1) * 4
+    assert np.all(t["start_ns"] > 0) and np.all(t["end_ns"] >= t["start_ns"])
+    assert t["compute_units"] > 0
+    check_c(gpu, tc, oracle_product(a, 1), positive=True)
+    monkeypatch.delenv("DMMA_TRACE_FILE")
+    path.unlink()
+    gpu.tilespgemm(pa, pb)
+    assert not path.exists()
+
+
 def test_rectangular_aat_and_empty_shapes(gpu):
     a = gpu.gen_random(37, 300, 5, 9)
     run(gpu, a, 1)
