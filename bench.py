@@ -126,6 +126,40 @@ def profile_traffic(workload, kernel):
     return None, None
 
 
+# The other product mode of the same benchmark configuration, reported
+# beside the main line: C = A*A on the R-MAT 2^20 matrix, in the reference's
+# default (BGRF-v1) layout.
+SECOND_MODE = {"workload": "rmat-20-aa", "layout": "bgrf", "steps": 3, "warmup": 1}
+
+
+def second_mode():
+    """Runs the other mode's workload in a child process (this script with
+    its own arguments) and returns the essentials of its JSON line, or an
+    error entry; a failure there does not end the benchmark."""
+    m = SECOND_MODE
+    cmd = [sys.executable, os.path.abspath(__file__), "--workload", m["workload"], "--layout",
+           m["layout"], "--steps", str(m["steps"]), "--warmup", str(m["warmup"]),
+           "--no-cpu-baseline", "--no-control", "--no-second-mode"]
+    try:
+        p = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    except subprocess.TimeoutExpired:
+        return {"workload": m["workload"], "error": "timed out after 600 s"}
+    lines = [l for l in p.stdout.splitlines() if l.startswith('{"metric"')]
+    if p.returncode != 0 or not lines:
+        return {"workload": m["workload"], "error": f"exit {p.returncode}: {p.stderr[-200:]}"}
+    r = json.loads(lines[-1])
+    # Most of this workload's rows take the slab kernels, which the per-kernel
+    # events do not cover: its roofline is given for the whole step, as the
+    # compulsory CSR bytes (A, B, C) over the step time.
+    frac = r["config"]["pipeline_hbm_frac"]
+    return {"workload": r["config"]["workload"], "layout": m["layout"], "value": r["value"],
+            "unit": r["unit"], "ms_per_step": r["ms_per_step"], "steps": r["steps"],
+            "warmup": r["warmup"],
+            "roofline": {"bound": "hbm", "scope": "whole step, CSR bytes of A, B and C",
+                         "achieved": round(frac * HBM_PEAK_GBS, 2), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": frac}}
+
+
 def rocsparse_control(workload):
     """rocSPARSE's SpGEMM on the same workload (the reference's vendor
     control, src/cusparse_core_benchmark.h:4-30): the buffer-size and nnz
@@ -212,6 +246,8 @@ def main():
     ap.add_argument("--no-control", action="store_true",
                     help="skip the rocSPARSE SpGEMM control on the same workload")
     ap.add_argument("--control-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--no-second-mode", action="store_true",
+                    help="skip the R-MAT 2^20 C = A*A measurement reported beside the main line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (traffic-counter scale)")
     args = ap.parse_args()
@@ -229,8 +265,10 @@ def main():
     import rtt_amd as rt
     spec = WORKLOADS[args.workload]
     aat = spec["aat"]
-    # The control's child process runs before this process touches the GPU.
+    # The child processes (control, other mode) run before this process
+    # touches the GPU.
     control = None if args.no_control else rocsparse_control(args.workload)
+    other = None if args.no_second_mode else second_mode()
     a, desc = make_matrix(rt, spec)
     rt.init(0)
     pa = rt.prepare_bgrf_a(a) if args.layout == "bgrf" else rt.prepare_identity_a(a)
@@ -320,6 +358,7 @@ def main():
                      "traffic_source": traffic_src},
         "cpu_baseline": cpu,
         "control": control,
+        "second_mode": other,
     }
     print(json.dumps(line), flush=True)
 
