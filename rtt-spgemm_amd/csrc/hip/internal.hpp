@@ -157,6 +157,11 @@ struct SpgemmWork {
     DevBuf queue;                           // work-queue heads of the slab kernels
     DevBuf row_cost, row_cost2, rows_sorted;   // slab rows, heaviest first
     DevBuf trace_start, trace_end, trace_cu;   // numeric timeline samples
+    // Slab rows' (column, mask) lists between the symbolic passes: entries
+    // allocated, and the most the previous calls needed (grown at the next
+    // call's start, never while a fill pass may still read the buffer).
+    DevBuf slab_stage_j, slab_stage_m, slab_row_off;
+    uint64_t slab_stage_cap = 0, slab_stage_need = 0;
 };
 
 // Scratch of the tile builders, reused across calls.

@@ -766,17 +766,31 @@ __device__ __forceinline__ int load_slab_chunk(const Operands &g, int c0, int n,
     return total;
 }
 
+// The count pass leaves each row's (column, mask) list, in column order, in
+// a grow-only staging buffer when it fits (row_off[row] = its first entry,
+// -1 otherwise); the fill pass then copies the list instead of repeating
+// the row's atomics.  cursor sums every row's tile count, fitted or not, so
+// that the caller learns the capacity the next call needs.
+struct SlabStage {
+    int *j = nullptr;
+    unsigned long long *m = nullptr;
+    int64_t *row_off = nullptr;
+    unsigned long long *cursor = nullptr;
+    unsigned long long capacity = 0;
+};
+
 template <bool FILL>
 __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
     Operands g, const int *__restrict__ rows, int nrows, int *__restrict__ queue,
     int static_rows, unsigned long long *__restrict__ slabs, int64_t *__restrict__ row_tiles,
-    int64_t *__restrict__ row_nnz, COut c)
+    int64_t *__restrict__ row_nnz, SlabStage ss, COut c)
 {
     __shared__ uint32_t s_ab[kThreads];
     __shared__ int s_b0[kThreads];
     __shared__ int s_pre[kThreads];
     __shared__ int s_tmp[kWavesPerBlock];
     __shared__ int s_lo, s_hi;
+    __shared__ long long s_base;
     unsigned long long *slab = slabs + static_cast<int64_t>(blockIdx.x) * g.b_tile_cols;
     const int tid = threadIdx.x;
     for (int step = 0;; ++step) {
@@ -785,6 +799,31 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
         if (ri >= nrows)
             break;
         const int row = rows[ri];
+        if constexpr (FILL) {
+            const int64_t base = ss.row_off ? ss.row_off[row] : -1;
+            if (base >= 0) {
+                // Staged: the structure in column order, value offsets by a scan.
+                const int64_t tb = c.tile_ptr[row];
+                const int nt = static_cast<int>(c.tile_ptr[row + 1] - tb);
+                int64_t zb = c.row_nnz_off[row];
+                for (int q0 = 0; q0 < nt; q0 += kThreads) {
+                    const int q = q0 + tid;
+                    unsigned long long m = 0;
+                    int j = 0;
+                    if (q < nt) {
+                        j = ss.j[base + q];
+                        m = ss.m[base + q];
+                    }
+                    int npc = 0;
+                    const int zpos = block_exclusive_scan(__popcll(m), s_tmp, &npc);
+                    if (q < nt)
+                        write_tile_structure(c, tb + q, j, m, zb + zpos);
+                    zb += npc;
+                }
+                __syncthreads();
+                continue;
+            }
+        }
         const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
         if (tid == 0) {
             s_lo = INT_MAX;
@@ -816,36 +855,72 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
         __threadfence();
         __syncthreads();
         const int lo = s_lo, hi = s_hi;
-        int64_t tiles = 0, nz = 0;
-        int64_t tb = 0, zb = 0;
-        if constexpr (FILL) {
-            tb = c.tile_ptr[row];
-            zb = c.row_nnz_off[row];
-        }
-        for (int j0 = lo; j0 <= hi; j0 += kThreads) {
-            const int j = j0 + tid;
-            unsigned long long m = 0;
-            if (j <= hi) {
-                m = load_coherent(&slab[j]);
-                if (m)
-                    slab[j] = 0;
-            }
-            const int occ = m ? 1 : 0;
-            const int pc = __popcll(m);
-            int ntile = 0, npc = 0;
-            const int tpos = block_exclusive_scan(occ, s_tmp, &ntile);
-            const int zpos = block_exclusive_scan(pc, s_tmp, &npc);
-            if constexpr (FILL) {
-                if (occ)
-                    write_tile_structure(c, tb + tiles + tpos, j, m, zb + nz + zpos);
-            }
-            tiles += ntile;
-            nz += npc;
-        }
         if constexpr (!FILL) {
+            // Sizes first, without consuming the masks.
+            int64_t tiles = 0, nz = 0;
+            for (int j0 = lo; j0 <= hi; j0 += kThreads) {
+                const int j = j0 + tid;
+                const unsigned long long m = j <= hi ? load_coherent(&slab[j]) : 0ull;
+                int ntile = 0, npc = 0;
+                block_exclusive_scan(m ? 1 : 0, s_tmp, &ntile);
+                block_exclusive_scan(__popcll(m), s_tmp, &npc);
+                tiles += ntile;
+                nz += npc;
+            }
             if (tid == 0) {
+                long long b = -1;
+                if (ss.cursor && tiles > 0) {
+                    const unsigned long long at =
+                        atomicAdd(ss.cursor, static_cast<unsigned long long>(tiles));
+                    if (at + static_cast<unsigned long long>(tiles) <= ss.capacity)
+                        b = static_cast<long long>(at);
+                }
+                s_base = b;
                 row_tiles[row] = tiles;
                 row_nnz[row] = nz;
+                if (ss.row_off)
+                    ss.row_off[row] = b;
+            }
+            __syncthreads();
+            // Then the list, staged when it fits, and the slab cleared.
+            const long long base = s_base;
+            int64_t done = 0;
+            for (int j0 = lo; j0 <= hi; j0 += kThreads) {
+                const int j = j0 + tid;
+                unsigned long long m = 0;
+                if (j <= hi) {
+                    m = load_coherent(&slab[j]);
+                    if (m)
+                        slab[j] = 0;
+                }
+                int ntile = 0;
+                const int tpos = block_exclusive_scan(m ? 1 : 0, s_tmp, &ntile);
+                if (base >= 0 && m) {
+                    ss.j[base + done + tpos] = j;
+                    ss.m[base + done + tpos] = m;
+                }
+                done += ntile;
+            }
+        } else {
+            // Not staged: the masks again, written straight to C's structure.
+            int64_t tiles = 0, nz = 0;
+            const int64_t tb = c.tile_ptr[row];
+            const int64_t zb = c.row_nnz_off[row];
+            for (int j0 = lo; j0 <= hi; j0 += kThreads) {
+                const int j = j0 + tid;
+                unsigned long long m = 0;
+                if (j <= hi) {
+                    m = load_coherent(&slab[j]);
+                    if (m)
+                        slab[j] = 0;
+                }
+                int ntile = 0, npc = 0;
+                const int tpos = block_exclusive_scan(m ? 1 : 0, s_tmp, &ntile);
+                const int zpos = block_exclusive_scan(__popcll(m), s_tmp, &npc);
+                if (m)
+                    write_tile_structure(c, tb + tiles + tpos, j, m, zb + nz + zpos);
+                tiles += ntile;
+                nz += npc;
             }
         }
         __threadfence();
@@ -1612,7 +1687,8 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     const int num_cus = ctx().num_cus > 0 ? ctx().num_cus : 256;
     // Counters read back by the host: [0] pairs, [1] symbolic slab rows,
     // [2] numeric slab rows, [3] numeric large-LDS rows, [4] C tiles,
-    // [5] nnz(C), [6] rows for the workgroup symbolic table.
+    // [5] nnz(C), [6] rows for the workgroup symbolic table, [7] slab tiles
+    // the staging cursor handed out.
     RTT_TRY(w->counters.ensure(8 * sizeof(unsigned long long)));
     RTT_TRY(w->host_counters.ensure(8 * sizeof(unsigned long long)));
     unsigned long long *ctr = w->counters.as<unsigned long long>();
@@ -1688,7 +1764,31 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     int *queue = w->queue.as<int>();
     const int static_rows = lb_mode == RTT_LB_STATIC ? 1 : 0;
     hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(4), 0, s, queue, int64_t(4), 0);
+    SlabStage ss;
     if (n_slab > 0) {
+        // Staging of the slab rows' tile lists: grown to what the previous
+        // calls needed, within a quarter of the free device memory; without
+        // it every slab row is recomputed by the fill pass.
+        if (w->slab_stage_need > w->slab_stage_cap) {
+            size_t free_b = 0, total_b = 0;
+            const uint64_t per = sizeof(int) + sizeof(unsigned long long);
+            uint64_t want = w->slab_stage_need + w->slab_stage_need / 16;
+            if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+                want = std::min<uint64_t>(want, free_b / 4 / per);
+            w->slab_stage_cap = 0;
+            if (want > 0 && w->slab_stage_j.ensure(want * sizeof(int)) == RTT_OK &&
+                w->slab_stage_m.ensure(want * sizeof(unsigned long long)) == RTT_OK)
+                w->slab_stage_cap = want;
+            clear_error();
+        }
+        RTT_TRY(w->slab_row_off.ensure(static_cast<size_t>(R) * sizeof(int64_t)));
+        ss.row_off = w->slab_row_off.as<int64_t>();
+        ss.cursor = ctr + 7;
+        if (w->slab_stage_cap > 0) {
+            ss.j = w->slab_stage_j.as<int>();
+            ss.m = w->slab_stage_m.as<unsigned long long>();
+            ss.capacity = w->slab_stage_cap;
+        }
         slab_blocks = std::min(n_slab, 4 * num_cus);
         slab_rows = w->l3_rows.as<int>();
         if (lb_mode == RTT_LB_ADAPTIVE_TAIL)
@@ -1701,7 +1801,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
         hipLaunchKernelGGL((symbolic_slab_kernel<false>), dim3(slab_blocks), dim3(kThreads), 0, s,
                            g, slab_rows, n_slab, queue, static_rows,
                            w->slab.as<unsigned long long>(), w->row_tiles.as<int64_t>(),
-                           w->row_nnz.as<int64_t>(), none);
+                           w->row_nnz.as<int64_t>(), ss, none);
         RTT_HIP(hipGetLastError());
     }
     RTT_TRY(exclusive_scan_i64(w->row_tiles.as<int64_t>(), c->tile_ptr.as<int64_t>(), R,
@@ -1720,8 +1820,9 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
                        reinterpret_cast<int64_t *>(ctr + 4), c->tile_ptr.as<int64_t>() + R);
     hipLaunchKernelGGL(set_last_kernel, dim3(1), dim3(1), 0, s,
                        reinterpret_cast<int64_t *>(ctr + 5), w->c_row_nnz_off.as<int64_t>() + R);
-    RTT_HIP(hipMemcpyAsync(hctr, ctr, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    RTT_HIP(hipMemcpyAsync(hctr, ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     RTT_HIP(hipStreamSynchronize(s));
+    w->slab_stage_need = std::max<uint64_t>(w->slab_stage_need, hctr[7]);
     const unsigned long long pairs = hctr[0];
     const int n_num_slab = static_cast<int>(hctr[2] & 0xffffffffu);
     const int n_num_lds = static_cast<int>(hctr[3] & 0xffffffffu);
@@ -1765,7 +1866,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     if (n_slab > 0)
         hipLaunchKernelGGL((symbolic_slab_kernel<true>), dim3(slab_blocks), dim3(kThreads), 0, s,
                            g, slab_rows, n_slab, queue + 1, static_rows,
-                           w->slab.as<unsigned long long>(), nullptr, nullptr, co);
+                           w->slab.as<unsigned long long>(), nullptr, nullptr, ss, co);
     hipLaunchKernelGGL(set_last_kernel, dim3(1), dim3(1), 0, s,
                        c->tile_nnz.as<int64_t>() + c->num_tiles,
                        w->c_row_nnz_off.as<int64_t>() + R);
