@@ -258,3 +258,70 @@ def test_amg_galerkin_rap(gpu, g):
     assert d["reason_name"] == "eligible-safe-auto"
     tc2, _ = gpu.tilespgemm(pa, pb)
     check_c(gpu, tc2, oracle.spgemm(as_tuple(a), as_tuple(p2)))
+
+
+# ---- block-structured FEM-like A^2 (4x4 dense blocks on a random 3-D mesh,
+#      log-uniform magnitudes over six decades, random signs) -------------
+
+def fem_blocks(nodes, seed):
+    from scipy.spatial import cKDTree
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    pts = rng.uniform(size=(nodes, 3))
+    _, nb = cKDTree(pts).query(pts, k=7)          # self + 6 nearest
+    pairs = set()
+    for i in range(nodes):
+        for j in nb[i]:
+            pairs.add((i, int(j)))
+            pairs.add((int(j), i))                   # symmetrised
+    bi = np.array([p[0] for p in pairs])
+    bj = np.array([p[1] for p in pairs])
+    nblk = len(bi)
+    mag = 10.0 ** rng.uniform(-3, 3, size=(nblk, 4, 4))
+    val = mag * rng.choice([-1.0, 1.0], size=(nblk, 4, 4))
+    r = (4 * bi[:, None, None] + np.arange(4)[None, :, None]).repeat(4, axis=2)
+    c = (4 * bj[:, None, None] + np.arange(4)[None, None, :]).repeat(4, axis=1)
+    return sp.csr_matrix((val.ravel(), (r.ravel(), c.ravel())), shape=(4 * nodes, 4 * nodes))
+
+
+def test_fem_block_matrix_square(gpu):
+    """Dense 4x4 blocks fill whole 8x4 A tiles and 4x8 B tiles (dense
+    payloads), and the magnitudes span 1e-3..1e3 with random signs, so the
+    values are checked with the forward-error bound."""
+    A = fem_blocks(1500, 77)
+    a = to_rtt(gpu, A)
+    pa = gpu.prepare_identity_a(a)
+    assert pa.tiles()["dense_tiles"] > 0
+    pb = gpu.rebuild_b(pa, 0)
+    tc, _ = gpu.tilespgemm(pa, pb)
+    ref = oracle.spgemm(as_tuple(a), as_tuple(a))
+    check_c(gpu, tc, ref)
+
+
+# ---- a batch of banded A^2 problems of uneven sizes ----------------------
+
+def banded(n, bw, per_row, rng):
+    import scipy.sparse as sp
+    rows, cols = [], []
+    for i in range(n):
+        lo, hi = max(0, i - bw), min(n, i + bw + 1)
+        k = min(per_row, hi - lo)
+        cs = set(rng.choice(np.arange(lo, hi), size=k, replace=False).tolist()) | {i}
+        rows += [i] * len(cs)
+        cols += sorted(cs)
+    return sp.csr_matrix((rng.uniform(-1, 1, size=len(rows)), (rows, cols)), shape=(n, n))
+
+
+def test_banded_batch(gpu):
+    """Uneven banded problems (log-normal sizes, bandwidth 8..64, 5..30
+    entries per row inside the band) solved one after another on one
+    handle set each, as the batched workload's problems are."""
+    rng = np.random.default_rng(2026)
+    sizes = np.clip(np.round(np.exp(rng.normal(np.log(3000), 0.8, size=6))), 1000, 20000)
+    for n in sizes.astype(int):
+        A = banded(int(n), int(rng.integers(8, 65)), int(rng.integers(5, 31)), rng)
+        a = to_rtt(gpu, A)
+        pa = gpu.prepare_identity_a(a)
+        pb = gpu.rebuild_b(pa, 0)
+        tc, _ = gpu.tilespgemm(pa, pb)
+        check_c(gpu, tc, oracle.spgemm(as_tuple(a), as_tuple(a)))
