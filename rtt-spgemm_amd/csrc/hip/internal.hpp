@@ -19,8 +19,9 @@ constexpr int kInputElems = 32;
 // row k, are "heavy"; the others are "light" and lie in one row k.  Every B
 // row keeps its heavy tiles first, then its light tiles grouped by k, so
 // that pairs of like cost can be taken together and an A tile with entries
-// in one column k meets only the light tiles of row k.
-constexpr int kHeavyMin = 4;
+// in one column k meets only the light tiles of row k.  A light tile's at
+// most two values fill the first value part of its record.
+constexpr int kHeavyMin = 3;
 constexpr int kClassSlots = 8;   // class slots per B row: heavy, light k = 0..3, unused
 
 // Per-B-tile record read by the numeric kernel: the packed meta and the

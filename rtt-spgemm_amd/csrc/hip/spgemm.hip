@@ -1195,13 +1195,15 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
                     const int64_t rb_idx = s_pw[v] + p;
                     BRecord rec;
                     rec.meta = recs[rb_idx];
-                    // Runs 2 and 3 hold the light B tiles (at most 3 entries,
-                    // one row): their values fit in the first two parts.
+                    // Runs 2 and 3 hold the light B tiles (fewer than
+                    // kHeavyMin entries, one row): their values fit in the
+                    // first part.
+                    static_assert(kHeavyMin - 1 <= 2, "light values fill one record part");
                     const bool light_b = v >= 2 * kNumChunk;
 #pragma unroll
                     for (int i = 0; i < kRecVals / 2; ++i) {
                         double2 d = make_double2(0.0, 0.0);
-                        if (i < 2 || !light_b)
+                        if (i < 1 || !light_b)
                             d = reinterpret_cast<const double2 *>(recs + (i + 1) * rec_n)[rb_idx];
                         rec.v[2 * i] = d.x;
                         rec.v[2 * i + 1] = d.y;
