@@ -314,6 +314,15 @@ __global__ void row_ptr_from_sorted_kernel(const int *__restrict__ keys, int64_t
 
 // Part `part` of a tile's record (meta, then the first kRecVals values of
 // the tile's k-major payload: row k's entries contiguous, columns ascending).
+// Parts of a record that hold something: the meta, then one part per two of
+// the tile's first kRecVals values.  The kernels that read records use a
+// value only when the tile has it, so the parts past these are never used;
+// the writers that hold the meta anyway skip them.
+__device__ __forceinline__ int record_parts_used(const int4 &m)
+{
+    return 1 + (min(kRecVals, __popc(static_cast<uint32_t>(m.y))) + 1) / 2;
+}
+
 __device__ __forceinline__ int4 record_part(int part, const int4 &m, const double *kmajor)
 {
     if (part == 0)
@@ -336,7 +345,9 @@ __global__ void records_kernel(int64_t n, const int4 *__restrict__ meta,
         return;
     const int part = static_cast<int>(gid / n);
     const int64_t t = gid - part * n;
-    planes[gid] = record_part(part, meta[t], kmajor);
+    const int4 m = meta[t];
+    if (part < record_parts_used(m))
+        planes[gid] = record_part(part, m, kmajor);
 }
 
 // Value parts (1 .. kRecParts-1) of B's record planes; part 0 (meta) is
@@ -350,7 +361,9 @@ __global__ void record_values_kernel(int64_t n, const int4 *__restrict__ meta,
         return;
     const int part = 1 + static_cast<int>(gid / n);
     const int64_t t = gid - (part - 1) * n;
-    planes[part * n + t] = record_part(part, meta[t], kmajor);
+    const int4 m = meta[t];
+    if (part < record_parts_used(m))
+        planes[part * n + t] = record_part(part, m, kmajor);
 }
 
 // Values-only update: the first source entry of each slot's run sums the
@@ -486,6 +499,8 @@ __global__ void gather_records_kernel(int64_t n, const int *__restrict__ perm,
     const int part = static_cast<int>(gid / n);
     const int64_t t = gid - part * n;
     const int src = perm[t];
+    if (part > 0 && part >= record_parts_used(meta_in[src]))
+        return;
     rec_out[gid] = rec_in[part * n + src];
     if (part == 0)
         meta_out[t] = meta_in[src];
