@@ -77,6 +77,16 @@ int upload(const rtt_smatrix &h, DevCsr *d)
     return 0;
 }
 
+// A device allocation freed when it goes out of scope.
+template <typename T>
+struct DevPtr {
+    T *p = nullptr;
+    DevPtr() = default;
+    DevPtr(const DevPtr &) = delete;
+    DevPtr &operator=(const DevPtr &) = delete;
+    ~DevPtr() { (void)hipFree(p); }
+};
+
 struct Handle {
     rocsparse_handle h = nullptr;
     ~Handle()
@@ -114,11 +124,11 @@ int spgemm_once(rocsparse_handle h, const Descr &a, const Descr &b, const Descr 
                 double *checksum)
 {
     const double alpha = 1.0, beta = 0.0;
-    int *crp = nullptr, *cci = nullptr;
-    double *cval = nullptr;
-    CK_HIP(hipMalloc(&crp, (static_cast<size_t>(rows) + 1) * sizeof(int)));
+    DevPtr<int> crp, cci;
+    DevPtr<double> cval;
+    CK_HIP(hipMalloc(&crp.p, (static_cast<size_t>(rows) + 1) * sizeof(int)));
     Descr c;
-    CK_SP(rocsparse_create_csr_descr(&c.d, rows, cols, 0, crp, nullptr, nullptr,
+    CK_SP(rocsparse_create_csr_descr(&c.d, rows, cols, 0, crp.p, nullptr, nullptr,
                                      rocsparse_indextype_i32, rocsparse_indextype_i32,
                                      rocsparse_index_base_zero, rocsparse_datatype_f64_r));
     CK_HIP(hipDeviceSynchronize());
@@ -141,9 +151,9 @@ int spgemm_once(rocsparse_handle h, const Descr &a, const Descr &b, const Descr 
                            *buffer));
     int64_t r = 0, cc = 0, nz = 0;
     CK_SP(rocsparse_spmat_get_size(c.d, &r, &cc, &nz));
-    CK_HIP(hipMalloc(&cci, std::max<int64_t>(1, nz) * sizeof(int)));
-    CK_HIP(hipMalloc(&cval, std::max<int64_t>(1, nz) * sizeof(double)));
-    CK_SP(rocsparse_csr_set_pointers(c.d, crp, cci, cval));
+    CK_HIP(hipMalloc(&cci.p, std::max<int64_t>(1, nz) * sizeof(int)));
+    CK_HIP(hipMalloc(&cval.p, std::max<int64_t>(1, nz) * sizeof(double)));
+    CK_SP(rocsparse_csr_set_pointers(c.d, crp.p, cci.p, cval.p));
     CK_SP(rocsparse_spgemm(h, rocsparse_operation_none, rocsparse_operation_none, &alpha, a.d,
                            b.d, &beta, d.d, c.d, rocsparse_datatype_f64_r,
                            rocsparse_spgemm_alg_default, rocsparse_spgemm_stage_compute, &bytes,
@@ -155,16 +165,13 @@ int spgemm_once(rocsparse_handle h, const Descr &a, const Descr &b, const Descr 
     if (checksum) {
         std::vector<double> v(static_cast<size_t>(nz));
         if (nz > 0)
-            CK_HIP(hipMemcpy(v.data(), cval, static_cast<size_t>(nz) * sizeof(double),
+            CK_HIP(hipMemcpy(v.data(), cval.p, static_cast<size_t>(nz) * sizeof(double),
                              hipMemcpyDeviceToHost));
         double s = 0.0;
         for (double x : v)
             s += x;
         *checksum = s;
     }
-    (void)hipFree(crp);
-    (void)hipFree(cci);
-    (void)hipFree(cval);
     return 0;
 }
 
@@ -212,15 +219,11 @@ int rtt_control_rocsparse(const rtt_smatrix *a_host, int aat, int warmup, int it
         size_t tb = 0;
         CK_SP(rocsparse_csr2csc_buffer_size(h.h, a.rows, a.cols, static_cast<int>(a.nnz), a.rp,
                                             a.ci, rocsparse_action_numeric, &tb));
-        void *tmp = nullptr;
-        CK_HIP(hipMalloc(&tmp, std::max<size_t>(tb, 1)));
-        const rocsparse_status st =
-            rocsparse_dcsr2csc(h.h, a.rows, a.cols, static_cast<int>(a.nnz), a.val, a.rp, a.ci,
-                               b.val, b.ci, b.rp, rocsparse_action_numeric,
-                               rocsparse_index_base_zero, tmp);
-        (void)hipFree(tmp);
-        if (st != rocsparse_status_success)
-            return fail("rocsparse_dcsr2csc", static_cast<int>(st));
+        DevPtr<void> tmp;
+        CK_HIP(hipMalloc(&tmp.p, std::max<size_t>(tb, 1)));
+        CK_SP(rocsparse_dcsr2csc(h.h, a.rows, a.cols, static_cast<int>(a.nnz), a.val, a.rp, a.ci,
+                                 b.val, b.ci, b.rp, rocsparse_action_numeric,
+                                 rocsparse_index_base_zero, tmp.p));
     } else if (upload(*a_host, &b)) {
         return 1;
     }
@@ -236,22 +239,19 @@ int rtt_control_rocsparse(const rtt_smatrix *a_host, int aat, int warmup, int it
     Descr da, db, dd;
     if (make_descr(a, &da) || make_descr(b, &db) || make_descr(d, &dd))
         return 1;
-    void *buffer = nullptr;
+    DevPtr<void> buffer;
     size_t capacity = 0;
     std::vector<double> times;
     int64_t nz = 0;
-    int rc = 0;
-    for (int it = 0; it < warmup + iterations && rc == 0; ++it) {
+    for (int it = 0; it < warmup + iterations; ++it) {
         double ms = 0.0;
         const bool last = it == warmup + iterations - 1;
-        rc = spgemm_once(h.h, da, db, dd, a.rows, b.cols, &buffer, &capacity, &ms, &nz,
-                         last ? checksum : nullptr);
+        if (spgemm_once(h.h, da, db, dd, a.rows, b.cols, &buffer.p, &capacity, &ms, &nz,
+                        last ? checksum : nullptr))
+            return 1;
         if (it >= warmup)
             times.push_back(ms);
     }
-    (void)hipFree(buffer);
-    if (rc)
-        return rc;
     std::sort(times.begin(), times.end());
     *median_ms = times[times.size() / 2];
     *nnz_c = static_cast<long long>(nz);
