@@ -1456,10 +1456,15 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
                     const int4 sl = slab[meta.x];
                     if (sl.x != row)
                         continue;
+                    // Only the parts that hold the tile's values: most tiles of
+                    // the rows that come here have one or two entries.
+                    const int nbv = __popc(static_cast<uint32_t>(meta.z));
                     double rv[kRecVals];
 #pragma unroll
                     for (int i = 0; i < kRecVals / 2; ++i) {
-                        const double2 d = reinterpret_cast<const double2 *>(recs + (i + 1) * rec_n)[b];
+                        double2 d = make_double2(0.0, 0.0);
+                        if (2 * i < nbv)
+                            d = reinterpret_cast<const double2 *>(recs + (i + 1) * rec_n)[b];
                         rv[2 * i] = d.x;
                         rv[2 * i + 1] = d.y;
                     }
