@@ -442,27 +442,43 @@ __device__ __forceinline__ int b_tile_class(uint32_t mask, uint32_t bytes)
 // Per A tile t in column K: {first B tile of row K, heavy tiles, all tiles}
 // and the light run its pairs need: for an A tile whose entries all lie in
 // one column k, only the light tiles of row k can meet it.
-__global__ void pair_extent_kernel(int64_t n, const int4 *__restrict__ a_meta,
-                                   const int *__restrict__ groups, int4 *__restrict__ out,
-                                   int2 *__restrict__ light)
+__global__ __launch_bounds__(kThreads) void pair_extent_kernel(
+    int64_t n, const int4 *__restrict__ a_meta, const int *__restrict__ groups,
+    int4 *__restrict__ out, int2 *__restrict__ light, unsigned long long *__restrict__ pairs)
 {
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= n)
-        return;
-    const int4 m = a_meta[t];
-    const int *gp = groups + static_cast<int64_t>(kClassSlots) * m.x;
-    const int b0 = gp[0];
-    out[t] = make_int4(b0, gp[1] - b0, gp[kClassSlots] - b0, 0);
-    const uint32_t bytes = static_cast<uint32_t>(m.z);
-    uint32_t cols = 0;
+    // Grid-stride, so that the pair total takes one atomic per workgroup.
+    __shared__ unsigned long long s_sum[kThreads / kWave];
+    unsigned long long sum = 0;
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+         t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int4 m = a_meta[t];
+        const int *gp = groups + static_cast<int64_t>(kClassSlots) * m.x;
+        const int b0 = gp[0];
+        const int total = gp[kClassSlots] - b0;
+        out[t] = make_int4(b0, gp[1] - b0, total, 0);
+        sum += static_cast<unsigned long long>(total);
+        const uint32_t bytes = static_cast<uint32_t>(m.z);
+        uint32_t cols = 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-        cols |= ((bytes >> (8 * k)) & 0xFFu) ? (1u << k) : 0u;
-    if (__popc(cols) == 1) {
-        const int k0 = __builtin_ctz(cols);
-        light[t] = make_int2(gp[1 + k0], gp[2 + k0] - gp[1 + k0]);
-    } else {
-        light[t] = make_int2(gp[1], gp[5] - gp[1]);
+        for (int k = 0; k < 4; ++k)
+            cols |= ((bytes >> (8 * k)) & 0xFFu) ? (1u << k) : 0u;
+        if (__popc(cols) == 1) {
+            const int k0 = __builtin_ctz(cols);
+            light[t] = make_int2(gp[1 + k0], gp[2 + k0] - gp[1 + k0]);
+        } else {
+            light[t] = make_int2(gp[1], gp[5] - gp[1]);
+        }
+    }
+    sum = wave_sum(sum);
+    if (lane_id() == 0)
+        s_sum[threadIdx.x / kWave] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long all = 0;
+        for (int w = 0; w < kThreads / kWave; ++w)
+            all += s_sum[w];
+        if (all)
+            atomicAdd(pairs, all);
     }
 }
 
@@ -740,11 +756,17 @@ int build_pair_extents(const DevTiles &a, DevTiles *b)
 {
     RTT_TRY(b->a_brange.ensure(a.num_tiles * sizeof(int4)));
     RTT_TRY(b->a_blight.ensure(a.num_tiles * sizeof(int2)));
+    RTT_TRY(b->pair_total.ensure(sizeof(unsigned long long)));
+    hipStream_t s = ctx().stream;
+    hipLaunchKernelGGL(set_i64_kernel, dim3(1), dim3(1), 0, s, b->pair_total.as<int64_t>(),
+                       int64_t(1), int64_t(0));
+    const int cus = ctx().num_cus > 0 ? ctx().num_cus : 256;
     if (a.num_tiles > 0)
-        hipLaunchKernelGGL(pair_extent_kernel, dim3(blocks_for(a.num_tiles, kThreads)),
-                           dim3(kThreads), 0, ctx().stream, a.num_tiles, a.meta.as<int4>(),
+        hipLaunchKernelGGL(pair_extent_kernel,
+                           dim3(static_cast<unsigned>(std::min<int64_t>(4 * cus, blocks_for(a.num_tiles, kThreads)))),
+                           dim3(kThreads), 0, s, a.num_tiles, a.meta.as<int4>(),
                            b->row_groups.as<int>(), b->a_brange.as<int4>(),
-                           b->a_blight.as<int2>());
+                           b->a_blight.as<int2>(), b->pair_total.as<unsigned long long>());
     RTT_HIP(hipGetLastError());
     return RTT_OK;
 }

@@ -110,6 +110,7 @@ struct DevTiles {
     DevBuf records;       // B only: kRecParts planes of int4, num_tiles entries each
     DevBuf a_brange;      // B only: per A tile, {first B tile, heavy, count} of B row K
     DevBuf a_blight;      // B only: per A tile, {first, count} of the light tiles it can meet
+    DevBuf pair_total;    // B only: u64, sum over A tiles of their B row's tile count
     DevBuf row_groups;    // B only: start of each (row, class) run, kClassSlots per row
     DevBuf values_cm;     // each tile's values with the stride-4 bit groups in order:
                           // A (8x4): column-major; B (4x8): k-major (row k's entries

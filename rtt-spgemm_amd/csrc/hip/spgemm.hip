@@ -1546,20 +1546,6 @@ __global__ void collect_rows_kernel(int n, int mode, const int *__restrict__ lev
 
 // Number of (A tile, B tile) pairs: the sum of the per-A-tile extents, one
 // atomic per workgroup.
-__global__ __launch_bounds__(kThreads) void pair_count_kernel(int64_t n,
-                                                              const int4 *__restrict__ brange,
-                                                              unsigned long long *__restrict__ out)
-{
-    __shared__ unsigned long long s_tmp[kWavesPerBlock];
-    unsigned long long sum = 0;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
-         i += static_cast<int64_t>(gridDim.x) * kThreads)
-        sum += static_cast<unsigned long long>(brange[i].z);
-    sum = block_sum(sum, s_tmp);
-    if (threadIdx.x == 0 && sum)
-        atomicAdd(out, sum);
-}
-
 __global__ void fill_i32_kernel(int *p, int64_t n, int v)
 {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1730,10 +1716,9 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, s,
                        reinterpret_cast<unsigned long long *>(w->row_nnz.as<int64_t>() + R),
                        int64_t(1));
-    if (a.num_tiles > 0)
-        hipLaunchKernelGGL(pair_count_kernel,
-                           dim3(static_cast<unsigned>(std::min<int64_t>(4 * num_cus, blocks_for(a.num_tiles, kThreads)))),
-                           dim3(kThreads), 0, s, a.num_tiles, g.a_brange, ctr);
+    // The pair total, summed by the B update's pair-extent pass.
+    hipLaunchKernelGGL(set_last_kernel, dim3(1), dim3(1), 0, s, reinterpret_cast<int64_t *>(ctr),
+                       b.pair_total.as<int64_t>());
     // One wave per row first; rows that outgrow its table go to the
     // workgroup table, and rows that outgrow that to the slab.
     RTT_HIP(hipEventRecord(ev->ev[0], s));
