@@ -411,17 +411,14 @@ __global__ void transpose_source_kernel(int64_t n, const int *__restrict__ a_row
 // each plane).
 __global__ void transposed_records_kernel(int64_t n, const int *__restrict__ ids,
                                           const BRecord *__restrict__ src,
-                                          int4 *__restrict__ b_meta, int4 *__restrict__ planes)
+                                          int4 *__restrict__ planes)
 {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const int64_t t = gid / kRecParts;
     if (t >= n)
         return;
     const int part = static_cast<int>(gid % kRecParts);
-    const int4 v = reinterpret_cast<const int4 *>(src + ids[t])[part];
-    planes[part * n + t] = v;
-    if (part == 0)
-        b_meta[t] = v;
+    planes[part * n + t] = reinterpret_cast<const int4 *>(src + ids[t])[part];
 }
 
 // Class of a B tile within its row: 0 for heavy tiles (kHeavyMin entries or
@@ -1058,7 +1055,6 @@ int transpose_tiles(const DevTiles &a, int b_cols, DevTiles *b, BuildWork *w)
     const int64_t slots = static_cast<int64_t>(kClassSlots) * R;
     RTT_TRY(b->tile_row_ptr.ensure((R + 1) * sizeof(int)));
     RTT_TRY(b->row_groups.ensure((slots + 1) * sizeof(int)));
-    RTT_TRY(b->meta.ensure(n * sizeof(int4)));
     RTT_TRY(b->sorted_ids.ensure(n * sizeof(int)));
     RTT_TRY(w->ids_in.ensure(n * sizeof(int)));
     RTT_TRY(w->keys_in.ensure(n * sizeof(int)));
@@ -1076,8 +1072,7 @@ int transpose_tiles(const DevTiles &a, int b_cols, DevTiles *b, BuildWork *w)
         RTT_TRY(b->records.ensure(n * sizeof(BRecord)));
         hipLaunchKernelGGL(transposed_records_kernel, dim3(blocks_for(n * kRecParts, kThreads)),
                            dim3(kThreads), 0, s, n, b->sorted_ids.as<int>(),
-                           a.transpose_src.as<BRecord>(), b->meta.as<int4>(),
-                           b->records.as<int4>());
+                           a.transpose_src.as<BRecord>(), b->records.as<int4>());
     }
     hipLaunchKernelGGL(row_ptr_from_sorted_kernel, dim3(blocks_for(n + 1, kThreads)),
                        dim3(kThreads), 0, s, w->keys_out.as<int>(), n, static_cast<int>(slots),

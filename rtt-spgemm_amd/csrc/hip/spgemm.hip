@@ -96,7 +96,7 @@ struct Operands {
     const int *__restrict__ a_row_off;      // A payload offset of each tile row
     const double *__restrict__ b_vals_km;   // B payload, each tile k-major
     const int *__restrict__ b_trp;
-    const int4 *__restrict__ b_meta;
+    const int4 *__restrict__ b_meta;        // B records' plane 0
     const double *__restrict__ b_vals;
     int b_tile_cols;
 };
@@ -1677,7 +1677,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     g.a_row_off = a.row_payload_off.as<int>();
     g.b_vals_km = b.payload_kmajor();
     g.b_trp = b.tile_row_ptr.as<int>();
-    g.b_meta = b.meta.as<int4>();
+    g.b_meta = b.records.as<int4>();   // part 0 of every record is the tile's meta
     g.b_vals = b.payload_values();
     g.b_tile_cols = BC;
 
