@@ -325,3 +325,25 @@ def test_banded_batch(gpu):
         pb = gpu.rebuild_b(pa, 0)
         tc, _ = gpu.tilespgemm(pa, pb)
         check_c(gpu, tc, oracle.spgemm(as_tuple(a), as_tuple(a)))
+
+
+def test_general_ab_empty_operands(gpu):
+    """An empty B (or an empty A) gives an empty C of the right shape, and a
+    values-only update of an empty B is a no-op."""
+    import numpy as np
+    a = gpu.gen_random(50, 40, 4, 71)
+    empty_b = gpu.CsrMatrix(40, 30, np.zeros(41, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    pa = gpu.prepare_identity_a(a)
+    src = gpu.upload_csr(empty_b)
+    pb = gpu.rebuild_b_general(pa, src)
+    tc, stats = gpu.tilespgemm(pa, pb)
+    assert stats["output_nnz"] == 0 and tc.nnz == 0 and (tc.m, tc.n) == (50, 30)
+    src.set_values(np.zeros(0))
+    d = gpu.update_b_values(pa, src, pb, "safe-auto")
+    assert pb.last["path"] == 4 and d["reason_name"] == "eligible-safe-auto"
+    assert gpu.nnz_cub_ab(pa, src) == 0
+    empty_a = gpu.CsrMatrix(20, 40, np.zeros(21, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    pe = gpu.prepare_identity_a(empty_a)
+    b = gpu.gen_random(40, 30, 3, 72)
+    tc2, stats2 = gpu.tilespgemm(pe, gpu.rebuild_b_general(pe, gpu.upload_csr(b)))
+    assert stats2["output_nnz"] == 0 and (tc2.m, tc2.n) == (20, 30)
