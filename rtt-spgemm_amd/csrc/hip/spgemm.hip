@@ -1372,7 +1372,7 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
 // when they fit, B's come from the record planes, and every product is an
 // FP64 atomic on C's values in HBM.
 // ---------------------------------------------------------------------------
-constexpr int kSlabStage = 4096;   // A values a slab chunk stages in LDS
+constexpr int kSlabStage = 1024;   // A values a slab chunk stages in LDS
 
 __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
     Operands g, const int4 *__restrict__ recs, int64_t rec_n, const int *__restrict__ rows,
@@ -1381,7 +1381,7 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
     __shared__ double s_acol[kSlabStage + 8];
     __shared__ int4 s_at[kThreads];   // {rows of column k (byte k), first value of column k (byte k), first value, unused}
     __shared__ int s_pw[kThreads];
-    __shared__ unsigned char s_seg[kSeg];
+    __shared__ int s_pre[kThreads];   // first pair of each chunk tile
     __shared__ int s_tmp[kWavesPerBlock];
     int4 *slab = slabs + static_cast<int64_t>(blockIdx.x) * g.b_tile_cols;
     const int tid = threadIdx.x;
@@ -1438,13 +1438,14 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
             int total = 0;
             const int pre = block_exclusive_scan(nb, s_tmp, &total);
             s_pw[tid] = bstart - pre;
-            for (int r0 = 0; r0 < total; r0 += kSeg) {
-                const int r1 = min(total, r0 + kSeg);
-                if (nb > 0)
-                    fill_segments(s_seg, pre, nb, r0, r1);
-                __syncthreads();
-                for (int p = r0 + tid; p < r1; p += kThreads) {
-                    const int v = s_seg[p - r0];
+            s_pre[tid] = pre;
+            __syncthreads();
+            // A pair finds its A tile by a binary search over the chunk's
+            // first pairs: a tile whose B row is a hub column would make one
+            // thread fill most of a segment table.
+            {
+                for (int p = tid; p < total; p += kThreads) {
+                    const int v = upper_index(s_pre, n, p);
                     const int4 at = s_at[v];
                     const int64_t b = s_pw[v] + p;
                     const int4 meta = recs[b];
@@ -1914,7 +1915,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
                            s, g, b.records.as<int4>(), b.num_tiles, w->lds_rows.as<int>(),
                            w->row_level.as<int>(), st, co, Timeline{});
     if (n_num_slab > 0) {
-        const int blocks = std::min(n_num_slab, 4 * num_cus);
+        const int blocks = std::min(n_num_slab, 6 * num_cus);
         DevBuf &tab = w->slab;   // reused: the symbolic slab pass has been queued before
         const int64_t ents = static_cast<int64_t>(blocks) * BC;
         RTT_TRY(tab.ensure(static_cast<size_t>(ents) * sizeof(int4)));
