@@ -97,6 +97,7 @@ struct Operands {
     const double *__restrict__ b_vals_km;   // B payload, each tile k-major
     const int *__restrict__ b_trp;
     const int4 *__restrict__ b_meta;        // B records' plane 0
+    const int *__restrict__ b_groups;       // per B row: start of each class run (kClassSlots)
     const double *__restrict__ b_vals;
     int b_tile_cols;
 };
@@ -1369,22 +1370,145 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
 // A direct-mapped HBM table, one int4 per B tile column {row, first value of
 // the C tile within the row, mask low word, mask high word}, gives a pair its
 // C tile in one load.  A's values are staged per chunk of A tiles in LDS
-// when they fit, B's come from the record planes, and every product is an
-// FP64 atomic on C's values in HBM.
+// when they fit; B's come from the record planes.
+//
+// A row whose A tiles fit one chunk is walked in windows of consecutive C
+// tiles that start within kSlabWinStep values of each other (at most
+// kSlabWin values): a window's products accumulate in LDS and the window is
+// then stored to C with plain coalesced stores, so its values take no HBM
+// atomics and no clearing pass.  Each A tile keeps a cursor into every class
+// run of its B row (each run is sorted by column) and gallops it to the
+// window's last column, about one load per run and window.  Rows with more
+// A tiles than a chunk clear their C values and add every product to them
+// with an FP64 atomic in HBM.
 // ---------------------------------------------------------------------------
 constexpr int kSlabStage = 1024;   // A values a slab chunk stages in LDS
+constexpr int kSlabWin = 2048;     // C values a slab window accumulates in LDS
+constexpr int kSlabWinStep = kSlabWin - 64;   // tile starts per window; a tile holds <= 64
+constexpr int kSlabRuns = 5;       // class runs of a B row: heavy, light k = 0..3
+static_assert(kSlabRuns + 1 <= kClassSlots, "the runs' ends come from the class slots");
+
+// First index after a, within [a, end), whose B tile column is >= j, given
+// that a's column is below j (end when none): doubling steps from a, then a
+// binary search in the last step.
+__device__ __forceinline__ int gallop_col(const int4 *__restrict__ meta, int a, int end, int j)
+{
+    int step = 1;
+    while (a + step < end && meta[a + step].x < j) {
+        a += step;
+        step <<= 1;
+    }
+    int lo = a + 1, hi = min(a + step, end);
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (meta[mid].x < j)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// First tile in [lo, hi] of a row whose first value is at or after v (hi
+// when none); nnz holds the row's tile starts (hi + 1 entries from the row's
+// first tile, the last one the row's end).
+__device__ __forceinline__ int first_tile_from(const int64_t *__restrict__ nnz, int lo, int hi,
+                                               int64_t v)
+{
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (nnz[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// The products of one pair (A tile at, B record b with meta) into the C tile
+// whose values start at acc and whose mask is cm: into an LDS window (WIN)
+// or with FP64 atomics on C's values in HBM.
+template <bool WIN>
+__device__ __forceinline__ void slab_products(const int4 *__restrict__ recs, int64_t rec_n,
+                                              const double *__restrict__ b_vals_km, int64_t b,
+                                              int4 meta, int4 at, const double *abase,
+                                              uint64_t cm, double *acc)
+{
+    // Only the parts that hold the tile's values: most tiles of the rows that
+    // come here have one or two entries.
+    const int nbv = __popc(static_cast<uint32_t>(meta.z));
+    double rv[kRecVals];
+#pragma unroll
+    for (int i = 0; i < kRecVals / 2; ++i) {
+        double2 d = make_double2(0.0, 0.0);
+        if (2 * i < nbv)
+            d = reinterpret_cast<const double2 *>(recs + (i + 1) * rec_n)[b];
+        rv[2 * i] = d.x;
+        rv[2 * i + 1] = d.y;
+    }
+    const uint32_t ab = static_cast<uint32_t>(at.x);
+    const double *acol = abase + at.z;
+    const uint32_t apf = static_cast<uint32_t>(at.y);
+    const uint32_t boff = static_cast<uint32_t>(meta.w) & ~kDenseBit;
+    auto entry = [&](int pos, double vb) {
+        const int k = pos >> 3, cl = pos & 7;
+        const double *ap = acol + byte_of(apf, k);
+        uint32_t rr = byte_of(ab, k);
+        int i = 0;
+        while (rr) {
+            const int r = __builtin_ctz(rr);
+            rr &= rr - 1;
+            double *dst = &acc[__popcll(cm & ((1ull << (r * 8 + cl)) - 1ull))];
+            if constexpr (WIN)
+                atomicAdd(dst, ap[i++] * vb);
+            else
+                unsafeAtomicAdd(dst, ap[i++] * vb);
+        }
+    };
+    uint32_t mz = static_cast<uint32_t>(meta.z);
+#pragma unroll
+    for (int j = 0; j < kRecVals; ++j) {
+        if (mz) {
+            const int pos = __builtin_ctz(mz);
+            mz &= mz - 1;
+            entry(pos, rv[j]);
+        }
+    }
+    int j = kRecVals;
+    while (mz) {
+        const int pos = __builtin_ctz(mz);
+        mz &= mz - 1;
+        entry(pos, b_vals_km[boff + j]);
+        ++j;
+    }
+}
+
+// Per-chunk A tile info: {rows of column k (byte k), first value of column k
+// (byte k), first value relative to the chunk's, unused}.
+__device__ __forceinline__ int4 slab_a_tile(int4 am, int v0)
+{
+    const uint32_t ab = static_cast<uint32_t>(am.z);
+    const uint32_t p1 = __popc(byte_of(ab, 0));
+    const uint32_t p2 = p1 + __popc(byte_of(ab, 1));
+    const uint32_t p3 = p2 + __popc(byte_of(ab, 2));
+    return make_int4(static_cast<int>(ab), static_cast<int>((p1 << 8) | (p2 << 16) | (p3 << 24)),
+                     static_cast<int>(static_cast<uint32_t>(am.w) & ~kDenseBit) - v0, 0);
+}
 
 __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
     Operands g, const int4 *__restrict__ recs, int64_t rec_n, const int *__restrict__ rows,
     int nrows, int *__restrict__ queue, int static_rows, int4 *__restrict__ slabs, COut c)
 {
+    __shared__ double s_win[kSlabWin];
     __shared__ double s_acol[kSlabStage + 8];
-    __shared__ int4 s_at[kThreads];   // {rows of column k (byte k), first value of column k (byte k), first value, unused}
-    __shared__ int s_pw[kThreads];
-    __shared__ int s_pre[kThreads];   // first pair of each chunk tile
+    __shared__ int4 s_at[kThreads];
+    __shared__ int s_pw[kSlabRuns * kThreads];    // per (tile, run): first B tile minus first pair
+    __shared__ int s_pre[kSlabRuns * kThreads];   // per (tile, run): first pair
     __shared__ int s_tmp[kWavesPerBlock];
     int4 *slab = slabs + static_cast<int64_t>(blockIdx.x) * g.b_tile_cols;
     const int tid = threadIdx.x;
+    for (int i = tid; i < kSlabWin; i += kThreads)
+        s_win[i] = 0.0;
     for (int step = 0;; ++step) {
         const int ri = static_rows ? static_cast<int>(blockIdx.x) + step * static_cast<int>(gridDim.x)
                                    : next_row(queue);
@@ -1395,6 +1519,8 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
         const int nt = static_cast<int>(c.tile_ptr[row + 1] - t0);
         const int64_t zb = c.row_nnz_off[row];
         const int64_t nnz_row = c.row_nnz_off[row + 1] - zb;
+        const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
+        const bool windowed = a1 - a0 <= kThreads;
         // Columns are checked against the table's extent, so that a damaged
         // structure cannot send a store outside it.
         for (int q = tid; q < nt; q += kThreads) {
@@ -1405,104 +1531,144 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
                                     static_cast<int>(static_cast<uint32_t>(m)),
                                     static_cast<int>(static_cast<uint32_t>(m >> 32)));
         }
-        for (int64_t i = tid; i < nnz_row; i += kThreads)
-            c.val[zb + i] = 0.0;
+        if (!windowed)
+            for (int64_t i = tid; i < nnz_row; i += kThreads)
+                c.val[zb + i] = 0.0;
         __threadfence();
         __syncthreads();
         double *cval = c.val + zb;
-        const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
-        for (int c0 = a0; c0 < a1; c0 += kThreads) {
-            const int n = min(kThreads, a1 - c0);
-            const int v0 = g.a_off[c0];
-            const int span = g.a_off[c0 + n] - v0;
+        if (windowed) {
+            // One chunk: the A tiles, their staged values and their run
+            // cursors stay put for every window of the row.
+            const int n = a1 - a0;
+            const int v0 = g.a_off[a0];
+            const int span = g.a_off[a0 + n] - v0;
             const bool staged = span <= kSlabStage;
-            int nb = 0, bstart = 0;
+            int cur[kSlabRuns], end[kSlabRuns];
+#pragma unroll
+            for (int r = 0; r < kSlabRuns; ++r)
+                cur[r] = end[r] = 0;
             if (tid < n) {
-                const int4 am = g.a_meta[c0 + tid];
-                const int4 br = g.a_brange[c0 + tid];
+                const int4 am = g.a_meta[a0 + tid];
+                s_at[tid] = slab_a_tile(am, v0);
                 const uint32_t ab = static_cast<uint32_t>(am.z);
-                const uint32_t p1 = __popc(byte_of(ab, 0));
-                const uint32_t p2 = p1 + __popc(byte_of(ab, 1));
-                const uint32_t p3 = p2 + __popc(byte_of(ab, 2));
-                s_at[tid] = make_int4(static_cast<int>(ab),
-                                      static_cast<int>((p1 << 8) | (p2 << 16) | (p3 << 24)),
-                                      static_cast<int>(static_cast<uint32_t>(am.w) & ~kDenseBit) - v0,
-                                      0);
-                nb = br.z;
-                bstart = br.x;
+                const int *gp = g.b_groups + static_cast<int64_t>(kClassSlots) * am.x;
+#pragma unroll
+                for (int r = 0; r < kSlabRuns; ++r) {
+                    // Light run k meets only A tiles with entries in column k.
+                    if (r == 0 || byte_of(ab, r - 1) != 0) {
+                        cur[r] = gp[r];
+                        end[r] = gp[r + 1];
+                    }
+                }
             }
             if (staged)
                 for (int i = tid; i < span; i += kThreads)
                     s_acol[i] = g.a_vals_cm[v0 + i];
             const double *abase = staged ? s_acol : g.a_vals_cm + v0;
-            int total = 0;
-            const int pre = block_exclusive_scan(nb, s_tmp, &total);
-            s_pw[tid] = bstart - pre;
-            s_pre[tid] = pre;
-            __syncthreads();
-            // A pair finds its A tile by a binary search over the chunk's
-            // first pairs: a tile whose B row is a hub column would make one
-            // thread fill most of a segment table.
-            {
+            const int64_t *tnz = c.tile_nnz + t0;
+            // Every thread finds the same window bounds, so they need no exchange.
+            int t_lo = 0;
+            for (int64_t wend = kSlabWinStep; t_lo < nt; wend += kSlabWinStep) {
+                const int t_hi = first_tile_from(tnz, t_lo, nt, zb + wend);
+                if (t_hi == t_lo)
+                    continue;
+                const int vb = static_cast<int>(tnz[t_lo] - zb);
+                const int vs = static_cast<int>(tnz[t_hi] - zb) - vb;
+                const int jhi = c.col[t0 + t_hi - 1] + 1;
+                // B tiles before the window's first column that a cursor still
+                // covers are not C tiles of the row: they meet nothing.
+                int cnt[kSlabRuns], lo[kSlabRuns];
+                int mine = 0;
+#pragma unroll
+                for (int r = 0; r < kSlabRuns; ++r) {
+                    lo[r] = cur[r];
+                    cnt[r] = 0;
+                    if (cur[r] < end[r] && g.b_meta[cur[r]].x < jhi) {
+                        const int h = gallop_col(g.b_meta, cur[r], end[r], jhi);
+                        cnt[r] = h - cur[r];
+                        cur[r] = h;
+                    }
+                    mine += cnt[r];
+                }
+                int total = 0;
+                int pre = block_exclusive_scan(mine, s_tmp, &total);
+#pragma unroll
+                for (int r = 0; r < kSlabRuns; ++r) {
+                    s_pre[kSlabRuns * tid + r] = pre;
+                    s_pw[kSlabRuns * tid + r] = lo[r] - pre;
+                    pre += cnt[r];
+                }
+                __syncthreads();
+                for (int p = tid; p < total; p += kThreads) {
+                    const int e = upper_index(s_pre, kSlabRuns * n, p);
+                    const int4 at = s_at[e / kSlabRuns];
+                    const int64_t b = s_pw[e] + p;
+                    const int4 meta = recs[b];
+                    if (!product_count(static_cast<uint32_t>(at.x), static_cast<uint32_t>(meta.z)))
+                        continue;
+                    if (meta.x < 0 || meta.x >= g.b_tile_cols)
+                        continue;
+                    const int4 sl = slab[meta.x];
+                    if (sl.x != row || static_cast<unsigned>(sl.y - vb) >= static_cast<unsigned>(vs))
+                        continue;
+                    const uint64_t cm = (static_cast<uint64_t>(static_cast<uint32_t>(sl.w)) << 32) |
+                                        static_cast<uint32_t>(sl.z);
+                    slab_products<true>(recs, rec_n, g.b_vals_km, b, meta, at, abase, cm,
+                                        s_win + (sl.y - vb));
+                }
+                __syncthreads();
+                // The window's values, every one written (zeros included), and
+                // the accumulator cleared for the next window by the thread
+                // that read it.
+                for (int i = tid; i < vs; i += kThreads) {
+                    cval[vb + i] = s_win[i];
+                    s_win[i] = 0.0;
+                }
+                __syncthreads();
+                t_lo = t_hi;
+            }
+        } else {
+            for (int c0 = a0; c0 < a1; c0 += kThreads) {
+                const int n = min(kThreads, a1 - c0);
+                const int v0 = g.a_off[c0];
+                const int span = g.a_off[c0 + n] - v0;
+                const bool staged = span <= kSlabStage;
+                int nb = 0, bstart = 0;
+                if (tid < n) {
+                    s_at[tid] = slab_a_tile(g.a_meta[c0 + tid], v0);
+                    const int4 br = g.a_brange[c0 + tid];
+                    nb = br.z;
+                    bstart = br.x;
+                }
+                if (staged)
+                    for (int i = tid; i < span; i += kThreads)
+                        s_acol[i] = g.a_vals_cm[v0 + i];
+                const double *abase = staged ? s_acol : g.a_vals_cm + v0;
+                int total = 0;
+                const int pre = block_exclusive_scan(nb, s_tmp, &total);
+                s_pw[tid] = bstart - pre;
+                s_pre[tid] = pre;
+                __syncthreads();
+                // A pair finds its A tile by a binary search over the chunk's
+                // first pairs: a tile whose B row is a hub column would make
+                // one thread fill most of a segment table.
                 for (int p = tid; p < total; p += kThreads) {
                     const int v = upper_index(s_pre, n, p);
                     const int4 at = s_at[v];
                     const int64_t b = s_pw[v] + p;
                     const int4 meta = recs[b];
-                    const uint32_t ab = static_cast<uint32_t>(at.x);
-                    if (!product_count(ab, static_cast<uint32_t>(meta.z)))
+                    if (!product_count(static_cast<uint32_t>(at.x), static_cast<uint32_t>(meta.z)))
                         continue;
                     if (meta.x < 0 || meta.x >= g.b_tile_cols)
                         continue;
                     const int4 sl = slab[meta.x];
                     if (sl.x != row)
                         continue;
-                    // Only the parts that hold the tile's values: most tiles of
-                    // the rows that come here have one or two entries.
-                    const int nbv = __popc(static_cast<uint32_t>(meta.z));
-                    double rv[kRecVals];
-#pragma unroll
-                    for (int i = 0; i < kRecVals / 2; ++i) {
-                        double2 d = make_double2(0.0, 0.0);
-                        if (2 * i < nbv)
-                            d = reinterpret_cast<const double2 *>(recs + (i + 1) * rec_n)[b];
-                        rv[2 * i] = d.x;
-                        rv[2 * i + 1] = d.y;
-                    }
                     const uint64_t cm = (static_cast<uint64_t>(static_cast<uint32_t>(sl.w)) << 32) |
                                         static_cast<uint32_t>(sl.z);
-                    double *acc = cval + sl.y;
-                    const double *acol = abase + at.z;
-                    const uint32_t apf = static_cast<uint32_t>(at.y);
-                    const uint32_t boff = static_cast<uint32_t>(meta.w) & ~kDenseBit;
-                    auto entry = [&](int pos, double vb) {
-                        const int k = pos >> 3, cl = pos & 7;
-                        const double *ap = acol + byte_of(apf, k);
-                        uint32_t rr = byte_of(ab, k);
-                        int i = 0;
-                        while (rr) {
-                            const int r = __builtin_ctz(rr);
-                            rr &= rr - 1;
-                            unsafeAtomicAdd(&acc[__popcll(cm & ((1ull << (r * 8 + cl)) - 1ull))],
-                                            ap[i++] * vb);
-                        }
-                    };
-                    uint32_t mz = static_cast<uint32_t>(meta.z);
-#pragma unroll
-                    for (int j = 0; j < kRecVals; ++j) {
-                        if (mz) {
-                            const int pos = __builtin_ctz(mz);
-                            mz &= mz - 1;
-                            entry(pos, rv[j]);
-                        }
-                    }
-                    int j = kRecVals;
-                    while (mz) {
-                        const int pos = __builtin_ctz(mz);
-                        mz &= mz - 1;
-                        entry(pos, g.b_vals_km[boff + j]);
-                        ++j;
-                    }
+                    slab_products<false>(recs, rec_n, g.b_vals_km, b, meta, at, abase, cm,
+                                         cval + sl.y);
                 }
                 __syncthreads();
             }
@@ -1679,6 +1845,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     g.b_vals_km = b.payload_kmajor();
     g.b_trp = b.tile_row_ptr.as<int>();
     g.b_meta = b.records.as<int4>();   // part 0 of every record is the tile's meta
+    g.b_groups = b.row_groups.as<int>();
     g.b_vals = b.payload_values();
     g.b_tile_cols = BC;
 
@@ -1915,7 +2082,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
                            s, g, b.records.as<int4>(), b.num_tiles, w->lds_rows.as<int>(),
                            w->row_level.as<int>(), st, co, Timeline{});
     if (n_num_slab > 0) {
-        const int blocks = std::min(n_num_slab, 6 * num_cus);
+        const int blocks = std::min(n_num_slab, 4 * num_cus);
         DevBuf &tab = w->slab;   // reused: the symbolic slab pass has been queued before
         const int64_t ents = static_cast<int64_t>(blocks) * BC;
         RTT_TRY(tab.ensure(static_cast<size_t>(ents) * sizeof(int4)));
