@@ -1372,20 +1372,23 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
 // C tile in one load.  A's values are staged per chunk of A tiles in LDS
 // when they fit; B's come from the record planes.
 //
-// A row whose A tiles fit one chunk is walked in windows of consecutive C
-// tiles that start within kSlabWinStep values of each other (at most
-// kSlabWin values): a window's products accumulate in LDS and the window is
-// then stored to C with plain coalesced stores, so its values take no HBM
-// atomics and no clearing pass.  Each A tile keeps a cursor into every class
-// run of its B row (each run is sorted by column) and gallops it to the
-// window's last column, about one load per run and window.  Rows with more
-// A tiles than a chunk clear their C values and add every product to them
-// with an FP64 atomic in HBM.
+// Each chunk of A tiles walks the row in windows of consecutive C tiles
+// that start within kSlabWinStep values of each other (at most kSlabWin
+// values).  A window's products accumulate in LDS.  Each A tile keeps a
+// cursor into every class run of its B row (each run is sorted by column)
+// and gallops it to the window's last column, about one load per run and
+// window.  A row of one chunk then stores each window to C with plain
+// coalesced stores (no clearing pass, no HBM atomics); a row of up to
+// kSlabWinTiles A tiles clears its C values first and each chunk adds its
+// windows' nonzero sums with FP64 atomics on consecutive addresses.  Longer
+// rows (hub rows, whose chunks would each walk every window) clear C and
+// add every product with an FP64 atomic in HBM.
 // ---------------------------------------------------------------------------
 constexpr int kSlabStage = 1024;   // A values a slab chunk stages in LDS
 constexpr int kSlabWin = 2048;     // C values a slab window accumulates in LDS
 constexpr int kSlabWinStep = kSlabWin - 64;   // tile starts per window; a tile holds <= 64
 constexpr int kSlabRuns = 5;       // class runs of a B row: heavy, light k = 0..3
+constexpr int kSlabWinTiles = 4096;  // A tiles up to which a row is walked in windows
 static_assert(kSlabRuns + 1 <= kClassSlots, "the runs' ends come from the class slots");
 
 // First index after a, within [a, end), whose B tile column is >= j, given
@@ -1520,7 +1523,11 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
         const int64_t zb = c.row_nnz_off[row];
         const int64_t nnz_row = c.row_nnz_off[row + 1] - zb;
         const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
-        const bool windowed = a1 - a0 <= kThreads;
+        // A row of one chunk stores its windows; a row of a few chunks clears
+        // C first and each chunk adds its windows' sums; a longer row, where
+        // chunks times windows would outweigh its pairs, adds every product.
+        const bool single = a1 - a0 <= kThreads;
+        const bool windowed = a1 - a0 <= kSlabWinTiles;
         // Columns are checked against the table's extent, so that a damaged
         // structure cannot send a store outside it.
         for (int q = tid; q < nt; q += kThreads) {
@@ -1531,25 +1538,26 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
                                     static_cast<int>(static_cast<uint32_t>(m)),
                                     static_cast<int>(static_cast<uint32_t>(m >> 32)));
         }
-        if (!windowed)
+        if (!single)
             for (int64_t i = tid; i < nnz_row; i += kThreads)
                 c.val[zb + i] = 0.0;
         __threadfence();
         __syncthreads();
         double *cval = c.val + zb;
-        if (windowed) {
-            // One chunk: the A tiles, their staged values and their run
-            // cursors stay put for every window of the row.
-            const int n = a1 - a0;
-            const int v0 = g.a_off[a0];
-            const int span = g.a_off[a0 + n] - v0;
+        const int64_t *tnz = c.tile_nnz + t0;
+        for (int c0 = a0; windowed && c0 < a1; c0 += kThreads) {
+            // The chunk's A tiles, their staged values and their run cursors
+            // stay put for every window of the row.
+            const int n = min(kThreads, a1 - c0);
+            const int v0 = g.a_off[c0];
+            const int span = g.a_off[c0 + n] - v0;
             const bool staged = span <= kSlabStage;
             int cur[kSlabRuns], end[kSlabRuns];
 #pragma unroll
             for (int r = 0; r < kSlabRuns; ++r)
                 cur[r] = end[r] = 0;
             if (tid < n) {
-                const int4 am = g.a_meta[a0 + tid];
+                const int4 am = g.a_meta[c0 + tid];
                 s_at[tid] = slab_a_tile(am, v0);
                 const uint32_t ab = static_cast<uint32_t>(am.z);
                 const int *gp = g.b_groups + static_cast<int64_t>(kClassSlots) * am.x;
@@ -1566,7 +1574,6 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
                 for (int i = tid; i < span; i += kThreads)
                     s_acol[i] = g.a_vals_cm[v0 + i];
             const double *abase = staged ? s_acol : g.a_vals_cm + v0;
-            const int64_t *tnz = c.tile_nnz + t0;
             // Every thread finds the same window bounds, so they need no exchange.
             int t_lo = 0;
             for (int64_t wend = kSlabWinStep; t_lo < nt; wend += kSlabWinStep) {
@@ -1600,6 +1607,11 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
                     pre += cnt[r];
                 }
                 __syncthreads();
+                // A window with no pairs in a longer row's chunk adds nothing.
+                if (total == 0 && !single) {
+                    t_lo = t_hi;
+                    continue;
+                }
                 for (int p = tid; p < total; p += kThreads) {
                     const int e = upper_index(s_pre, kSlabRuns * n, p);
                     const int4 at = s_at[e / kSlabRuns];
@@ -1618,60 +1630,64 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
                                         s_win + (sl.y - vb));
                 }
                 __syncthreads();
-                // The window's values, every one written (zeros included), and
-                // the accumulator cleared for the next window by the thread
-                // that read it.
+                // The window's values, and the accumulator cleared for the next
+                // window by the thread that read it.  A single-chunk row
+                // stores every value (zeros included); a longer row adds the
+                // nonzero sums, consecutive addresses per wave.
                 for (int i = tid; i < vs; i += kThreads) {
-                    cval[vb + i] = s_win[i];
+                    const double v = s_win[i];
                     s_win[i] = 0.0;
+                    if (single)
+                        cval[vb + i] = v;
+                    else if (v != 0.0)
+                        unsafeAtomicAdd(&cval[vb + i], v);
                 }
                 __syncthreads();
                 t_lo = t_hi;
             }
-        } else {
-            for (int c0 = a0; c0 < a1; c0 += kThreads) {
-                const int n = min(kThreads, a1 - c0);
-                const int v0 = g.a_off[c0];
-                const int span = g.a_off[c0 + n] - v0;
-                const bool staged = span <= kSlabStage;
-                int nb = 0, bstart = 0;
-                if (tid < n) {
-                    s_at[tid] = slab_a_tile(g.a_meta[c0 + tid], v0);
-                    const int4 br = g.a_brange[c0 + tid];
-                    nb = br.z;
-                    bstart = br.x;
-                }
-                if (staged)
-                    for (int i = tid; i < span; i += kThreads)
-                        s_acol[i] = g.a_vals_cm[v0 + i];
-                const double *abase = staged ? s_acol : g.a_vals_cm + v0;
-                int total = 0;
-                const int pre = block_exclusive_scan(nb, s_tmp, &total);
-                s_pw[tid] = bstart - pre;
-                s_pre[tid] = pre;
-                __syncthreads();
-                // A pair finds its A tile by a binary search over the chunk's
-                // first pairs: a tile whose B row is a hub column would make
-                // one thread fill most of a segment table.
-                for (int p = tid; p < total; p += kThreads) {
-                    const int v = upper_index(s_pre, n, p);
-                    const int4 at = s_at[v];
-                    const int64_t b = s_pw[v] + p;
-                    const int4 meta = recs[b];
-                    if (!product_count(static_cast<uint32_t>(at.x), static_cast<uint32_t>(meta.z)))
-                        continue;
-                    if (meta.x < 0 || meta.x >= g.b_tile_cols)
-                        continue;
-                    const int4 sl = slab[meta.x];
-                    if (sl.x != row)
-                        continue;
-                    const uint64_t cm = (static_cast<uint64_t>(static_cast<uint32_t>(sl.w)) << 32) |
-                                        static_cast<uint32_t>(sl.z);
-                    slab_products<false>(recs, rec_n, g.b_vals_km, b, meta, at, abase, cm,
-                                         cval + sl.y);
-                }
-                __syncthreads();
+        }
+        for (int c0 = a0; !windowed && c0 < a1; c0 += kThreads) {
+            const int n = min(kThreads, a1 - c0);
+            const int v0 = g.a_off[c0];
+            const int span = g.a_off[c0 + n] - v0;
+            const bool staged = span <= kSlabStage;
+            int nb = 0, bstart = 0;
+            if (tid < n) {
+                s_at[tid] = slab_a_tile(g.a_meta[c0 + tid], v0);
+                const int4 br = g.a_brange[c0 + tid];
+                nb = br.z;
+                bstart = br.x;
             }
+            if (staged)
+                for (int i = tid; i < span; i += kThreads)
+                    s_acol[i] = g.a_vals_cm[v0 + i];
+            const double *abase = staged ? s_acol : g.a_vals_cm + v0;
+            int total = 0;
+            const int pre = block_exclusive_scan(nb, s_tmp, &total);
+            s_pw[tid] = bstart - pre;
+            s_pre[tid] = pre;
+            __syncthreads();
+            // A pair finds its A tile by a binary search over the chunk's
+            // first pairs: a tile whose B row is a hub column would make one
+            // thread fill most of a segment table.
+            for (int p = tid; p < total; p += kThreads) {
+                const int v = upper_index(s_pre, n, p);
+                const int4 at = s_at[v];
+                const int64_t b = s_pw[v] + p;
+                const int4 meta = recs[b];
+                if (!product_count(static_cast<uint32_t>(at.x), static_cast<uint32_t>(meta.z)))
+                    continue;
+                if (meta.x < 0 || meta.x >= g.b_tile_cols)
+                    continue;
+                const int4 sl = slab[meta.x];
+                if (sl.x != row)
+                    continue;
+                const uint64_t cm = (static_cast<uint64_t>(static_cast<uint32_t>(sl.w)) << 32) |
+                                    static_cast<uint32_t>(sl.z);
+                slab_products<false>(recs, rec_n, g.b_vals_km, b, meta, at, abase, cm,
+                                     cval + sl.y);
+            }
+            __syncthreads();
         }
         __threadfence();
         __syncthreads();

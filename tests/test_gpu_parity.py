@@ -224,6 +224,34 @@ def test_wide_rows_take_the_slab_paths(gpu, aat):
     assert stats["heavy_rows"] > 0
 
 
+@pytest.mark.parametrize("aat", [0, 1])
+def test_slab_row_beyond_the_window_walk(gpu, aat):
+    # The first eight rows alternate over 65536 columns, so the first tile
+    # row holds 16384 A tiles: more than the slab numeric pass walks in LDS
+    # windows, so its products go to C as HBM atomics. The other rows hold
+    # two entries each and stay small. In A*A^T the first rows meet over
+    # 32768 columns; values from {0.5, 1, 1.5} keep every such sum exact in
+    # FP64 whatever its order, so the check stays at the usual tolerance and
+    # a lost or repeated product still shows.
+    n = 65536
+    rng = np.random.default_rng(5)
+    rows, cols = [], []
+    for r in range(8):
+        c = np.arange(r % 2, n, 2)
+        rows.append(np.full(len(c), r))
+        cols.append(c)
+    i = np.arange(8, n)
+    rows += [i, i]
+    cols += [i, (7 * i) % n]
+    rows, cols = np.concatenate(rows), np.concatenate(cols)
+    keep = np.unique(rows.astype(np.int64) * n + cols, return_index=True)[1]
+    rows, cols = rows[keep], cols[keep]
+    a = csr_from(n, n, rows, cols, rng.choice([0.5, 1.0, 1.5], len(rows)))
+    pa, _, _, stats, _ = run(gpu, a, aat, positive=True)
+    assert stats["heavy_rows"] > 0
+    assert pa.tiles()["tile_row_ptr"][1] > 8192
+
+
 @pytest.mark.parametrize("scale,aat", [(12, 0), (14, 0), (12, 1)])
 def test_rmat_power_law(gpu, scale, aat):
     # R-MAT rows follow a power law: hub rows with thousands of entries give
