@@ -767,6 +767,34 @@ __device__ __forceinline__ int load_slab_chunk(const Operands &g, int c0, int n,
     return total;
 }
 
+// The slab kernels walk a row in windows: each A tile keeps a cursor into
+// every class run of its B row, each run sorted by column.
+constexpr int kSlabRuns = 5;       // class runs of a B row: heavy, light k = 0..3
+constexpr int kSlabWinTiles = 4096;  // A tiles up to which a slab row is walked in windows
+static_assert(kSlabRuns + 1 <= kClassSlots, "the runs' ends come from the class slots");
+constexpr int kSymWin = 2048;      // B tile columns a symbolic slab window covers
+
+// First index after a, within [a, end), whose B tile column is >= j, given
+// that a's column is below j (end when none): doubling steps from a, then a
+// binary search in the last step.
+__device__ __forceinline__ int gallop_col(const int4 *__restrict__ meta, int a, int end, int j)
+{
+    int step = 1;
+    while (a + step < end && meta[a + step].x < j) {
+        a += step;
+        step <<= 1;
+    }
+    int lo = a + 1, hi = min(a + step, end);
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (meta[mid].x < j)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 // The count pass leaves each row's (column, mask) list, in column order, in
 // a grow-only staging buffer when it fits (row_off[row] = its first entry,
 // -1 otherwise); the fill pass then copies the list instead of repeating
@@ -790,10 +818,15 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
     __shared__ int s_b0[kThreads];
     __shared__ int s_pre[kThreads];
     __shared__ int s_tmp[kWavesPerBlock];
-    __shared__ int s_lo, s_hi;
+    __shared__ int s_lo, s_hi, s_min;
     __shared__ long long s_base;
+    __shared__ unsigned long long s_wm[kSymWin];   // a window's masks
+    __shared__ int s_wpw[kSlabRuns * kThreads];    // per (tile, run): first B tile minus first pair
+    __shared__ int s_wpre[kSlabRuns * kThreads];   // per (tile, run): first pair
     unsigned long long *slab = slabs + static_cast<int64_t>(blockIdx.x) * g.b_tile_cols;
     const int tid = threadIdx.x;
+    for (int i = tid; i < kSymWin; i += kThreads)
+        s_wm[i] = 0;
     for (int step = 0;; ++step) {
         const int ri = static_rows ? static_cast<int>(blockIdx.x) + step * static_cast<int>(gridDim.x)
                                    : next_row(queue);
@@ -829,12 +862,98 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
         if (tid == 0) {
             s_lo = INT_MAX;
             s_hi = -1;
+            s_min = INT_MAX;
         }
         __syncthreads();
         // The column range the row's masks land in, from the pairs
         // themselves (the tiles of a B row are not in column order).
         int jlo = INT_MAX, jhi = -1;
-        for (int c0 = a0; c0 < a1; c0 += kThreads) {
+        const bool windowed = a1 - a0 <= kSlabWinTiles;
+        // Windowed: each chunk's A tiles walk their B runs in windows of
+        // kSymWin columns, starting each window at the lowest column a
+        // cursor holds.  A window's masks are ORed in LDS, then its nonzero
+        // masks into the HBM table at consecutive addresses.
+        for (int c0 = a0; windowed && c0 < a1; c0 += kThreads) {
+            const int n = min(kThreads, a1 - c0);
+            int cur[kSlabRuns], end[kSlabRuns];
+#pragma unroll
+            for (int r = 0; r < kSlabRuns; ++r)
+                cur[r] = end[r] = 0;
+            if (tid < n) {
+                const int4 am = g.a_meta[c0 + tid];
+                const uint32_t ab = static_cast<uint32_t>(am.z);
+                s_ab[tid] = ab;
+                const int *gp = g.b_groups + static_cast<int64_t>(kClassSlots) * am.x;
+#pragma unroll
+                for (int r = 0; r < kSlabRuns; ++r) {
+                    // Light run k meets only A tiles with entries in column k.
+                    if (r == 0 || byte_of(ab, r - 1) != 0) {
+                        cur[r] = gp[r];
+                        end[r] = gp[r + 1];
+                    }
+                }
+            }
+            for (;;) {
+                int mn = INT_MAX;
+#pragma unroll
+                for (int r = 0; r < kSlabRuns; ++r)
+                    if (cur[r] < end[r])
+                        mn = min(mn, g.b_meta[cur[r]].x);
+                if (mn != INT_MAX)
+                    atomicMin(&s_min, mn);
+                __syncthreads();
+                const int wlo = s_min;
+                if (wlo == INT_MAX)
+                    break;
+                const int whi = wlo + kSymWin;
+                int cnt[kSlabRuns], lo[kSlabRuns];
+                int mine = 0;
+#pragma unroll
+                for (int r = 0; r < kSlabRuns; ++r) {
+                    lo[r] = cur[r];
+                    cnt[r] = 0;
+                    if (cur[r] < end[r] && g.b_meta[cur[r]].x < whi) {
+                        const int h = gallop_col(g.b_meta, cur[r], end[r], whi);
+                        cnt[r] = h - cur[r];
+                        cur[r] = h;
+                    }
+                    mine += cnt[r];
+                }
+                int total = 0;
+                int pre = block_exclusive_scan(mine, s_tmp, &total);
+#pragma unroll
+                for (int r = 0; r < kSlabRuns; ++r) {
+                    s_wpre[kSlabRuns * tid + r] = pre;
+                    s_wpw[kSlabRuns * tid + r] = lo[r] - pre;
+                    pre += cnt[r];
+                }
+                if (tid == 0)
+                    s_min = INT_MAX;   // every thread read it before the scan's barriers
+                __syncthreads();
+                for (int p = tid; p < total; p += kThreads) {
+                    const int e = upper_index(s_wpre, kSlabRuns * n, p);
+                    const int4 bm = g.b_meta[s_wpw[e] + p];
+                    const uint64_t pm = bytes_product(s_ab[e / kSlabRuns], static_cast<uint32_t>(bm.z));
+                    if (pm && bm.x >= wlo && bm.x < whi)
+                        atomicOr(&s_wm[bm.x - wlo], static_cast<unsigned long long>(pm));
+                }
+                __syncthreads();
+                for (int i = tid; i < kSymWin; i += kThreads) {
+                    const unsigned long long m = s_wm[i];
+                    if (m) {
+                        s_wm[i] = 0;
+                        atomicOr(&slab[wlo + i], m);
+                        jlo = min(jlo, wlo + i);
+                        jhi = max(jhi, wlo + i);
+                    }
+                }
+                __syncthreads();
+            }
+            // Every thread has read the final (empty) window start before the
+            // next chunk lowers it again.
+            __syncthreads();
+        }
+        for (int c0 = a0; !windowed && c0 < a1; c0 += kThreads) {
             const int n = min(kThreads, a1 - c0);
             const int total = load_slab_chunk(g, c0, n, s_ab, s_b0, s_pre, s_tmp);
             for (int p = tid; p < total; p += kThreads) {
@@ -1387,30 +1506,6 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
 constexpr int kSlabStage = 1024;   // A values a slab chunk stages in LDS
 constexpr int kSlabWin = 2048;     // C values a slab window accumulates in LDS
 constexpr int kSlabWinStep = kSlabWin - 64;   // tile starts per window; a tile holds <= 64
-constexpr int kSlabRuns = 5;       // class runs of a B row: heavy, light k = 0..3
-constexpr int kSlabWinTiles = 4096;  // A tiles up to which a row is walked in windows
-static_assert(kSlabRuns + 1 <= kClassSlots, "the runs' ends come from the class slots");
-
-// First index after a, within [a, end), whose B tile column is >= j, given
-// that a's column is below j (end when none): doubling steps from a, then a
-// binary search in the last step.
-__device__ __forceinline__ int gallop_col(const int4 *__restrict__ meta, int a, int end, int j)
-{
-    int step = 1;
-    while (a + step < end && meta[a + step].x < j) {
-        a += step;
-        step <<= 1;
-    }
-    int lo = a + 1, hi = min(a + step, end);
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (meta[mid].x < j)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
 
 // First tile in [lo, hi] of a row whose first value is at or after v (hi
 // when none); nnz holds the row's tile starts (hi + 1 entries from the row's
