@@ -747,26 +747,6 @@ __device__ __forceinline__ unsigned long long load_coherent(const unsigned long 
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Chunk of up to kThreads A tiles for the slab kernels (binary search).
-__device__ __forceinline__ int load_slab_chunk(const Operands &g, int c0, int n, uint32_t *s_ab,
-                                               int *s_b0, int *s_pre, int *s_tmp)
-{
-    const int t = threadIdx.x;
-    int nb = 0;
-    if (t < n) {
-        const int4 br = g.a_brange[c0 + t];
-        nb = br.z;
-        s_ab[t] = static_cast<uint32_t>(g.a_meta[c0 + t].z);
-        s_b0[t] = br.x;
-    }
-    int total = 0;
-    const int excl = block_exclusive_scan(nb, s_tmp, &total);
-    if (t < n)
-        s_pre[t] = excl;
-    __syncthreads();
-    return total;
-}
-
 // The slab kernels walk a row in windows: each A tile keeps a cursor into
 // every class run of its B row, each run sorted by column.
 constexpr int kSlabRuns = 5;       // class runs of a B row: heavy, light k = 0..3
@@ -815,8 +795,6 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
     int64_t *__restrict__ row_nnz, SlabStage ss, COut c)
 {
     __shared__ uint32_t s_ab[kThreads];
-    __shared__ int s_b0[kThreads];
-    __shared__ int s_pre[kThreads];
     __shared__ int s_tmp[kWavesPerBlock];
     __shared__ int s_lo, s_hi, s_min;
     __shared__ long long s_base;
@@ -868,12 +846,12 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
         // The column range the row's masks land in, from the pairs
         // themselves (the tiles of a B row are not in column order).
         int jlo = INT_MAX, jhi = -1;
-        const bool windowed = a1 - a0 <= kSlabWinTiles;
-        // Windowed: each chunk's A tiles walk their B runs in windows of
-        // kSymWin columns, starting each window at the lowest column a
-        // cursor holds.  A window's masks are ORed in LDS, then its nonzero
-        // masks into the HBM table at consecutive addresses.
-        for (int c0 = a0; windowed && c0 < a1; c0 += kThreads) {
+        // Each chunk's A tiles walk their B runs in windows of kSymWin
+        // columns, starting each window at the lowest column a cursor holds
+        // (so a chunk visits only windows it has pairs in).  A window's masks
+        // are ORed in LDS, then its nonzero masks into the HBM table at
+        // consecutive addresses.
+        for (int c0 = a0; c0 < a1; c0 += kThreads) {
             const int n = min(kThreads, a1 - c0);
             int cur[kSlabRuns], end[kSlabRuns];
 #pragma unroll
@@ -951,21 +929,6 @@ __global__ __launch_bounds__(kThreads) void symbolic_slab_kernel(
             }
             // Every thread has read the final (empty) window start before the
             // next chunk lowers it again.
-            __syncthreads();
-        }
-        for (int c0 = a0; !windowed && c0 < a1; c0 += kThreads) {
-            const int n = min(kThreads, a1 - c0);
-            const int total = load_slab_chunk(g, c0, n, s_ab, s_b0, s_pre, s_tmp);
-            for (int p = tid; p < total; p += kThreads) {
-                const int ai = upper_index(s_pre, n, p);
-                const int4 bm = g.b_meta[s_b0[ai] + (p - s_pre[ai])];
-                const uint64_t pm = bytes_product(s_ab[ai], static_cast<uint32_t>(bm.z));
-                if (pm) {
-                    atomicOr(&slab[bm.x], static_cast<unsigned long long>(pm));
-                    jlo = min(jlo, bm.x);
-                    jhi = max(jhi, bm.x);
-                }
-            }
             __syncthreads();
         }
         if (jhi >= 0) {
