@@ -37,13 +37,22 @@ WORKLOADS = {
     "laplace2d-64-aa": dict(kind="laplace2d", g=64, aat=0),
     "rmat-20-aa": dict(kind="rmat", scale=20, aat=0),
     "rmat-18-aa": dict(kind="rmat", scale=18, aat=0),
+    "fem-500k-aa": dict(kind="fem", nodes=500000, aat=0),
+    "fem-100k-aa": dict(kind="fem", nodes=100000, aat=0),
 }
+FP64_MFMA_PEAK_TFS = 78.6
+MFMA_PAIR_FLOPS = 512   # v_mfma_f64_4x4x4f64: 4 blocks x 4x4x4 x 2
 
 
 def make_matrix(rt, spec):
     if spec["kind"] == "stencil27":
         return rt.gen_stencil27(spec["g"], 0.5, 1.5, 20260101), (
             f"3D 27-point stencil {spec['g']}^3, values U[0.5,1.5], seed 20260101")
+    if spec["kind"] == "fem":
+        return rt.gen_fem_blocks(spec["nodes"], 6, True, 20260101), (
+            f"block FEM, {spec['nodes']} nodes uniform in the unit cube, symmetrised 6 nearest "
+            f"neighbours + self, dense 4x4 blocks, magnitudes 10^U[-3,3] with random signs, "
+            f"Morton node numbering, seed 20260101")
     if spec["kind"] == "rmat":
         return rt.gen_rmat(spec["scale"], 16, 0.57, 0.19, 0.19, 20260101), (
             f"R-MAT 2^{spec['scale']}, 16 draws per row, (a,b,c,d) = (0.57,0.19,0.19,0.05), "
@@ -248,6 +257,9 @@ def main():
     ap.add_argument("--control-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-second-mode", action="store_true",
                     help="skip the R-MAT 2^20 C = A*A measurement reported beside the main line")
+    ap.add_argument("--numeric-path", default="auto", choices=["auto", "vector", "tile"],
+                    help="numeric kernels: LDS accumulator (vector), FP64 matrix cores per "
+                         "tile pair (tile), or chosen by A's tile fill (auto)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (traffic-counter scale)")
     args = ap.parse_args()
@@ -273,7 +285,7 @@ def main():
     rt.init(0)
     pa = rt.prepare_bgrf_a(a) if args.layout == "bgrf" else rt.prepare_identity_a(a)
     nnz_cub = rt.nnz_cub(pa, aat)
-    sched = rt.schedule(args.lb_mode, materialize_output=False)
+    sched = rt.schedule(args.lb_mode, materialize_output=False, numeric_path=args.numeric_path)
     pb = None
     for _ in range(args.warmup):
         pb = rt.rebuild_b(pa, aat, pb)
@@ -305,11 +317,15 @@ def main():
     #  symbolic_fill  C structure of the rows the LDS kernels cannot hold
     #                 (none on the structured workloads)
     #  symbolic_count reads A and B structure, writes per-row counts
+    #  On the tile path the structure pass writes all of C's structure and
+    #  the numeric kernel reads A and B and writes C's values.
     m_b = a.n if aat else a.m
     a_struct = 4 * (a.m + 1) + 4 * a.nnz
     b_struct = 4 * (m_b + 1) + 4 * a.nnz
+    tile_path = st["numeric_path"] == rt.NUMERIC_PATHS["tile"]
     alg = {
-        "numeric": csr_bytes(a.m, a.nnz) + csr_bytes(m_b, a.nnz) + csr_bytes(a.m, nnz_c),
+        "numeric": csr_bytes(a.m, a.nnz) + csr_bytes(m_b, a.nnz) + (
+            8 * nnz_c if tile_path else csr_bytes(a.m, nnz_c)),
         "symbolic_fill": a_struct + b_struct + 4 * (a.m + 1) + 4 * nnz_c,
         "symbolic_count": a_struct + b_struct + 8 * (a.m + 1),
     }
@@ -324,8 +340,16 @@ def main():
         "numeric_ms": float(np.median([s[2]["numeric_ms"] for s in steps])),
         "kernel_ms": {k: round(float(v), 4) for k, v in kern.items()},
     }
+    mfma = None
+    if tile_path:
+        flops = MFMA_PAIR_FLOPS * st["mfma_pairs"]
+        mfma = {"pairs": st["mfma_pairs"], "flops": flops,
+                "utilisation": round(flops / (kern["numeric"] * 1e-3) / (FP64_MFMA_PEAK_TFS * 1e12), 5),
+                "useful_fraction": round(2.0 * nnz_cub / max(flops, 1), 4),
+                "peak": f"{FP64_MFMA_PEAK_TFS} TFLOP/s FP64 MFMA"}
+        breakdown["b_columns_ms"] = float(np.median([s[2]["b_columns_ms"] for s in steps]))
     print(json.dumps({"breakdown": breakdown, "nnz_cub": nnz_cub, "nnz_c": nnz_c,
-                      "tiles_c": st["output_tiles"], "pairs": st["pair_count"]}),
+                      "tiles_c": st["output_tiles"], "pairs": st["pair_count"], "mfma": mfma}),
           file=sys.stderr, flush=True)
     control = finish_control(control, nnz_cub, nnz_c)
     cpu = None
@@ -347,6 +371,7 @@ def main():
         "config": {"workload": f"{args.workload}: C = A*A{'^T' if aat else ''}, {desc}, "
                                f"{a.m} rows, {a.nnz} nnz, nnzCub {nnz_cub}, nnzC {nnz_c}",
                    "precision": "plain FP64", "lb_mode": args.lb_mode,
+                   "numeric_path": "tile" if tile_path else "vector",
                    "layout": args.layout,
                    "pipeline_hbm_frac": round(b_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                               5)},
@@ -355,7 +380,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": None if traffic is None else int(traffic),
                      "algorithmic_bytes": int(alg[dom]),
-                     "traffic_source": traffic_src},
+                     "traffic_source": traffic_src,
+                     "mfma": mfma},
         "cpu_baseline": cpu,
         "control": control,
         "second_mode": other,

@@ -334,15 +334,32 @@ enum rtt_lb_mode {               /* --lb-mode (src/main.cu:132-166) */
     RTT_LB_ADAPTIVE_TAIL = 2
 };
 
+/* Numeric kernels of a call.  VECTOR: one workgroup per A tile row adds
+ * every scalar product into an LDS (or, for very wide rows, HBM) copy of the
+ * row's output with FP64 atomics.  TILE: C's structure is written first, then
+ * every 8x8 C tile sums its (A tile, B tile) pairs in increasing K on the
+ * FP64 matrix cores (v_mfma_f64_4x4x4f64, one instruction per pair), so its
+ * values are bit-reproducible.  AUTO takes TILE when A's tiles hold at least
+ * tile_fill_threshold stored values on average (the reference's DMMA path
+ * multiplies every matched tile pair on tensor cores, src/dmma_spgemm.h:
+ * 4176-4246; sparse tiles pay there for slots they do not use). */
+enum rtt_numeric_path {
+    RTT_NUMERIC_AUTO = 0,
+    RTT_NUMERIC_VECTOR = 1,
+    RTT_NUMERIC_TILE = 2
+};
+
 typedef struct rtt_schedule {    /* DmmaNumericScheduleConfig subset */
     int lb_mode;                 /* enum rtt_lb_mode; default adaptive-tail */
     int materialize_output;      /* export C tiles to the host SMatrix */
     double split_threshold;      /* heavy admission cost (16384) */
     double split_chunk_target;   /* cost per chunk (8192) */
-    int split_max_chunks;        /* 8 */
-    double max_heavy_fraction;   /* 0.01 */
+    int split_max_chunks;        /* 8; valid 2..32 */
+    double max_heavy_fraction;   /* 0.01; valid 0..1 */
     int tail_record_capacity;    /* 1 << 20 */
     int force_tail_split;        /* admit every multi-step task (tests) */
+    int numeric_path;            /* enum rtt_numeric_path; default AUTO */
+    double tile_fill_threshold;  /* AUTO: stored values per A tile for TILE (12) */
 } rtt_schedule;
 
 /* Fills *s with the reference defaults (src/main.cu:61-130). */
@@ -361,9 +378,14 @@ typedef struct rtt_stats {       /* DmmaSpGemmStats subset */
     unsigned long long pair_count;     /* (A tile, B tile) products visited */
     unsigned long long output_tiles;   /* 8x8 C tiles */
     unsigned long long output_nnz;     /* nnz(C) */
-    unsigned long long heavy_rows;     /* tile rows on the split (tail) path */
-    unsigned long long heavy_chunks;   /* chunks those rows were split into */
+    unsigned long long heavy_rows;     /* VECTOR: tile rows beyond the small LDS
+                                          budget (larger LDS kernel or HBM slab) */
+    unsigned long long heavy_chunks;   /* chunks of split heavy C tiles */
     int output_materialized;
+    int numeric_path;                  /* the path that ran (VECTOR or TILE) */
+    double b_columns_ms;               /* TILE: B's tile columns built for the call */
+    unsigned long long mfma_pairs;     /* TILE: tile pairs multiplied, one FP64 MFMA
+                                          (4 blocks x 4x4x4, 512 flops) each */
 } rtt_stats;
 
 /* dmma_tilespgemm (src/dmma_spgemm.h:6490): C = A * B with A 8x4 and B 4x8
@@ -424,6 +446,49 @@ int rtt_gen_rmat(int scale, int edge_factor, double a, double b, double c,
 /* Random m x n matrix with `per_row` distinct sorted columns per row drawn
  * uniformly, values U[-1, 1] (test helper). */
 int rtt_gen_random(int m, int n, int per_row, uint64_t seed, rtt_smatrix *out);
+
+/* Block-structured FEM-like matrix: `nodes` points uniform in the unit cube,
+ * each joined to its `neighbours` nearest points (exact, ties by index),
+ * symmetrised, plus itself; every joined pair (i, j) is a dense 4x4 block
+ * (rows 4i.., columns 4j..) whose entries have magnitudes 10^U[-3,3] and
+ * random signs.  spatial_order != 0 numbers the nodes along a Morton curve
+ * (a mesher's locality numbering); 0 keeps the draw order. */
+int rtt_gen_fem_blocks(int nodes, int neighbours, int spatial_order, uint64_t seed,
+                       rtt_smatrix *out);
+/* Banded n x n matrix: row i holds min(per_row, band width) distinct columns
+ * drawn uniformly from [i - bandwidth, i + bandwidth], plus the diagonal;
+ * values U[-1, 1]. */
+int rtt_gen_banded(int n, int bandwidth, int per_row, uint64_t seed, rtt_smatrix *out);
+/* Shapes of a batch of banded problems: rows round(exp(N(ln median, sigma)))
+ * clipped to [min_rows, max_rows], bandwidth U{min_bw..max_bw}, entries per
+ * row U{min_per_row..max_per_row}. */
+int rtt_gen_banded_batch_shapes(int problems, double median_rows, double sigma, int min_rows,
+                                int max_rows, int min_bw, int max_bw, int min_per_row,
+                                int max_per_row, uint64_t seed, int *rows, int *bandwidth,
+                                int *per_row);
+/* 3D 7-point Poisson operator on g^3 cells (row = x + g*y + g*g*z) with cell
+ * coefficients 10^U[-decades, decades]: a face between cells i and j weighs
+ * (k_i + k_j) / 2, a boundary face adds k_i to the diagonal (Dirichlet). */
+int rtt_gen_poisson7_jumps(int g, double decades, uint64_t seed, rtt_smatrix *out);
+/* Smoothed-aggregation prolongator P = (I - omega D^-1 A) T of a g^3 grid
+ * operator A, with T the 2x2x2 cell aggregation onto ceil(g/2)^3 aggregates. */
+int rtt_gen_aggregation_prolongator(const rtt_smatrix *a, int g, double omega,
+                                    rtt_smatrix *out);
+/* Host transpose of a CSR matrix (rows increasing within each column). */
+int rtt_csr_transpose(const rtt_smatrix *in, rtt_smatrix *out);
+
+/* ------------------------------------------------------------------------ */
+/* Batches                                                                   */
+/* ------------------------------------------------------------------------ */
+/* Block-diagonal concatenation of `count` CSR matrices, every block starting
+ * at row and column offsets that are multiples of `align` (8 keeps every 8x4
+ * A, 4x8 B and 8x8 C tile inside one problem).  row_offsets / col_offsets
+ * (count + 1 entries each, may be NULL) receive the block starts and the
+ * padded totals.  The product of two such concatenations with matching
+ * offsets is the concatenation of the problems' products, so one SpGEMM call
+ * runs the whole batch: one launch per stage over every problem's tiles. */
+int rtt_block_diag(int count, const rtt_smatrix *const *mats, int align, rtt_smatrix *out,
+                   int64_t *row_offsets, int64_t *col_offsets);
 
 #ifdef __cplusplus
 }

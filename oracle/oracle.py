@@ -65,6 +65,11 @@ def lib() -> C.CDLL:
     return _lib
 
 
+def ref_available() -> bool:
+    """True when the reference-source checker was built (without loading it)."""
+    return os.path.exists(REF_LIB)
+
+
 def ref_lib() -> C.CDLL | None:
     """Reference-source checker, or None when it could not be built."""
     global _ref

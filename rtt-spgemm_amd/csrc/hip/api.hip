@@ -1,11 +1,14 @@
 // extern "C" entry points of the device path: A preparation, B rebuild,
 // nnzCub, the SpGEMM core and the host export of C.
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
+#include <unordered_set>
 #include <vector>
 
 #include "internal.hpp"
@@ -27,12 +30,16 @@ struct rtt_prepared_a {
     rtt::Events events;
     rtt::DevC last_c;
     bool has_c = false;
+    uint64_t id = 0;          // unique per preparation (never reused in a process)
 };
 
+// A B is built for one prepared A: its per-A-tile pair extents are indexed by
+// that A's tiles and, for B = A^T at tile level, its payload is that A's.
 struct rtt_dynamic_b {
     rtt::DevTiles tiles;
     rtt::BuildWork build;
     int aat = 0;
+    uint64_t a_id = 0;        // rtt_prepared_a::id of the A it was built for
 };
 
 struct rtt_device_csr {
@@ -43,6 +50,44 @@ struct rtt_device_csr {
 namespace {
 
 using rtt::fail;
+
+// Identities of the prepared A handles alive in this process, so that a B
+// bound to a destroyed A is refused instead of dereferenced.
+std::atomic<uint64_t> g_next_a_id{1};
+std::mutex g_live_mu;
+std::unordered_set<uint64_t> g_live_a;
+
+uint64_t register_a()
+{
+    const uint64_t id = g_next_a_id.fetch_add(1);
+    std::lock_guard<std::mutex> lock(g_live_mu);
+    g_live_a.insert(id);
+    return id;
+}
+
+void unregister_a(uint64_t id)
+{
+    std::lock_guard<std::mutex> lock(g_live_mu);
+    g_live_a.erase(id);
+}
+
+bool a_alive(uint64_t id)
+{
+    std::lock_guard<std::mutex> lock(g_live_mu);
+    return g_live_a.count(id) != 0;
+}
+
+// B must have been built (rtt_rebuild_b / rtt_rebuild_b_general) for this A.
+int check_binding(const rtt_prepared_a *a, const rtt_dynamic_b *b, const char *who)
+{
+    if (b->a_id == 0)
+        return fail(RTT_ERR_INVALID, "%s: B has not been built", who);
+    if (b->a_id != a->id)
+        return fail(RTT_ERR_INVALID,
+                    "%s: B was built for another prepared A (a B is bound to the A passed to "
+                    "its rebuild)", who);
+    return RTT_OK;
+}
 
 int export_c(const rtt::DevC &c, rtt_smatrix *out)
 {
@@ -303,6 +348,7 @@ int rtt_prepare_external_a(const rtt_smatrix *host, const int *row_new_to_old,
     st.total_ms = rtt::ms_since(t_total);
     if (stats)
         *stats = st;
+    a->id = register_a();
     *out = a.release();
     return RTT_OK;
 }
@@ -346,6 +392,8 @@ void rtt_destroy_prepared_a(rtt_prepared_a *a)
 {
     if (a && rtt::ctx().stream)
         (void)hipStreamSynchronize(rtt::ctx().stream);
+    if (a)
+        unregister_a(a->id);
     delete a;
 }
 
@@ -366,6 +414,7 @@ int rtt_rebuild_b(const rtt_prepared_a *a_const, int aat, rtt_dynamic_b **inout,
     }
     rtt_dynamic_b *b = *inout;
     b->aat = aat;
+    b->a_id = 0;   // unbound until the rebuild completes
     rtt_b_update_stats st;
     std::memset(&st, 0, sizeof(st));
     auto t0 = rtt::Clock::now();
@@ -394,6 +443,7 @@ int rtt_rebuild_b(const rtt_prepared_a *a_const, int aat, rtt_dynamic_b **inout,
     }
     RTT_TRY(rtt::build_pair_extents(a->tiles, &b->tiles));
     RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
+    b->a_id = a->id;
     st.total_ms = rtt::ms_since(t0);
     st.csr2tile_ms = st.total_ms - st.csr_row_permute_ms;
     if (stats)
@@ -455,6 +505,7 @@ int rtt_rebuild_b_general(const rtt_prepared_a *a_const, const rtt_device_csr *s
     }
     rtt_dynamic_b *b = *inout;
     b->aat = 0;
+    b->a_id = 0;
     rtt_b_update_stats st;
     std::memset(&st, 0, sizeof(st));
     st.path = 3;
@@ -469,6 +520,7 @@ int rtt_rebuild_b_general(const rtt_prepared_a *a_const, const rtt_device_csr *s
     RTT_TRY(rtt::order_by_class(&b->tiles, &b->build));
     RTT_TRY(rtt::build_pair_extents(a_const->tiles, &b->tiles));
     RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
+    b->a_id = a_const->id;
     st.total_ms = rtt::ms_since(t0);
     st.csr2tile_ms = st.total_ms;
     if (stats)
@@ -499,6 +551,8 @@ int rtt_update_b_values(const rtt_prepared_a *a_const, const rtt_device_csr *src
     RTT_TRY(rtt::require_ctx());
     if (!a_const || !src || !b)
         return fail(RTT_ERR_INVALID, "rtt_update_b_values: NULL argument");
+    if (b->a_id != 0)
+        RTT_TRY(check_binding(a_const, b, "rtt_update_b_values"));
     const rtt::DevTiles &t = b->tiles;
     const bool valid = t.map_source_id != 0 && t.map_source_id == src->csr.structure_id &&
                        t.map_nnz == src->csr.nnz && !t.transpose_of &&
@@ -538,6 +592,9 @@ int rtt_dynamic_b_view(const rtt_dynamic_b *b_const, rtt_device_tiles *view)
     // pointer + packed per-tile records); the remaining DmmaDeviceTiles
     // arrays and the tile CSC are completed here, on first request.
     rtt_dynamic_b *b = const_cast<rtt_dynamic_b *>(b_const);
+    if (b->tiles.transpose_of && !a_alive(b->a_id))
+        return fail(RTT_ERR_INVALID,
+                    "rtt_dynamic_b_view: the A this B = A^T was built from has been destroyed");
     RTT_TRY(rtt::materialize_tiles(&b->tiles, &b->build));
     b->tiles.view(view);
     return RTT_OK;
@@ -577,6 +634,8 @@ void rtt_schedule_defaults(rtt_schedule *s)
     s->max_heavy_fraction = 0.01;
     s->tail_record_capacity = 1 << 20;
     s->force_tail_split = 0;
+    s->numeric_path = RTT_NUMERIC_AUTO;
+    s->tile_fill_threshold = 12.0;
 }
 
 int rtt_tilespgemm(const rtt_prepared_a *a_const, const rtt_dynamic_b *b, rtt_smatrix *output,
@@ -586,12 +645,27 @@ int rtt_tilespgemm(const rtt_prepared_a *a_const, const rtt_dynamic_b *b, rtt_sm
     RTT_TRY(rtt::require_ctx());
     if (!a_const || !b || !stats)
         return fail(RTT_ERR_INVALID, "rtt_tilespgemm: NULL argument");
+    RTT_TRY(check_binding(a_const, b, "rtt_tilespgemm"));
     rtt_prepared_a *a = const_cast<rtt_prepared_a *>(a_const);
     rtt_schedule defaults;
     rtt_schedule_defaults(&defaults);
     const rtt_schedule &sc = sched ? *sched : defaults;
     if (sc.lb_mode < RTT_LB_STATIC || sc.lb_mode > RTT_LB_ADAPTIVE_TAIL)
         return fail(RTT_ERR_INVALID, "unknown lb_mode %d", sc.lb_mode);
+    // The tail planner's own validity conditions (dmma_plan_tile_tail_queue,
+    // src/dmma_tile_tail_queue.h:190-292; max_chunks 2..32).
+    if (!(sc.split_threshold > 0.0) || !(sc.split_chunk_target > 0.0) ||
+        sc.split_max_chunks < 2 || sc.split_max_chunks > 32 || !(sc.max_heavy_fraction >= 0.0) ||
+        !(sc.max_heavy_fraction <= 1.0) || sc.tail_record_capacity <= 0)
+        return fail(RTT_ERR_INVALID,
+                    "bad tail schedule: threshold %g, chunk target %g, max chunks %d, heavy "
+                    "fraction %g, record capacity %d",
+                    sc.split_threshold, sc.split_chunk_target, sc.split_max_chunks,
+                    sc.max_heavy_fraction, sc.tail_record_capacity);
+    if (sc.numeric_path < RTT_NUMERIC_AUTO || sc.numeric_path > RTT_NUMERIC_TILE ||
+        !(sc.tile_fill_threshold >= 0.0))
+        return fail(RTT_ERR_INVALID, "unknown numeric path %d (fill threshold %g)",
+                    sc.numeric_path, sc.tile_fill_threshold);
     // The same shape contract as dmma_tilespgemm (src/dmma_spgemm.h:6495-6510).
     if (a->tiles.tile_col_count != b->tiles.tile_row_count || a->tiles.tile_rows != rtt::kTileM ||
         a->tiles.tile_cols != rtt::kTileK || b->tiles.tile_rows != rtt::kTileK ||
@@ -603,8 +677,27 @@ int rtt_tilespgemm(const rtt_prepared_a *a_const, const rtt_dynamic_b *b, rtt_sm
     auto t0 = rtt::Clock::now();
     rtt::CoreTimes times;
     a->has_c = false;
+    int path = sc.numeric_path;
+    if (path == RTT_NUMERIC_AUTO) {
+        const double fill = a->tiles.num_tiles > 0 ? static_cast<double>(a->tiles.payload) /
+                                                         static_cast<double>(a->tiles.num_tiles)
+                                                   : 0.0;
+        path = fill >= sc.tile_fill_threshold && a->tiles.num_tiles > 0 ? RTT_NUMERIC_TILE
+                                                                        : RTT_NUMERIC_VECTOR;
+    }
+    rtt::TileColumns bc;
+    if (path == RTT_NUMERIC_TILE) {
+        rtt_dynamic_b *bm = const_cast<rtt_dynamic_b *>(b);
+        auto tc = rtt::Clock::now();
+        RTT_TRY(rtt::build_b_columns(&bm->tiles, &bm->build));
+        RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
+        stats->b_columns_ms = rtt::ms_since(tc);
+        bc = rtt::b_columns(b->tiles);
+    }
     RTT_TRY(rtt::spgemm_core(a->tiles, b->tiles, &a->last_c, &a->core, &a->events, &times,
-                            sc.lb_mode));
+                            sc.lb_mode, path == RTT_NUMERIC_TILE ? 1 : 0, &bc));
+    stats->numeric_path = path;
+    stats->mfma_pairs = times.mfma_pairs;
     a->has_c = true;
     stats->symbolic_ms = times.symbolic_ms;
     stats->allocation_ms = times.allocation_ms;

@@ -81,7 +81,8 @@ __global__ void validate_csr_kernel(int rows, int cols, int64_t nnz,
         atomicOr(flags, 1);
     if (i < rows) {
         const int b = rp[i], e = rp[i + 1];
-        if (e < b) {
+        // Bounds first: a row outside [0, nnz] is flagged, never walked.
+        if (b < 0 || e < b || static_cast<int64_t>(e) > nnz) {
             atomicOr(flags, 1);
         } else {
             for (int x = b + 1; x < e; ++x)
@@ -805,6 +806,61 @@ int order_by_class(DevTiles *b, BuildWork *w)
     return RTT_OK;
 }
 
+// Tile columns of B for the tile-pair numeric path: the tiles of each tile
+// column J in increasing tile-row order, each as {tile row K, mask, per-k
+// byte sets, payload offset | dense bit}.  A stable sort of the (class
+// ordered) tiles by column keeps rows ascending within a column.
+__global__ void column_keys_kernel(int64_t n, const int4 *__restrict__ meta, int *__restrict__ keys)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < n)
+        keys[t] = meta[t].x;
+}
+
+__global__ void column_entries_kernel(int64_t n, const int *__restrict__ ids,
+                                      const int *__restrict__ row_of,
+                                      const int4 *__restrict__ meta, int4 *__restrict__ out)
+{
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    const int t = ids[p];
+    const int4 m = meta[t];
+    out[p] = make_int4(row_of[t], m.y, m.z, m.w);
+}
+
+int build_b_columns(DevTiles *b, BuildWork *w)
+{
+    if (b->transpose_of || b->cols_ready)
+        return RTT_OK;   // B = A^T reads A's tile rows as its columns
+    hipStream_t s = ctx().stream;
+    const int64_t n = b->num_tiles;
+    const int BC = b->tile_col_count;
+    RTT_TRY(b->col_ptr.ensure((static_cast<size_t>(BC) + 1) * sizeof(int)));
+    RTT_TRY(b->col_ent.ensure((n > 0 ? n : 1) * sizeof(int4)));
+    if (n > 0) {
+        RTT_TRY(fill_tile_row_of(b));
+        RTT_TRY(w->keys_in.ensure(n * sizeof(int)));
+        RTT_TRY(w->keys_out.ensure(n * sizeof(int)));
+        RTT_TRY(w->ids_in.ensure(n * sizeof(int)));
+        RTT_TRY(w->ids_out.ensure(n * sizeof(int)));
+        hipLaunchKernelGGL(column_keys_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
+                           n, b->meta.as<int4>(), w->keys_in.as<int>());
+        hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
+                           w->ids_in.as<int>(), n);
+        RTT_TRY(sort_pairs(w->keys_in.as<int>(), w->keys_out.as<int>(), w->ids_in.as<int>(),
+                           w->ids_out.as<int>(), n, bits_for(BC), &w->scan_tmp, s));
+        hipLaunchKernelGGL(column_entries_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0,
+                           s, n, w->ids_out.as<int>(), b->tile_row_of.as<int>(),
+                           b->meta.as<int4>(), b->col_ent.as<int4>());
+    }
+    hipLaunchKernelGGL(row_ptr_from_sorted_kernel, dim3(blocks_for(n + 1, kThreads)),
+                       dim3(kThreads), 0, s, w->keys_out.as<int>(), n, BC, b->col_ptr.as<int>());
+    RTT_HIP(hipGetLastError());
+    b->cols_ready = true;
+    return RTT_OK;
+}
+
 // Thread per A tile: its payload re-packed column-major.  Entry (r, k) goes
 // to off + (rows of columns < k) + (rows < r in column k).
 __global__ void colmajor_values_kernel(int64_t n, const int4 *__restrict__ meta,
@@ -992,6 +1048,7 @@ int build_tiles(const DevCsr &src, const int *row_map, int active_rows, int acti
     out->tile_col_count = (active_cols + TC - 1) / TC;
     out->has_csc = false;
     out->arrays_ready = true;
+    out->cols_ready = false;
     out->transpose_of = nullptr;
     hipStream_t s = ctx().stream;
     int2 *src_slot = nullptr;
@@ -1051,6 +1108,7 @@ int transpose_tiles(const DevTiles &a, int b_cols, DevTiles *b, BuildWork *w)
     b->transpose_of = &a;
     b->arrays_ready = false;
     b->has_csc = false;
+    b->cols_ready = false;
     const int R = b->tile_row_count;
     const int64_t slots = static_cast<int64_t>(kClassSlots) * R;
     RTT_TRY(b->tile_row_ptr.ensure((R + 1) * sizeof(int)));

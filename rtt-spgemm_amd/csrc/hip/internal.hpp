@@ -102,6 +102,7 @@ struct DevTiles {
     int tile_row_count = 0, tile_col_count = 0;
     int64_t num_tiles = 0, payload = 0, dense = 0, sparse = 0;
     bool has_csc = false;       // tile_col_ptr / tile_row_idx / csc_tile_ids valid
+    bool cols_ready = false;    // col_ptr / col_ent valid (B built from a CSR)
     bool arrays_ready = true;   // tile_col_idx / masks / value_offsets / values valid
     DevBuf tile_row_ptr, tile_col_idx, value_offsets, masks, values;
     DevBuf tile_col_ptr, tile_row_idx, csc_tile_ids;
@@ -112,6 +113,9 @@ struct DevTiles {
     DevBuf a_blight;      // B only: per A tile, {first, count} of the light tiles it can meet
     DevBuf pair_total;    // B only: u64, sum over A tiles of their B row's tile count
     DevBuf row_groups;    // B only: start of each (row, class) run, kClassSlots per row
+    DevBuf col_ptr;       // B only: tile-column pointer of col_ent (tile_col_count + 1)
+    DevBuf col_ent;       // B only: per tile column, its tiles in tile-row order as
+                          // {tile row K, mask, per-k byte sets, offset | dense bit}
     DevBuf values_cm;     // each tile's values with the stride-4 bit groups in order:
                           // A (8x4): column-major; B (4x8): k-major (row k's entries
                           // contiguous).  Packed from the tile's offset.  For B built as
@@ -164,6 +168,8 @@ struct SpgemmWork {
     // call's start, never while a fill pass may still read the buffer).
     DevBuf slab_stage_j, slab_stage_m, slab_row_off;
     uint64_t slab_stage_cap = 0, slab_stage_need = 0;
+    DevBuf tile_queue;                      // row-queue heads of the tile-pair kernels
+    DevBuf tile_pairs;                      // u64: pairs the tile-pair kernel multiplied
 };
 
 // Scratch of the tile builders, reused across calls.
@@ -230,22 +236,46 @@ int build_records(DevTiles *t);
 // b->a_brange[t] = {B.tile_row_ptr[K], heavy tiles, count} and b->a_blight[t]
 // for every A tile t in column K.
 int build_pair_extents(const DevTiles &a, DevTiles *b);
+// Builds b->col_ptr / b->col_ent (tile columns, tile rows ascending) of a B
+// built from a CSR; no-op for B = A^T at tile level, whose columns are A's
+// tile rows (b_columns below).
+int build_b_columns(DevTiles *b, BuildWork *w);
 // Reorders b->meta and b->records of a B built from CSR into class order
 // within every row (stable) and fills b->row_groups.
 int order_by_class(DevTiles *b, BuildWork *w);
+
+// ---- tile-pair numeric path (tiletask.hip) ---------------------------------
+// B's tile columns as the tile-pair kernels read them: ptr[J]..ptr[J+1] index
+// ent, entries {tile row K, mask, per-k bytes, payload offset | dense bit} in
+// increasing K, payload `vals` (each tile column-major).
+struct TileColumns {
+    const int *ptr = nullptr;
+    const int4 *ent = nullptr;
+    const double *vals = nullptr;
+};
+TileColumns b_columns(const DevTiles &b);
+// Values of every C tile from C's finished structure (numeric path 1: FP64
+// matrix cores).  Deterministic: each tile sums its pairs in increasing K.
+int tile_numeric(const DevTiles &a, const TileColumns &bc, const DevC &c, int precision,
+                 SpgemmWork *w, hipStream_t s);
 
 // ---- SpGEMM core (spgemm.hip) ---------------------------------------------
 struct CoreTimes {
     double symbolic_ms = 0, allocation_ms = 0, numeric_ms = 0;
     double sym_count_kernel_ms = 0, sym_fill_kernel_ms = 0, numeric_kernel_ms = 0;
     unsigned long long pairs = 0, heavy_rows = 0;
+    unsigned long long mfma_pairs = 0;   // tile path: (A tile, B tile) products on MFMA
 };
 // lb_mode (enum rtt_lb_mode) assigns the rows of the slab kernels, which
 // hold one workgroup per row for many rows: static hands row i to
 // workgroup i mod grid; dynamic takes rows from a work queue in list order;
 // adaptive-tail takes them from the queue heaviest first (most pairs).
+// numeric_path 0 runs the LDS-accumulator kernels; 1 writes C's structure
+// for every row first and computes the values with the tile-pair kernel
+// (bc: B's tile columns).
 int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *work,
-                Events *events, CoreTimes *times, int lb_mode = RTT_LB_ADAPTIVE_TAIL);
+                Events *events, CoreTimes *times, int lb_mode = RTT_LB_ADAPTIVE_TAIL,
+                int numeric_path = 0, const TileColumns *bc = nullptr);
 
 // ---- scans (scan.hip) -------------------------------------------------------
 // Exclusive sum of n int64 values into out[0..n] (out[n] = total).

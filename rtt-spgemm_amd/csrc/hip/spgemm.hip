@@ -645,7 +645,7 @@ __global__ __launch_bounds__(kThreads) void symbolic_write_kernel(
     const int idx = blockIdx.x * kWavesPerBlock + w;
     if (idx >= count)
         return;
-    const int row = rows[idx];
+    const int row = rows ? rows[idx] : idx;
     if (row_level[row] != kLevelLds)
         return;
     const int64_t tb = c.tile_ptr[row];
@@ -1894,8 +1894,10 @@ static int order_rows_by_cost(const int *rows, int n, const Operands &g, SpgemmW
 }
 
 int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Events *ev,
-                CoreTimes *times, int lb_mode)
+                CoreTimes *times, int lb_mode, int numeric_path, const TileColumns *bc)
 {
+    if (numeric_path != 0 && !bc)
+        return fail(RTT_ERR_INVALID, "tile-pair numeric path without B's tile columns");
     RTT_TRY(require_ctx());
     RTT_TRY(ev->create());
     hipStream_t s = ctx().stream;
@@ -2094,8 +2096,12 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     RTT_HIP(hipEventRecord(ev->ev[2], s));
     // The numeric LDS kernels write the structure of their own rows from the
     // staged lists; this pass writes it for the staged rows they cannot hold
-    // (the numeric fallback list, whose slab rows it skips).
-    if (n_num_slab > 0 && c->num_tiles > 0)
+    // (the numeric fallback list, whose slab rows it skips).  The tile-pair
+    // path needs the whole structure first: every staged row.
+    if (numeric_path != 0 && R > 0 && c->num_tiles > 0)
+        hipLaunchKernelGGL(symbolic_write_kernel, dim3(blocks_for(R, kWavesPerBlock)),
+                           dim3(kThreads), 0, s, nullptr, R, st, w->row_level.as<int>(), co);
+    else if (n_num_slab > 0 && c->num_tiles > 0)
         hipLaunchKernelGGL(symbolic_write_kernel, dim3(blocks_for(n_num_slab, kWavesPerBlock)),
                            dim3(kThreads), 0, s, w->num_rows.as<int>(), n_num_slab, st,
                            w->row_level.as<int>(), co);
@@ -2139,7 +2145,10 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
         }
     }
     RTT_HIP(hipEventRecord(ev->ev[4], s));
-    if (R > 0 && c->nnz > 0) {
+    if (numeric_path != 0) {
+        RTT_TRY(tile_numeric(a, *bc, *c, numeric_path - 1, w, s));
+        RTT_HIP(hipEventRecord(ev->ev[5], s));
+    } else if (R > 0 && c->nnz > 0) {
         if (trace_slots)
             hipLaunchKernelGGL((numeric_row_kernel<SmallRow, true>), dim3(R), dim3(kThreads), 0,
                                s, g, b.records.as<int4>(), b.num_tiles, nullptr,
@@ -2150,12 +2159,13 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
                                w->row_level.as<int>(), st, co, tl);
     }
     RTT_HIP(hipGetLastError());
-    RTT_HIP(hipEventRecord(ev->ev[5], s));
-    if (n_num_lds > 0)
+    if (numeric_path == 0)
+        RTT_HIP(hipEventRecord(ev->ev[5], s));
+    if (numeric_path == 0 && n_num_lds > 0)
         hipLaunchKernelGGL((numeric_row_kernel<LargeRow>), dim3(n_num_lds), dim3(kThreads), 0,
                            s, g, b.records.as<int4>(), b.num_tiles, w->lds_rows.as<int>(),
                            w->row_level.as<int>(), st, co, Timeline{});
-    if (n_num_slab > 0) {
+    if (numeric_path == 0 && n_num_slab > 0) {
         const int blocks = std::min(n_num_slab, 4 * num_cus);
         DevBuf &tab = w->slab;   // reused: the symbolic slab pass has been queued before
         const int64_t ents = static_cast<int64_t>(blocks) * BC;
@@ -2173,6 +2183,9 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     }
     RTT_HIP(hipEventRecord(ev->ev[7], s));
     RTT_HIP(hipStreamSynchronize(s));
+    if (numeric_path != 0 && c->num_tiles > 0)
+        RTT_HIP(hipMemcpy(&times->mfma_pairs, w->tile_pairs.ptr, sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost));
     if (trace_slots)
         RTT_TRY(write_timeline(trace_path, tl, trace_slots, static_cast<uint64_t>(R),
                                static_cast<uint64_t>(c->num_tiles), num_cus));

@@ -68,6 +68,15 @@ int alloc_csr(int m, int n, int64_t nnz, rtt_smatrix *out)
     return RTT_OK;
 }
 
+} // namespace
+
+int rtt::alloc_host_csr(int m, int n, int64_t nnz, rtt_smatrix *out)
+{
+    return alloc_csr(m, n, nnz, out);
+}
+
+namespace {
+
 // Matrix Market banner and size line (the NIST mmio conventions used by
 // src/mmio.h:398-600).
 struct MtxHeader {

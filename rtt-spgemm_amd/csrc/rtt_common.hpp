@@ -13,6 +13,10 @@ namespace rtt {
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();
 
+// Zeroes *out and mallocs its CSR arrays (m + 1 row pointers, nnz columns and
+// values); RTT_ERR_OVERFLOW beyond the int32 ABI, RTT_ERR_OOM on failure.
+int alloc_host_csr(int m, int n, int64_t nnz, rtt_smatrix *out);
+
 using Clock = std::chrono::steady_clock;
 inline double ms_since(Clock::time_point t0)
 {

@@ -107,7 +107,8 @@ class Schedule(C.Structure):
     _fields_ = [("lb_mode", C.c_int), ("materialize_output", C.c_int),
                 ("split_threshold", C.c_double), ("split_chunk_target", C.c_double),
                 ("split_max_chunks", C.c_int), ("max_heavy_fraction", C.c_double),
-                ("tail_record_capacity", C.c_int), ("force_tail_split", C.c_int)]
+                ("tail_record_capacity", C.c_int), ("force_tail_split", C.c_int),
+                ("numeric_path", C.c_int), ("tile_fill_threshold", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -117,7 +118,9 @@ class Stats(C.Structure):
                 ("symbolic_fill_kernel_ms", C.c_double), ("numeric_kernel_ms", C.c_double),
                 ("pair_count", C.c_ulonglong), ("output_tiles", C.c_ulonglong),
                 ("output_nnz", C.c_ulonglong), ("heavy_rows", C.c_ulonglong),
-                ("heavy_chunks", C.c_ulonglong), ("output_materialized", C.c_int)]
+                ("heavy_chunks", C.c_ulonglong), ("output_materialized", C.c_int),
+                ("numeric_path", C.c_int), ("b_columns_ms", C.c_double),
+                ("mfma_pairs", C.c_ulonglong)]
 
 
 class DeviceC(C.Structure):
@@ -190,6 +193,16 @@ def load() -> C.CDLL:
         "rtt_gen_rmat": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                    C.c_uint64, P(SMatrix)]),
         "rtt_gen_random": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, P(SMatrix)]),
+        "rtt_gen_fem_blocks": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, P(SMatrix)]),
+        "rtt_gen_banded": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, P(SMatrix)]),
+        "rtt_gen_banded_batch_shapes": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_int,
+                                                  C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                  C.c_uint64, vp, vp, vp]),
+        "rtt_gen_poisson7_jumps": (C.c_int, [C.c_int, C.c_double, C.c_uint64, P(SMatrix)]),
+        "rtt_gen_aggregation_prolongator": (C.c_int, [P(SMatrix), C.c_int, C.c_double,
+                                                      P(SMatrix)]),
+        "rtt_csr_transpose": (C.c_int, [P(SMatrix), P(SMatrix)]),
+        "rtt_block_diag": (C.c_int, [C.c_int, vp, C.c_int, P(SMatrix), vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -307,6 +320,69 @@ def gen_random(m: int, n: int, per_row: int, seed: int = 1) -> CsrMatrix:
     return _consume_csr(s)
 
 
+def gen_fem_blocks(nodes: int, neighbours: int = 6, spatial_order: bool = True,
+                   seed: int = 20260101) -> CsrMatrix:
+    s = SMatrix()
+    _check(load().rtt_gen_fem_blocks(nodes, neighbours, int(spatial_order), seed, C.byref(s)),
+           "gen_fem_blocks")
+    return _consume_csr(s)
+
+
+def gen_banded(n: int, bandwidth: int, per_row: int, seed: int = 1) -> CsrMatrix:
+    s = SMatrix()
+    _check(load().rtt_gen_banded(n, bandwidth, per_row, seed, C.byref(s)), "gen_banded")
+    return _consume_csr(s)
+
+
+def gen_banded_batch_shapes(problems: int, median_rows: float = 16000.0, sigma: float = 1.0,
+                            min_rows: int = 1000, max_rows: int = 200000, min_bw: int = 8,
+                            max_bw: int = 64, min_per_row: int = 5, max_per_row: int = 30,
+                            seed: int = 20260101):
+    """(rows, bandwidth, per_row) arrays of a banded batch."""
+    rows = np.zeros(max(problems, 1), np.int32)
+    bw = np.zeros_like(rows)
+    pr = np.zeros_like(rows)
+    _check(load().rtt_gen_banded_batch_shapes(problems, median_rows, sigma, min_rows, max_rows,
+                                              min_bw, max_bw, min_per_row, max_per_row, seed,
+                                              _ptr(rows), _ptr(bw), _ptr(pr)),
+           "gen_banded_batch_shapes")
+    return rows[:problems], bw[:problems], pr[:problems]
+
+
+def gen_poisson7_jumps(g: int, decades: float = 3.0, seed: int = 1) -> CsrMatrix:
+    s = SMatrix()
+    _check(load().rtt_gen_poisson7_jumps(g, decades, seed, C.byref(s)), "gen_poisson7_jumps")
+    return _consume_csr(s)
+
+
+def gen_aggregation_prolongator(a: CsrMatrix, g: int, omega: float = 2.0 / 3.0) -> CsrMatrix:
+    s = SMatrix()
+    src = a.as_smatrix()
+    _check(load().rtt_gen_aggregation_prolongator(C.byref(src), g, omega, C.byref(s)),
+           "gen_aggregation_prolongator")
+    return _consume_csr(s)
+
+
+def csr_transpose(a: CsrMatrix) -> CsrMatrix:
+    s = SMatrix()
+    src = a.as_smatrix()
+    _check(load().rtt_csr_transpose(C.byref(src), C.byref(s)), "csr_transpose")
+    return _consume_csr(s)
+
+
+def block_diag(mats, align: int = 8):
+    """Block-diagonal batch of CSR matrices; returns (CsrMatrix, row_offsets, col_offsets)."""
+    mats = list(mats)
+    sm = [m.as_smatrix() for m in mats]
+    arr = (C.POINTER(SMatrix) * max(len(sm), 1))(*[C.pointer(x) for x in sm])
+    ro = np.zeros(len(mats) + 1, np.int64)
+    co = np.zeros(len(mats) + 1, np.int64)
+    s = SMatrix()
+    _check(load().rtt_block_diag(len(mats), C.cast(arr, C.c_void_p), align, C.byref(s),
+                                 _ptr(ro), _ptr(co)), "block_diag")
+    return _consume_csr(s), ro, co
+
+
 # ---- tile matrices -------------------------------------------------------------
 
 @dataclass
@@ -404,10 +480,14 @@ class PreparedA:
         self.stats = {f: getattr(stats, f) for f, _ in OfflineAStats._fields_}
         self.m, self.n = csr.m, csr.n
 
-    def __del__(self):
+    def close(self) -> None:
+        """Destroys the device handle now (also done on garbage collection)."""
         if getattr(self, "handle", None) and _lib is not None:
             _lib.rtt_destroy_prepared_a(self.handle)
             self.handle = None
+
+    def __del__(self):
+        self.close()
 
     def tiles(self) -> dict:
         v = DeviceTiles()
@@ -641,11 +721,16 @@ def update_b_values(a: PreparedA, src: DeviceCsr, b: DynamicB,
     return out
 
 
-def schedule(lb_mode: str = "adaptive-tail", materialize_output: bool = True, **kw) -> Schedule:
+NUMERIC_PATHS = {"auto": 0, "vector": 1, "tile": 2}
+
+
+def schedule(lb_mode: str = "adaptive-tail", materialize_output: bool = True,
+             numeric_path: str = "auto", **kw) -> Schedule:
     s = Schedule()
     load().rtt_schedule_defaults(C.byref(s))
     s.lb_mode = LB_MODES[lb_mode]
     s.materialize_output = int(materialize_output)
+    s.numeric_path = NUMERIC_PATHS[numeric_path]
     for k, v in kw.items():
         setattr(s, k, v)
     return s

@@ -1,0 +1,108 @@
+"""GPU parity of the tile-pair numeric path (FP64 matrix cores, one
+v_mfma_f64_4x4x4f64 per (A tile, B tile) pair) against the CPU oracle:
+structure bit-exact, values within the 1e-14 forward-error bound, and the
+same bits on every call (each C tile sums its pairs in increasing K)."""
+import numpy as np
+import pytest
+
+import oracle
+from test_gpu_parity import as_tuple, check_c, oracle_product
+
+pytestmark = pytest.mark.gpu
+
+
+def tile_run(rt, a, aat, dense_threshold=24, positive=False, **kw):
+    pa = rt.prepare_identity_a(a, dense_threshold)
+    pb = rt.rebuild_b(pa, aat)
+    tc, stats = rt.tilespgemm(pa, pb, rt.schedule(numeric_path="tile", **kw))
+    assert stats["numeric_path"] == rt.NUMERIC_PATHS["tile"]
+    ref = oracle_product(a, aat)
+    err = check_c(rt, tc, ref, positive)
+    assert stats["output_nnz"] == ref["nnz"]
+    return pa, pb, tc, stats, err
+
+
+CASES = [
+    ("laplace64", lambda rt: rt.gen_laplace2d(64, 0.1, 20260101), 0, False),
+    ("stencil5", lambda rt: rt.gen_stencil27(5, 0.5, 1.5, 7), 1, True),
+    ("stencil12", lambda rt: rt.gen_stencil27(12, 0.5, 1.5, 8), 1, True),
+    ("random_aa", lambda rt: rt.gen_random(300, 300, 9, 31), 0, False),
+    ("random_aat", lambda rt: rt.gen_random(257, 190, 7, 32), 1, False),
+    ("fem_morton", lambda rt: rt.gen_fem_blocks(1500, 6, True, 77), 0, False),
+    ("fem_draw_order", lambda rt: rt.gen_fem_blocks(600, 6, False, 78), 0, False),
+    ("rmat12", lambda rt: rt.gen_rmat(12, 16, seed=5), 0, False),
+    ("rmat12_aat", lambda rt: rt.gen_rmat(12, 16, seed=6), 1, False),
+]
+
+
+@pytest.mark.parametrize("name,make,aat,positive", CASES, ids=[c[0] for c in CASES])
+def test_tile_path_matches_oracle(gpu, name, make, aat, positive):
+    print(f"case {name}", flush=True)
+    tile_run(gpu, make(gpu), aat, positive=positive)
+
+
+@pytest.mark.parametrize("dense_threshold", [1, 24, 32])
+def test_tile_path_dense_thresholds(gpu, dense_threshold):
+    """Dense (zero-filled, 32-slot) and sparse payloads decode the same."""
+    a = gpu.gen_random(200, 200, 20, 90 + dense_threshold)
+    tile_run(gpu, a, 0, dense_threshold=dense_threshold)
+    tile_run(gpu, a, 1, dense_threshold=dense_threshold)
+
+
+def test_tile_path_general_ab_rectangular(gpu):
+    a = gpu.gen_random(333, 170, 6, 41)
+    b = gpu.gen_random(170, 251, 5, 42)
+    pa = gpu.prepare_identity_a(a)
+    pb = gpu.rebuild_b_general(pa, gpu.upload_csr(b))
+    tc, stats = gpu.tilespgemm(pa, pb, gpu.schedule(numeric_path="tile"))
+    ref = oracle.spgemm(as_tuple(a), as_tuple(b))
+    check_c(gpu, tc, ref)
+
+
+def test_tile_path_is_bit_reproducible(gpu):
+    a = gpu.gen_fem_blocks(2000, 6, True, 5)
+    pa = gpu.prepare_identity_a(a)
+    pb = gpu.rebuild_b(pa, 0)
+    s = gpu.schedule(numeric_path="tile")
+    t1, _ = gpu.tilespgemm(pa, pb, s)
+    t2, _ = gpu.tilespgemm(pa, pb, s)
+    assert np.array_equal(t1.tile_csr_Value.view(np.uint64), t2.tile_csr_Value.view(np.uint64))
+    r = gpu.gen_rmat(13, 16, seed=9)
+    pr = gpu.prepare_identity_a(r)
+    pbr = gpu.rebuild_b(pr, 0)
+    u1, _ = gpu.tilespgemm(pr, pbr, s)
+    u2, _ = gpu.tilespgemm(pr, pbr, s)
+    assert np.array_equal(u1.tile_csr_Value.view(np.uint64), u2.tile_csr_Value.view(np.uint64))
+
+
+def test_auto_path_follows_tile_fill(gpu):
+    """AUTO takes the matrix cores for the block-FEM matrix (16..32 stored
+    values per A tile) and the LDS-accumulator kernels for the stencil
+    (about 6 per tile); both give the oracle's C."""
+    f = gpu.gen_fem_blocks(800, 6, True, 3)
+    pa = gpu.prepare_identity_a(f)
+    tc, st = gpu.tilespgemm(pa, gpu.rebuild_b(pa, 0))
+    assert st["numeric_path"] == gpu.NUMERIC_PATHS["tile"]
+    check_c(gpu, tc, oracle_product(f, 0))
+    s = gpu.gen_stencil27(8, 0.5, 1.5, 3)
+    ps = gpu.prepare_identity_a(s)
+    tc2, st2 = gpu.tilespgemm(ps, gpu.rebuild_b(ps, 1))
+    assert st2["numeric_path"] == gpu.NUMERIC_PATHS["vector"]
+    check_c(gpu, tc2, oracle_product(s, 1), positive=True)
+
+
+def test_tile_path_values_update(gpu):
+    """A values-only B update keeps B's tile columns (structure and payload
+    offsets) and the tile path reads the new values."""
+    a = gpu.gen_random(180, 150, 6, 61)
+    b = gpu.gen_random(150, 140, 5, 62)
+    pa = gpu.prepare_identity_a(a)
+    src = gpu.upload_csr(b)
+    pb = gpu.rebuild_b_general(pa, src)
+    s = gpu.schedule(numeric_path="tile")
+    gpu.tilespgemm(pa, pb, s)
+    b2 = gpu.CsrMatrix(b.m, b.n, b.rowptr, b.col, b.val * -2.5 + 0.25)
+    src.set_values(b2.val)
+    gpu.update_b_values(pa, src, pb, "safe-auto")
+    tc, _ = gpu.tilespgemm(pa, pb, s)
+    check_c(gpu, tc, oracle.spgemm(as_tuple(a), as_tuple(b2)))
