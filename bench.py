@@ -257,6 +257,8 @@ def main():
     ap.add_argument("--control-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-second-mode", action="store_true",
                     help="skip the R-MAT 2^20 C = A*A measurement reported beside the main line")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "split-bf16"],
+                    help="tile-path arithmetic: FP64 MFMA, or the Ozaki split on BF16 MFMA")
     ap.add_argument("--numeric-path", default="auto", choices=["auto", "vector", "tile"],
                     help="numeric kernels: LDS accumulator (vector), FP64 matrix cores per "
                          "tile pair (tile), or chosen by A's tile fill (auto)")
@@ -285,7 +287,8 @@ def main():
     rt.init(0)
     pa = rt.prepare_bgrf_a(a) if args.layout == "bgrf" else rt.prepare_identity_a(a)
     nnz_cub = rt.nnz_cub(pa, aat)
-    sched = rt.schedule(args.lb_mode, materialize_output=False, numeric_path=args.numeric_path)
+    sched = rt.schedule(args.lb_mode, materialize_output=False, numeric_path=args.numeric_path,
+                        precision=args.precision)
     pb = None
     for _ in range(args.warmup):
         pb = rt.rebuild_b(pa, aat, pb)

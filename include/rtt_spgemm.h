@@ -349,6 +349,22 @@ enum rtt_numeric_path {
     RTT_NUMERIC_TILE = 2
 };
 
+/* Arithmetic of the TILE path.  FP64: one FP64 MFMA per tile pair.
+ * SPLIT_BF16: the Ozaki scheme on BF16 matrix cores -- every A entry is cut,
+ * relative to the largest exponent of its tile row over a batch of at most
+ * eight pairs of one C tile, into eight exact 8-bit slices (B entries
+ * likewise per tile column); all 64 slice products of a pair run on
+ * v_mfma_f32_16x16x32_bf16 with exact FP32 sums (eight instructions per
+ * pair), and the slice sums are recombined and accumulated in FP64 on the
+ * GPU.  Each entry keeps 64 bits below its row (column) maximum, so
+ * |c - c*| <= 2^-62 sum_k (max_row|a| |b_kj| + |a_ik| max_col|b|) plus FP64
+ * rounding.  Bit-reproducible like FP64.  Requires the TILE path (AUTO
+ * takes it when this is set). */
+enum rtt_precision {
+    RTT_PRECISION_FP64 = 0,
+    RTT_PRECISION_SPLIT_BF16 = 1
+};
+
 typedef struct rtt_schedule {    /* DmmaNumericScheduleConfig subset */
     int lb_mode;                 /* enum rtt_lb_mode; default adaptive-tail */
     int materialize_output;      /* export C tiles to the host SMatrix */
@@ -360,6 +376,7 @@ typedef struct rtt_schedule {    /* DmmaNumericScheduleConfig subset */
     int force_tail_split;        /* admit every multi-step task (tests) */
     int numeric_path;            /* enum rtt_numeric_path; default AUTO */
     double tile_fill_threshold;  /* AUTO: stored values per A tile for TILE (12) */
+    int precision;               /* enum rtt_precision; default FP64 */
 } rtt_schedule;
 
 /* Fills *s with the reference defaults (src/main.cu:61-130). */
@@ -384,8 +401,10 @@ typedef struct rtt_stats {       /* DmmaSpGemmStats subset */
     int output_materialized;
     int numeric_path;                  /* the path that ran (VECTOR or TILE) */
     double b_columns_ms;               /* TILE: B's tile columns built for the call */
-    unsigned long long mfma_pairs;     /* TILE: tile pairs multiplied, one FP64 MFMA
-                                          (4 blocks x 4x4x4, 512 flops) each */
+    unsigned long long mfma_pairs;     /* TILE: tile pairs multiplied: one FP64 MFMA
+                                          (4 blocks x 4x4x4, 512 flops) each, or for
+                                          SPLIT_BF16 eight BF16 16x16x32 MFMAs */
+    int precision;                     /* enum rtt_precision that ran */
 } rtt_stats;
 
 /* dmma_tilespgemm (src/dmma_spgemm.h:6490): C = A * B with A 8x4 and B 4x8

@@ -636,6 +636,7 @@ void rtt_schedule_defaults(rtt_schedule *s)
     s->force_tail_split = 0;
     s->numeric_path = RTT_NUMERIC_AUTO;
     s->tile_fill_threshold = 12.0;
+    s->precision = RTT_PRECISION_FP64;
 }
 
 int rtt_tilespgemm(const rtt_prepared_a *a_const, const rtt_dynamic_b *b, rtt_smatrix *output,
@@ -666,6 +667,10 @@ int rtt_tilespgemm(const rtt_prepared_a *a_const, const rtt_dynamic_b *b, rtt_sm
         !(sc.tile_fill_threshold >= 0.0))
         return fail(RTT_ERR_INVALID, "unknown numeric path %d (fill threshold %g)",
                     sc.numeric_path, sc.tile_fill_threshold);
+    if (sc.precision != RTT_PRECISION_FP64 && sc.precision != RTT_PRECISION_SPLIT_BF16)
+        return fail(RTT_ERR_INVALID, "unknown precision %d", sc.precision);
+    if (sc.precision == RTT_PRECISION_SPLIT_BF16 && sc.numeric_path == RTT_NUMERIC_VECTOR)
+        return fail(RTT_ERR_INVALID, "split precision runs on the TILE path only");
     // The same shape contract as dmma_tilespgemm (src/dmma_spgemm.h:6495-6510).
     if (a->tiles.tile_col_count != b->tiles.tile_row_count || a->tiles.tile_rows != rtt::kTileM ||
         a->tiles.tile_cols != rtt::kTileK || b->tiles.tile_rows != rtt::kTileK ||
@@ -678,6 +683,8 @@ int rtt_tilespgemm(const rtt_prepared_a *a_const, const rtt_dynamic_b *b, rtt_sm
     rtt::CoreTimes times;
     a->has_c = false;
     int path = sc.numeric_path;
+    if (path == RTT_NUMERIC_AUTO && sc.precision == RTT_PRECISION_SPLIT_BF16)
+        path = RTT_NUMERIC_TILE;
     if (path == RTT_NUMERIC_AUTO) {
         const double fill = a->tiles.num_tiles > 0 ? static_cast<double>(a->tiles.payload) /
                                                          static_cast<double>(a->tiles.num_tiles)
@@ -695,8 +702,9 @@ int rtt_tilespgemm(const rtt_prepared_a *a_const, const rtt_dynamic_b *b, rtt_sm
         bc = rtt::b_columns(b->tiles);
     }
     RTT_TRY(rtt::spgemm_core(a->tiles, b->tiles, &a->last_c, &a->core, &a->events, &times,
-                            sc.lb_mode, path == RTT_NUMERIC_TILE ? 1 : 0, &bc));
+                            sc.lb_mode, path == RTT_NUMERIC_TILE ? 1 + sc.precision : 0, &bc));
     stats->numeric_path = path;
+    stats->precision = path == RTT_NUMERIC_TILE ? sc.precision : RTT_PRECISION_FP64;
     stats->mfma_pairs = times.mfma_pairs;
     a->has_c = true;
     stats->symbolic_ms = times.symbolic_ms;

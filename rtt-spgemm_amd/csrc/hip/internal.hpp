@@ -254,8 +254,9 @@ struct TileColumns {
     const double *vals = nullptr;
 };
 TileColumns b_columns(const DevTiles &b);
-// Values of every C tile from C's finished structure (numeric path 1: FP64
-// matrix cores).  Deterministic: each tile sums its pairs in increasing K.
+// Values of every C tile from C's finished structure: precision 0 on FP64
+// matrix cores, 1 by BF16 slices (Ozaki scheme) with FP64 recombination.
+// Deterministic: each tile sums its pairs in increasing K.
 int tile_numeric(const DevTiles &a, const TileColumns &bc, const DevC &c, int precision,
                  SpgemmWork *w, hipStream_t s);
 
@@ -270,9 +271,9 @@ struct CoreTimes {
 // hold one workgroup per row for many rows: static hands row i to
 // workgroup i mod grid; dynamic takes rows from a work queue in list order;
 // adaptive-tail takes them from the queue heaviest first (most pairs).
-// numeric_path 0 runs the LDS-accumulator kernels; 1 writes C's structure
-// for every row first and computes the values with the tile-pair kernel
-// (bc: B's tile columns).
+// numeric_path 0 runs the LDS-accumulator kernels; 1 (FP64) and 2 (split
+// BF16) write C's structure for every row first and compute the values with
+// the tile-pair kernel (bc: B's tile columns).
 int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *work,
                 Events *events, CoreTimes *times, int lb_mode = RTT_LB_ADAPTIVE_TAIL,
                 int numeric_path = 0, const TileColumns *bc = nullptr);

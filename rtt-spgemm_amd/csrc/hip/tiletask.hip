@@ -92,22 +92,304 @@ struct LaneRoles {
     }
 };
 
-// Stores lane's entry of the finished C tile `c_local` (0..63 of the chunk,
-// whose mask and value offset lanes hold as cm / co).
-__device__ __forceinline__ void store_tile(const TileTask &g, int c_local, unsigned long long cm,
-                                           long long co, int c_bit, double acc)
+// 64-bit value held by lane `l` (l uniform across the wave).
+__device__ __forceinline__ unsigned long long read_lane64(unsigned long long v, int l)
 {
-    const unsigned long long m =
-        (static_cast<unsigned long long>(static_cast<uint32_t>(read_lane(static_cast<int>(cm >> 32), c_local))) << 32) |
-        static_cast<uint32_t>(read_lane(static_cast<int>(cm), c_local));
-    const long long off =
-        (static_cast<long long>(read_lane(static_cast<int>(co >> 32), c_local)) << 32) |
-        static_cast<uint32_t>(read_lane(static_cast<int>(co), c_local));
-    if ((m >> c_bit) & 1ull)
-        g.c_val[off + __popcll(m & ((1ull << c_bit) - 1ull))] = acc;
+    return (static_cast<unsigned long long>(static_cast<uint32_t>(
+                read_lane(static_cast<int>(v >> 32), l))) << 32) |
+           static_cast<uint32_t>(read_lane(static_cast<int>(v), l));
 }
 
-__global__ __launch_bounds__(kTThreads) void tile_numeric_f64_kernel(TileTask g)
+// ---------------------------------------------------------------------------
+// Plain FP64: one v_mfma_f64_4x4x4f64 per pair into a running sum of one
+// register per lane.  A batch of kBatch pairs (which may span C tiles) has its
+// operand loads issued together.
+// ---------------------------------------------------------------------------
+struct Fp64Product {
+    struct Lane {
+        LaneRoles roles;
+        __device__ Lane(int lane) : roles(lane) {}
+    };
+    struct Acc {
+        double v = 0.0;
+    };
+    static constexpr unsigned long long kMfmaPerPair = 1;
+
+    __device__ static void store(const TileTask &g, const Lane &ln, int c_local,
+                                 unsigned long long cm, long long co, const Acc &acc)
+    {
+        const unsigned long long m = read_lane64(cm, c_local);
+        const long long off = static_cast<long long>(read_lane64(static_cast<unsigned long long>(co), c_local));
+        const int bit = ln.roles.c_bit;
+        if ((m >> bit) & 1ull)
+            g.c_val[off + __popcll(m & ((1ull << bit) - 1ull))] = acc.v;
+    }
+
+    // Runs buffered pairs [0, n) in order; a change of C tile stores the
+    // finished one.
+    __device__ static void run(const TileTask &g, const Lane &ln, const int4 *pm,
+                               const unsigned char *pc, int n, int &cur, Acc &acc,
+                               unsigned long long cm, long long co)
+    {
+        for (int p0 = 0; p0 < n; p0 += kBatch) {
+            const int nbat = min(kBatch, n - p0);
+            double av[kBatch], bv[kBatch];
+#pragma unroll
+            for (int i = 0; i < kBatch; ++i) {
+                if (i < nbat) {
+                    const int4 m = pm[p0 + i];
+                    av[i] = tile_entry(g.a_vals, static_cast<uint32_t>(m.x),
+                                       static_cast<uint32_t>(m.y), ln.roles.a_phys);
+                    bv[i] = tile_entry(g.b_vals, static_cast<uint32_t>(m.z),
+                                       static_cast<uint32_t>(m.w), ln.roles.b_phys);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < kBatch; ++i) {
+                if (i < nbat) {
+                    const int t = pc[p0 + i];
+                    if (t != cur) {
+                        if (cur >= 0)
+                            store(g, ln, cur, cm, co, acc);
+                        cur = t;
+                        acc.v = 0.0;
+                    }
+                    acc.v = __builtin_amdgcn_mfma_f64_4x4x4f64(av[i], bv[i], acc.v, 0, 0, 0);
+                }
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Split precision (Ozaki scheme, error-free slicing): for a batch of at most
+// eight pairs of one C tile, every A entry is written relative to the largest
+// exponent E_r of its tile row r over the batch as a 64-bit fixed-point
+// fraction, sign * sum_e q_e 2^(E_r - 8(e+1)) with eight integer slices
+// q_e in [0, 255] (the bytes of the fraction), and every B entry likewise
+// relative to its tile column's largest exponent.  Slices are exact in
+// BF16, and a product of two slices is exact in FP32.  All 64 slice pairs
+// are kept; grouped by d = e + f (0..14), the sums
+//   P_d[r][c] = sum_{k, e+f=d} qa_e[r][k] qb_f[k][c]
+// have at most 8 x 4 terms of at most 255^2 per pair, so eight pairs' sums
+// stay below 2^24 and the FP32 accumulation of v_mfma_f32_16x16x32_bf16 is
+// exact.  The reduction dimension of one instruction is 32 = (k, slice e):
+// lane group k holds the eight slices of a(r, k); B supplies, for output
+// column c + 8h, the slices qb_{d-e} with d = 2t + h, so instruction t gives
+// P_{2t} and P_{2t+1} side by side (rows 8..15 of the 16x16 output are
+// unused).  Each batch is then recombined in FP64 on the GPU,
+//   C[r][c] += 2^(E_r + E_c) sum_d P_d 2^(-8(d+2)),
+// and batches and C tiles accumulate in FP64.  The only approximation is the
+// truncation of every operand 64 bits below its row (column) maximum, a
+// relative error of at most 2^-63 of that maximum per entry.
+// Lane roles of v_mfma_f32_16x16x32_bf16 (measured on gfx950): A[i][k'] in
+// lane i + 16(k' >> 3), element k' & 7; B[k'][j] in lane j + 16(k' >> 3),
+// element k' & 7; D[i][j] in lane j + 16(i >> 2), element i & 3.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kSplitBatch = 8;   // pairs whose slice sums stay exact in FP32
+
+// Biased exponent of |x| for alignment (subnormals count as 1, zero as 0).
+__device__ __forceinline__ int align_exp(double x)
+{
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));
+    const int ex = static_cast<int>((b >> 52) & 0x7FF);
+    return ex ? ex : ((b << 1) ? 1 : 0);
+}
+
+// 64-bit fixed-point fraction of |x| / 2^(emax - 1022) (emax >= align_exp(x),
+// emax >= 1): floor(|x| 2^(64 - (emax - 1022))).
+__device__ __forceinline__ unsigned long long fixed_fraction(double x, int emax)
+{
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));
+    int ex = static_cast<int>((b >> 52) & 0x7FF);
+    unsigned long long m = b & ((1ull << 52) - 1ull);
+    if (ex)
+        m |= 1ull << 52;
+    else
+        ex = 1;
+    const int sh = ex - emax + 11;   // |x| = m 2^(ex - 1075)
+    if (sh >= 0)
+        return m << sh;
+    return sh > -64 ? (m >> -sh) : 0ull;
+}
+
+// BF16 bits of the integer byte (0..255) of v at byte position `byte`.
+__device__ __forceinline__ uint32_t byte_bf16(uint32_t v, int byte)
+{
+    return __float_as_uint(static_cast<float>((v >> (8 * byte)) & 0xFFu)) >> 16;
+}
+
+struct SplitProduct {
+    struct Lane {
+        int a_phys;      // A tile bit this lane supplies (rows 0..7 only)
+        bool a_live;     // lane's A row < 8
+        int b_phys;      // B tile bit
+        int h;           // output column half: d = 2t + h
+        int out_row0;    // first of the four C rows this lane's outputs cover
+        int out_col;     // C column of this lane's outputs
+        bool out_live;   // lane holds C outputs (rows 0..7, h = 0) after the halves combine
+        __device__ Lane(int lane)
+        {
+            const int i = lane & 15, k = lane >> 4;
+            a_live = i < 8;
+            a_phys = (i & 7) * 4 + k;
+            b_phys = (lane & 7) * 4 + k;
+            h = (lane >> 3) & 1;
+            out_row0 = 4 * (lane >> 4);
+            out_col = lane & 7;
+            out_live = lane < 32 && h == 0;
+        }
+    };
+    struct Acc {
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+    };
+    static constexpr unsigned long long kMfmaPerPair = 8;
+
+    __device__ static void store(const TileTask &g, const Lane &ln, int c_local,
+                                 unsigned long long cm, long long co, const Acc &acc)
+    {
+        const unsigned long long m = read_lane64(cm, c_local);
+        const long long off = static_cast<long long>(read_lane64(static_cast<unsigned long long>(co), c_local));
+        if (!ln.out_live)
+            return;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int bit = (ln.out_row0 + v) * 8 + ln.out_col;
+            if ((m >> bit) & 1ull)
+                g.c_val[off + __popcll(m & ((1ull << bit) - 1ull))] = acc.v[v];
+        }
+    }
+
+    // Pairs [p0, p0 + n) of one C tile, n <= kSplitBatch, added into acc.
+    __device__ static void batch(const TileTask &g, const Lane &ln, const int4 *pm, int p0, int n,
+                                 Acc &acc)
+    {
+        double av[kSplitBatch], bv[kSplitBatch];
+        int ea = 0, eb = 0;
+#pragma unroll
+        for (int i = 0; i < kSplitBatch; ++i) {
+            av[i] = 0.0;
+            bv[i] = 0.0;
+            if (i < n) {
+                const int4 m = pm[p0 + i];
+                if (ln.a_live)
+                    av[i] = tile_entry(g.a_vals, static_cast<uint32_t>(m.x),
+                                       static_cast<uint32_t>(m.y), ln.a_phys);
+                bv[i] = tile_entry(g.b_vals, static_cast<uint32_t>(m.z),
+                                   static_cast<uint32_t>(m.w), ln.b_phys);
+            }
+            ea = max(ea, align_exp(av[i]));
+            eb = max(eb, align_exp(bv[i]));
+        }
+        // Row (column) maxima over the four k of the tile: lanes x, x+16,
+        // x+32, x+48 hold one row (column).
+        ea = max(ea, __shfl_xor(ea, 16, kWave));
+        ea = max(ea, __shfl_xor(ea, 32, kWave));
+        eb = max(eb, __shfl_xor(eb, 16, kWave));
+        eb = max(eb, __shfl_xor(eb, 32, kWave));
+        ea = max(ea, 1);
+        eb = max(eb, 1);
+        f32x4 acc_t[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+            acc_t[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int i = 0; i < kSplitBatch; ++i) {
+            if (i >= n)
+                break;
+            // A slices in element order e = 0..7 (most significant first).
+            const unsigned long long fa = fixed_fraction(av[i], ea);
+            const uint32_t ahi = static_cast<uint32_t>(fa >> 32), alo = static_cast<uint32_t>(fa);
+            const uint32_t asg = av[i] < 0.0 ? 0x80008000u : 0u;
+            u32x4 aw;
+            aw[0] = (byte_bf16(ahi, 3) | (byte_bf16(ahi, 2) << 16)) | asg;
+            aw[1] = (byte_bf16(ahi, 1) | (byte_bf16(ahi, 0) << 16)) | asg;
+            aw[2] = (byte_bf16(alo, 3) | (byte_bf16(alo, 2) << 16)) | asg;
+            aw[3] = (byte_bf16(alo, 1) | (byte_bf16(alo, 0) << 16)) | asg;
+            // B slices q0..q7 laid out so that instruction t reads
+            // qb_{2t+h-e} for element e from words 8-t..11-t.
+            const unsigned long long fb = fixed_fraction(bv[i], eb);
+            const uint32_t bhi = static_cast<uint32_t>(fb >> 32), blo = static_cast<uint32_t>(fb);
+            const uint32_t bsg = bv[i] < 0.0 ? 0x80008000u : 0u;
+            const uint32_t q0 = byte_bf16(bhi, 3), q1 = byte_bf16(bhi, 2), q2 = byte_bf16(bhi, 1),
+                           q3 = byte_bf16(bhi, 0), q4 = byte_bf16(blo, 3), q5 = byte_bf16(blo, 2),
+                           q6 = byte_bf16(blo, 1), q7 = byte_bf16(blo, 0);
+            uint32_t wv[12];
+#pragma unroll
+            for (int x = 0; x < 12; ++x)
+                wv[x] = 0u;
+            if (ln.h == 0) {
+                wv[4] = (q7 << 16) | bsg;
+                wv[5] = (q6 | (q5 << 16)) | bsg;
+                wv[6] = (q4 | (q3 << 16)) | bsg;
+                wv[7] = (q2 | (q1 << 16)) | bsg;
+                wv[8] = q0 | bsg;
+            } else {
+                wv[5] = (q7 | (q6 << 16)) | bsg;
+                wv[6] = (q5 | (q4 << 16)) | bsg;
+                wv[7] = (q3 | (q2 << 16)) | bsg;
+                wv[8] = (q1 | (q0 << 16)) | bsg;
+            }
+            const bf16x8 aop = __builtin_bit_cast(bf16x8, aw);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                u32x4 bw;
+                bw[0] = wv[8 - t];
+                bw[1] = wv[9 - t];
+                bw[2] = wv[10 - t];
+                bw[3] = wv[11 - t];
+                acc_t[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    aop, __builtin_bit_cast(bf16x8, bw), acc_t[t], 0, 0, 0);
+            }
+        }
+        // Recombination in FP64: lane's outputs are rows out_row0 + v, column
+        // (lane & 15) -> C column lane & 7 at d = 2t + h.
+        const int lane = lane_id();
+        const int ecol = __shfl(eb, lane & 7, kWave) - 1022;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int erow = __shfl(ea, (ln.out_row0 + v) & 7, kWave) - 1022;
+            double sum = 0.0;
+            double w = ln.h ? 0x1p-24 : 0x1p-16;   // 2^(-8(d+2)) at t = 0
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                sum += static_cast<double>(acc_t[t][v]) * w;
+                w *= 0x1p-16;
+            }
+            sum = ldexp(sum, erow + ecol);
+            sum += __shfl_xor(sum, 8, kWave);   // the d = 2t + 1 half
+            acc.v[v] += sum;
+        }
+    }
+
+    __device__ static void run(const TileTask &g, const Lane &ln, const int4 *pm,
+                               const unsigned char *pc, int n, int &cur, Acc &acc,
+                               unsigned long long cm, long long co)
+    {
+        int p = 0;
+        while (p < n) {
+            const int t = pc[p];
+            if (t != cur) {
+                if (cur >= 0)
+                    store(g, ln, cur, cm, co, acc);
+                cur = t;
+                acc = Acc{};
+            }
+            int q = p + 1;
+            while (q < n && q - p < kSplitBatch && pc[q] == t)
+                ++q;
+            batch(g, ln, pm, p, q - p, acc);
+            p = q;
+        }
+    }
+};
+
+template <class P>
+__global__ __launch_bounds__(kTThreads) void tile_numeric_kernel(TileTask g)
 {
     __shared__ int s_keys[kTWaves][kAKeys];
     __shared__ int4 s_pm[kTWaves][kPairCap];            // {A mask, A offset, B mask, B offset}
@@ -121,7 +403,7 @@ __global__ __launch_bounds__(kTThreads) void tile_numeric_f64_kernel(TileTask g)
     unsigned char *pc = s_pc[w];
     int *start = s_start[w];
     int *cbeg = s_cb[w];
-    const LaneRoles roles(lane);
+    const typename P::Lane ln(lane);
     unsigned long long done = 0;   // pairs this wave multiplied
     const int home = blockIdx.x % kXcds;
     for (int probe = 0; probe < kXcds; ++probe) {
@@ -168,7 +450,7 @@ __global__ __launch_bounds__(kTThreads) void tile_numeric_f64_kernel(TileTask g)
                 wave_sync();
                 int npairs = 0;
                 int cur = -1;
-                double acc = 0.0;
+                typename P::Acc acc;
                 for (int q0 = 0; q0 < total; q0 += kWave) {
                     const int q = q0 + lane;
                     int ai = -1;
@@ -201,41 +483,14 @@ __global__ __launch_bounds__(kTThreads) void tile_numeric_f64_kernel(TileTask g)
                     const bool last = q0 + kWave >= total;
                     if (npairs + kWave > kPairCap || last) {
                         wave_sync();
-                        for (int p0 = 0; p0 < npairs; p0 += kBatch) {
-                            const int nbat = min(kBatch, npairs - p0);
-                            double av[kBatch], bv[kBatch];
-#pragma unroll
-                            for (int i = 0; i < kBatch; ++i) {
-                                if (i < nbat) {
-                                    const int4 m = pm[p0 + i];
-                                    av[i] = tile_entry(g.a_vals, static_cast<uint32_t>(m.x),
-                                                       static_cast<uint32_t>(m.y), roles.a_phys);
-                                    bv[i] = tile_entry(g.b_vals, static_cast<uint32_t>(m.z),
-                                                       static_cast<uint32_t>(m.w), roles.b_phys);
-                                }
-                            }
-#pragma unroll
-                            for (int i = 0; i < kBatch; ++i) {
-                                if (i < nbat) {
-                                    const int t = pc[p0 + i];
-                                    if (t != cur) {
-                                        if (cur >= 0)
-                                            store_tile(g, cur, cm, co, roles.c_bit, acc);
-                                        cur = t;
-                                        acc = 0.0;
-                                    }
-                                    acc = __builtin_amdgcn_mfma_f64_4x4x4f64(av[i], bv[i], acc, 0,
-                                                                            0, 0);
-                                }
-                            }
-                        }
+                        P::run(g, ln, pm, pc, npairs, cur, acc, cm, co);
                         done += static_cast<unsigned long long>(npairs);
                         npairs = 0;
                         wave_sync();
                     }
                 }
                 if (cur >= 0)
-                    store_tile(g, cur, cm, co, roles.c_bit, acc);
+                    P::store(g, ln, cur, cm, co, acc);
                 wave_sync();
             }
             if (lds_keys)
@@ -278,8 +533,8 @@ int tile_numeric(const DevTiles &a, const TileColumns &bc, const DevC &c, int pr
 {
     if (c.tilem == 0 || c.num_tiles == 0)
         return RTT_OK;
-    if (precision != 0)
-        return fail(RTT_ERR_UNSUPPORTED, "tile numeric precision %d is not built", precision);
+    if (precision != 0 && precision != 1)
+        return fail(RTT_ERR_INVALID, "unknown tile numeric precision %d", precision);
     RTT_TRY(w->tile_queue.ensure(kXcds * sizeof(int)));
     RTT_TRY(w->tile_pairs.ensure(sizeof(unsigned long long)));
     TileTask g;
@@ -303,8 +558,12 @@ int tile_numeric(const DevTiles &a, const TileColumns &bc, const DevC &c, int pr
     // Persistent: enough workgroups to fill every CU several times over.
     const int blocks = static_cast<int>(std::min<int64_t>(
         static_cast<int64_t>(cus) * 8, (static_cast<int64_t>(g.rows) + kTWaves - 1) / kTWaves));
-    hipLaunchKernelGGL(tile_numeric_f64_kernel, dim3(std::max(blocks, 1)), dim3(kTThreads), 0, s,
-                       g);
+    if (precision == 0)
+        hipLaunchKernelGGL(tile_numeric_kernel<Fp64Product>, dim3(std::max(blocks, 1)),
+                           dim3(kTThreads), 0, s, g);
+    else
+        hipLaunchKernelGGL(tile_numeric_kernel<SplitProduct>, dim3(std::max(blocks, 1)),
+                           dim3(kTThreads), 0, s, g);
     RTT_HIP(hipGetLastError());
     return RTT_OK;
 }

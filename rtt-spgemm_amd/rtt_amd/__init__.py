@@ -108,7 +108,8 @@ class Schedule(C.Structure):
                 ("split_threshold", C.c_double), ("split_chunk_target", C.c_double),
                 ("split_max_chunks", C.c_int), ("max_heavy_fraction", C.c_double),
                 ("tail_record_capacity", C.c_int), ("force_tail_split", C.c_int),
-                ("numeric_path", C.c_int), ("tile_fill_threshold", C.c_double)]
+                ("numeric_path", C.c_int), ("tile_fill_threshold", C.c_double),
+                ("precision", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -120,7 +121,7 @@ class Stats(C.Structure):
                 ("output_nnz", C.c_ulonglong), ("heavy_rows", C.c_ulonglong),
                 ("heavy_chunks", C.c_ulonglong), ("output_materialized", C.c_int),
                 ("numeric_path", C.c_int), ("b_columns_ms", C.c_double),
-                ("mfma_pairs", C.c_ulonglong)]
+                ("mfma_pairs", C.c_ulonglong), ("precision", C.c_int)]
 
 
 class DeviceC(C.Structure):
@@ -722,15 +723,17 @@ def update_b_values(a: PreparedA, src: DeviceCsr, b: DynamicB,
 
 
 NUMERIC_PATHS = {"auto": 0, "vector": 1, "tile": 2}
+PRECISIONS = {"fp64": 0, "split-bf16": 1}
 
 
 def schedule(lb_mode: str = "adaptive-tail", materialize_output: bool = True,
-             numeric_path: str = "auto", **kw) -> Schedule:
+             numeric_path: str = "auto", precision: str = "fp64", **kw) -> Schedule:
     s = Schedule()
     load().rtt_schedule_defaults(C.byref(s))
     s.lb_mode = LB_MODES[lb_mode]
     s.materialize_output = int(materialize_output)
     s.numeric_path = NUMERIC_PATHS[numeric_path]
+    s.precision = PRECISIONS[precision]
     for k, v in kw.items():
         setattr(s, k, v)
     return s

@@ -106,3 +106,98 @@ def test_tile_path_values_update(gpu):
     gpu.update_b_values(pa, src, pb, "safe-auto")
     tc, _ = gpu.tilespgemm(pa, pb, s)
     check_c(gpu, tc, oracle.spgemm(as_tuple(a), as_tuple(b2)))
+
+
+# ---- split precision (Ozaki scheme on BF16 matrix cores) --------------------
+
+SPLIT_FWD_TOL = 1e-12
+
+
+def split_bound_ok(rt, a, b, tc, ref):
+    """Checks every entry of C against the split form's error bound: each
+    operand entry keeps 64 bits below its row (A) / column (B) maximum, so
+    |c - c*| <= 2^-61 (maxrow|a|_i (P_A |B|)_ij + maxcol|b|_j (|A| P_B)_ij)
+    plus FP64 rounding of the recombination, 2^-50 (|A||B|)_ij.  Returns the
+    value errors (oracle.value_errors) for the caller's own tolerance."""
+    import scipy.sparse as sp
+    A, B = a.to_scipy().tocsr(), b.to_scipy().tocsr()
+    absA, absB = abs(A), abs(B)
+    PA, PB = absA.copy(), absB.copy()
+    PA.data[:] = 1.0
+    PB.data[:] = 1.0
+    ra = np.asarray(absA.max(axis=1).todense()).ravel()
+    rb = np.asarray(absB.max(axis=0).todense()).ravel()
+    t1 = (sp.diags(ra) @ (PA @ absB)).tocsr()
+    t2 = ((absA @ PB) @ sp.diags(rb)).tocsr()
+    ab = (absA @ absB).tocsr()
+    for M in (t1, t2, ab):
+        M.sort_indices()
+    csr = rt.tile2csr(tc, canonical=True)
+    assert np.array_equal(t1.indices, csr.col) and np.array_equal(ab.indices, ref["col"])
+    err = np.abs((csr.val - ref["hi"]) - ref["lo"])
+    bound = 2.0 ** -61 * (t1.data + t2.data) + 2.0 ** -50 * ab.data
+    worst = np.max(err / bound) if err.size else 0.0
+    assert worst <= 1.0, worst
+    return oracle.value_errors(tc.tile_csr_Value, *(
+        oracle.c_tiles(ref["m"], ref["n"], ref["rowptr"], ref["col"], v)["tile_csr_val"]
+        for v in (ref["hi"], ref["lo"], ref["absprod"])))
+
+
+def split_run(rt, a, aat):
+    pa = rt.prepare_identity_a(a)
+    pb = rt.rebuild_b(pa, aat)
+    tc, stats = rt.tilespgemm(pa, pb, rt.schedule(precision="split-bf16"))
+    assert stats["numeric_path"] == rt.NUMERIC_PATHS["tile"]
+    assert stats["precision"] == rt.PRECISIONS["split-bf16"]
+    ref = oracle_product(a, aat)
+    # structure is exact whatever the arithmetic
+    ct = oracle.c_tiles(ref["m"], ref["n"], ref["rowptr"], ref["col"], ref["val"])
+    assert np.array_equal(tc.tile_columnidx, ct["tile_col"])
+    assert np.array_equal(tc.tile_nnz, ct["tile_nnz"])
+    assert np.array_equal(tc.tile_csr_Col, ct["tile_csr_col"])
+    b = a
+    if aat:
+        b = rt.csr_transpose(a)
+    return pa, pb, tc, split_bound_ok(rt, a, b, tc, ref)
+
+
+SPLIT_CASES = [
+    ("fem_morton", lambda rt: rt.gen_fem_blocks(1500, 6, True, 77), 0, True),
+    ("fem_draw_order", lambda rt: rt.gen_fem_blocks(600, 6, False, 78), 0, True),
+    ("laplace64", lambda rt: rt.gen_laplace2d(64, 0.1, 20260101), 0, True),
+    ("stencil8", lambda rt: rt.gen_stencil27(8, 0.5, 1.5, 8), 1, True),
+    ("random_aa", lambda rt: rt.gen_random(300, 300, 9, 31), 0, False),
+    ("rmat12", lambda rt: rt.gen_rmat(12, 16, seed=5), 0, False),
+]
+
+
+@pytest.mark.parametrize("name,make,aat,bounded_spread", SPLIT_CASES,
+                         ids=[c[0] for c in SPLIT_CASES])
+def test_split_precision_within_its_bound(gpu, name, make, aat, bounded_spread):
+    """Every entry within the slicing bound; on inputs whose magnitudes span
+    at most six decades per tile row / column (block FEM, Laplacian,
+    stencil) that is a forward error of at most 1e-12.  Uniform [-1, 1]
+    values (random, R-MAT) have unbounded spread, so only the bound holds."""
+    print(f"case {name}", flush=True)
+    _, _, _, err = split_run(gpu, make(gpu), aat)
+    print(name, err, flush=True)
+    if bounded_spread:
+        assert err["fwd_max"] <= SPLIT_FWD_TOL, err
+
+
+def test_split_precision_is_bit_reproducible(gpu):
+    a = gpu.gen_fem_blocks(1200, 6, True, 19)
+    pa = gpu.prepare_identity_a(a)
+    pb = gpu.rebuild_b(pa, 0)
+    s = gpu.schedule(precision="split-bf16")
+    t1, _ = gpu.tilespgemm(pa, pb, s)
+    t2, _ = gpu.tilespgemm(pa, pb, s)
+    assert np.array_equal(t1.tile_csr_Value.view(np.uint64), t2.tile_csr_Value.view(np.uint64))
+
+
+def test_split_precision_rejects_the_vector_path(gpu):
+    a = gpu.gen_random(40, 40, 3, 1)
+    pa = gpu.prepare_identity_a(a)
+    pb = gpu.rebuild_b(pa, 0)
+    with pytest.raises(gpu.RttError):
+        gpu.tilespgemm(pa, pb, gpu.schedule(numeric_path="vector", precision="split-bf16"))
