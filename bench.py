@@ -61,6 +61,157 @@ def make_matrix(rt, spec):
         f"2D 5-point Laplacian {spec['g']}^2, off-diagonals -1+U[-0.1,0.1], seed 20260101")
 
 
+# ---- batched workloads -----------------------------------------------------
+
+BATCHES = {
+    # 512 independent banded A^2 problems (log-normal row counts, median 16k,
+    # clipped to [1k, 200k]; bandwidth U{8..64}; U{5..30} entries per row
+    # inside the band plus the diagonal; values U[-1,1]).
+    "banded-batch-512": dict(kind="banded", problems=512),
+    "banded-batch-64": dict(kind="banded", problems=64),
+    # AMG Galerkin products R*(A*P) of 7-point Poisson hierarchies on g^3
+    # grids, g ~ U{32..96}, coefficient jumps 10^U[-3,3], smoothed
+    # aggregation P (2x2x2 aggregates, omega = 2/3), R = P^T.
+    "amg-batch-64": dict(kind="amg", problems=64),
+    "amg-batch-8": dict(kind="amg", problems=8),
+}
+
+
+def make_batch(rt, spec, seed=20260101):
+    if spec["kind"] == "banded":
+        rows, bw, pr = rt.gen_banded_batch_shapes(spec["problems"], seed=seed)
+        mats = [rt.gen_banded(int(rows[p]), int(bw[p]), int(pr[p]), seed + p)
+                for p in range(spec["problems"])]
+        return {"A": mats, "desc": f"{spec['problems']} banded problems, rows "
+                f"{int(rows.min())}..{int(rows.max())} (median {int(np.median(rows))}), "
+                f"sum {int(rows.sum())} rows, {sum(m.nnz for m in mats)} nnz"}
+    gs = np.random.default_rng(seed).integers(32, 97, spec["problems"])
+    A = [rt.gen_poisson7_jumps(int(g), 3.0, seed + p) for p, g in enumerate(gs)]
+    P = [rt.gen_aggregation_prolongator(a, int(g), 2.0 / 3.0) for a, g in zip(A, gs)]
+    R = [rt.csr_transpose(p) for p in P]
+    return {"A": A, "P": P, "R": R, "g": gs,
+            "desc": f"{spec['problems']} AMG hierarchies, g {int(gs.min())}..{int(gs.max())}, "
+                    f"sum {int(sum(int(g) ** 3 for g in gs))} fine rows, "
+                    f"{sum(a.nnz for a in A)} nnz(A), {sum(p.nnz for p in P)} nnz(P)"}
+
+
+def timed_steps(step, steps, warmup):
+    import rtt_amd as rt
+    for _ in range(warmup):
+        step()
+    rt.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    rt.sync()
+    return 1e3 * (time.perf_counter() - t0) / steps
+
+
+def batch_banded(rt, data, steps, warmup, precision):
+    """One block-diagonal call (one launch per stage over every problem's
+    tiles, schedule from the autotuner) against one call per problem with
+    the default schedule."""
+    mats = data["A"]
+    big, ro, co = rt.block_diag(mats, 8)
+    pa = rt.prepare_identity_a(big)
+    nnz_cub = rt.nnz_cub(pa, 0)
+    pb = rt.rebuild_b(pa, 0)
+    sched, table, chosen = rt.autotune(pa, pb, repeats=2)
+    sched.precision = rt.PRECISIONS[precision]
+    state = {}
+
+    def batched():
+        state["pb"] = rt.rebuild_b(pa, 0, state.get("pb"))
+        state["st"] = rt.tilespgemm(pa, state["pb"], sched)[1]
+
+    ms_b = timed_steps(batched, steps, warmup)
+    nnz_c = state["st"]["output_nnz"]
+    # one product per call, default schedule
+    pas = [rt.prepare_identity_a(m) for m in mats]
+    pbs = [None] * len(pas)
+    seq_sched = rt.schedule(materialize_output=False, precision=precision)
+    seq_nnz = [0]
+
+    def sequential():
+        tot = 0
+        for i, p in enumerate(pas):
+            pbs[i] = rt.rebuild_b(p, 0, pbs[i])
+            tot += rt.tilespgemm(p, pbs[i], seq_sched)[1]["output_nnz"]
+        seq_nnz[0] = tot
+
+    ms_s = timed_steps(sequential, max(1, steps // 2), 1)
+    return {"nnz_cub": nnz_cub, "nnz_c": nnz_c, "ms_batched": ms_b, "ms_sequential": ms_s,
+            "nnz_c_matches_sequential": nnz_c == seq_nnz[0], "tuned": {
+                "chosen": table[chosen], "candidates": table}}
+
+
+def batch_amg(rt, data, steps, warmup, precision, sequential=True):
+    """R*(A*P) over the block-diagonal hierarchies: the two products of one
+    step, P and A*P resident on the device as B operands (their tile
+    rebuilds are inside the step), against the same per hierarchy."""
+    def prep(As, Ps, Rs):
+        out = []
+        for A, P, R in zip(As, Ps, Rs):
+            pa = rt.prepare_identity_a(A)
+            sp_ = rt.upload_csr(P)
+            pr = rt.prepare_identity_a(R)
+            # A*P once to obtain A*P as a CSR operand (outside the step)
+            pb = rt.rebuild_b_general(pa, sp_)
+            tc, _ = rt.tilespgemm(pa, pb, rt.schedule())
+            ap = rt.tile2csr(tc, canonical=True)
+            sap = rt.upload_csr(ap)
+            out.append({"pa": pa, "sp": sp_, "pr": pr, "sap": sap, "pb": None, "pb2": None,
+                        "cub": rt.nnz_cub_ab(pa, sp_) + rt.nnz_cub_ab(pr, sap)})
+        return out
+
+    sched = rt.schedule(materialize_output=False, precision=precision)
+
+    def run(items):
+        st = None
+        for it in items:
+            it["pb"] = rt.rebuild_b_general(it["pa"], it["sp"], it["pb"])
+            rt.tilespgemm(it["pa"], it["pb"], sched)
+            it["pb2"] = rt.rebuild_b_general(it["pr"], it["sap"], it["pb2"])
+            st = rt.tilespgemm(it["pr"], it["pb2"], sched)[1]
+        return st
+
+    bA, _, _ = rt.block_diag(data["A"], 8)
+    bP, _, _ = rt.block_diag(data["P"], 8)
+    bR, _, _ = rt.block_diag(data["R"], 8)
+    big = prep([bA], [bP], [bR])
+    ms_b = timed_steps(lambda: run(big), steps, warmup)
+    res = {"nnz_cub": big[0]["cub"], "ms_batched": ms_b}
+    if sequential:
+        items = prep(data["A"], data["P"], data["R"])
+        res["ms_sequential"] = timed_steps(lambda: run(items), max(1, steps // 2), 1)
+    return res
+
+
+def batch_main(args):
+    import rtt_amd as rt
+    spec = BATCHES[args.workload]
+    rt.init(0)
+    data = make_batch(rt, spec)
+    if spec["kind"] == "banded":
+        r = batch_banded(rt, data, args.steps, args.warmup, args.precision)
+    else:
+        r = batch_amg(rt, data, args.steps, args.warmup, args.precision)
+    flop = 2.0 * r["nnz_cub"]
+    line = {"workload": f"{args.workload}: {data['desc']}", "precision": args.precision,
+            "steps": args.steps, "warmup": args.warmup, "nnz_cub": r["nnz_cub"],
+            "batched": {"ms_per_step": round(r["ms_batched"], 4),
+                        "value": round(flop / (r["ms_batched"] * 1e6), 3), "unit": "GFLOP/s"}}
+    if "ms_sequential" in r:
+        line["sequential"] = {"ms_per_step": round(r["ms_sequential"], 4),
+                              "value": round(flop / (r["ms_sequential"] * 1e6), 3),
+                              "unit": "GFLOP/s"}
+        line["batched_over_sequential"] = round(r["ms_sequential"] / r["ms_batched"], 3)
+    for k in ("nnz_c", "nnz_c_matches_sequential", "tuned"):
+        if k in r:
+            line[k] = r[k]
+    print(json.dumps(line), flush=True)
+
+
 def csr_bytes(m, nnz):
     # Compulsory CSR traffic of one operand (src/external/cusparse/main.cu:205-208).
     return 4 * (m + 1) + 12 * nnz
@@ -245,7 +396,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="stencil27-128-aat", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="stencil27-128-aat",
+                    choices=sorted(WORKLOADS) + sorted(BATCHES))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--lb-mode", default="adaptive-tail",
                     choices=["static", "dynamic", "adaptive-tail"],
@@ -275,6 +427,9 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1).", file=sys.stderr)
         sys.exit(2)
+    if args.workload in BATCHES:
+        batch_main(args)
+        return
 
     import rtt_amd as rt
     spec = WORKLOADS[args.workload]

@@ -334,15 +334,19 @@ enum rtt_lb_mode {               /* --lb-mode (src/main.cu:132-166) */
     RTT_LB_ADAPTIVE_TAIL = 2
 };
 
-/* Numeric kernels of a call.  VECTOR: one workgroup per A tile row adds
- * every scalar product into an LDS (or, for very wide rows, HBM) copy of the
- * row's output with FP64 atomics.  TILE: C's structure is written first, then
- * every 8x8 C tile sums its (A tile, B tile) pairs in increasing K on the
- * FP64 matrix cores (v_mfma_f64_4x4x4f64, one instruction per pair), so its
- * values are bit-reproducible.  AUTO takes TILE when A's tiles hold at least
- * tile_fill_threshold stored values on average (the reference's DMMA path
- * multiplies every matched tile pair on tensor cores, src/dmma_spgemm.h:
- * 4176-4246; sparse tiles pay there for slots they do not use). */
+/* Numeric kernels of a call, chosen per A tile row.  VECTOR: one workgroup
+ * per A tile row adds every scalar product into an LDS (or, for very wide
+ * rows, HBM) copy of the row's output with FP64 atomics.  TILE: the row's C
+ * structure is written first, then every 8x8 C tile sums its (A tile,
+ * B tile) pairs in increasing K on the FP64 matrix cores
+ * (v_mfma_f64_4x4x4f64, one instruction per pair), so its values are
+ * bit-reproducible.  AUTO sends a tile row to TILE when its A tiles hold at
+ * least tile_fill_threshold stored values on average and to VECTOR
+ * otherwise, in the same call (the reference's DMMA path multiplies every
+ * matched tile pair on tensor cores, src/dmma_spgemm.h:4176-4246; sparse
+ * tiles pay there for slots they do not use).  rtt_stats.numeric_path
+ * reports TILE or VECTOR when every nonempty row took that path, AUTO for a
+ * mix. */
 enum rtt_numeric_path {
     RTT_NUMERIC_AUTO = 0,
     RTT_NUMERIC_VECTOR = 1,
@@ -399,7 +403,7 @@ typedef struct rtt_stats {       /* DmmaSpGemmStats subset */
                                           budget (larger LDS kernel or HBM slab) */
     unsigned long long heavy_chunks;   /* chunks of split heavy C tiles */
     int output_materialized;
-    int numeric_path;                  /* the path that ran (VECTOR or TILE) */
+    int numeric_path;                  /* VECTOR, TILE, or AUTO when rows took both */
     double b_columns_ms;               /* TILE: B's tile columns built for the call */
     unsigned long long mfma_pairs;     /* TILE: tile pairs multiplied: one FP64 MFMA
                                           (4 blocks x 4x4x4, 512 flops) each, or for
@@ -437,6 +441,43 @@ int rtt_last_c_view(const rtt_prepared_a *a, rtt_device_c *view);
 
 /* Copies device memory to host (test and tool helper). */
 int rtt_memcpy_d2h(void *dst, const void *src, size_t bytes);
+
+/* Per row of the last C on `a` (rows in A's tiled order, a's row count
+ * entries each): its number of entries and the sum of its values, computed on
+ * the device from C's tile structure (checking helper for products too large
+ * to download). */
+int rtt_last_c_row_stats(const rtt_prepared_a *a, int64_t *row_nnz, double *row_sum);
+
+/* ------------------------------------------------------------------------ */
+/* Autotuning (no reference counterpart: the reference fixes its schedule   */
+/* through driver flags, src/main.cu:61-166)                                 */
+/* ------------------------------------------------------------------------ */
+#define RTT_TUNE_MAX_CANDIDATES 16
+typedef struct rtt_tune_report {
+    int candidates;
+    int chosen;                                  /* index of the fastest */
+    double ms[RTT_TUNE_MAX_CANDIDATES];          /* median call time */
+    int numeric_path[RTT_TUNE_MAX_CANDIDATES];
+    double fill[RTT_TUNE_MAX_CANDIDATES];
+    int lb_mode[RTT_TUNE_MAX_CANDIDATES];
+} rtt_tune_report;
+
+/* Times rtt_tilespgemm(a, b) (symbolic + allocate + numeric, no export)
+ * `repeats` times after one warm-up for each candidate schedule -- the
+ * per-row numeric routing threshold (VECTOR, AUTO at 24/16/12/8 stored values
+ * per A tile, TILE) and the slab rows' queue assignment -- and writes the
+ * fastest into *sched (other fields kept).  The tile shapes are fixed by the
+ * format (8x4 A, 4x8 B, 8x8 C).  report may be NULL. */
+int rtt_autotune(const rtt_prepared_a *a, const rtt_dynamic_b *b, int repeats,
+                 rtt_schedule *sched, rtt_tune_report *report);
+
+/* One problem of a block-diagonal batch (rtt_block_diag) from the last C on
+ * `a`: the m x n block whose first row and column are *row_offset and
+ * *col_offset (multiples of 8), as a host tile-CSR SMatrix with the block's
+ * own tile columns (no reference counterpart: the reference runs one
+ * product per call). */
+int rtt_batch_extract(const rtt_prepared_a *a, const int64_t *row_offset,
+                      const int64_t *col_offset, int m, int n, rtt_smatrix *out);
 
 /* ------------------------------------------------------------------------ */
 /* Output conversion (host, OpenMP)                                          */

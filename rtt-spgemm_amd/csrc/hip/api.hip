@@ -51,6 +51,35 @@ namespace {
 
 using rtt::fail;
 
+// Thread per row of C: its entry count and value sum, from C's tile
+// structure (a checking utility; walks the row's tiles in order).
+__global__ void c_row_stats_kernel(int64_t rows, const int64_t *__restrict__ tile_ptr,
+                                   const uint64_t *__restrict__ mask,
+                                   const int64_t *__restrict__ tile_nnz,
+                                   const double *__restrict__ val, int64_t *__restrict__ nnz_out,
+                                   double *__restrict__ sum_out)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= rows)
+        return;
+    const int64_t tr = i / 8;
+    const int r = static_cast<int>(i % 8);
+    int64_t n = 0;
+    double sum = 0.0;
+    for (int64_t t = tile_ptr[tr]; t < tile_ptr[tr + 1]; ++t) {
+        const uint64_t m = mask[t];
+        const uint64_t below = r ? (m & ((1ull << (8 * r)) - 1ull)) : 0ull;
+        const int before = __popcll(below);
+        const int here = __popc(static_cast<uint32_t>((m >> (8 * r)) & 0xFFull));
+        const int64_t base = tile_nnz[t] + before;
+        for (int x = 0; x < here; ++x)
+            sum += val[base + x];
+        n += here;
+    }
+    nnz_out[i] = n;
+    sum_out[i] = sum;
+}
+
 // Identities of the prepared A handles alive in this process, so that a B
 // bound to a destroyed A is refused instead of dereferenced.
 std::atomic<uint64_t> g_next_a_id{1};
@@ -437,10 +466,8 @@ int rtt_rebuild_b(const rtt_prepared_a *a_const, int aat, rtt_dynamic_b **inout,
         RTT_TRY(rtt::build_tiles(a->csr_t, inner_map, a->active_inner, a->rows, a->cols, a->rows,
                                  true, a->dense_threshold, false, &b->tiles, &b->build));
     }
-    if (!b->tiles.transpose_of) {
-        RTT_TRY(rtt::build_records(&b->tiles));   // the transpose writes its records itself
+    if (!b->tiles.transpose_of)   // the transpose writes its records itself
         RTT_TRY(rtt::order_by_class(&b->tiles, &b->build));
-    }
     RTT_TRY(rtt::build_pair_extents(a->tiles, &b->tiles));
     RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
     b->a_id = a->id;
@@ -516,7 +543,6 @@ int rtt_rebuild_b_general(const rtt_prepared_a *a_const, const rtt_device_csr *s
     RTT_TRY(rtt::build_tiles(src->csr, inner_map, a_const->active_inner, src->csr.cols,
                              a_const->cols, src->csr.cols, true, a_const->dense_threshold, false,
                              &b->tiles, &b->build, true));
-    RTT_TRY(rtt::build_records(&b->tiles));
     RTT_TRY(rtt::order_by_class(&b->tiles, &b->build));
     RTT_TRY(rtt::build_pair_extents(a_const->tiles, &b->tiles));
     RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
@@ -682,29 +708,39 @@ int rtt_tilespgemm(const rtt_prepared_a *a_const, const rtt_dynamic_b *b, rtt_sm
     auto t0 = rtt::Clock::now();
     rtt::CoreTimes times;
     a->has_c = false;
+    // Numeric routing per tile row: TILE takes every row to the tile-pair
+    // kernel, VECTOR none, AUTO the rows whose A tiles hold at least
+    // tile_fill_threshold stored values on average.  Split precision runs
+    // every row on the tile-pair kernel.
     int path = sc.numeric_path;
-    if (path == RTT_NUMERIC_AUTO && sc.precision == RTT_PRECISION_SPLIT_BF16)
+    if (sc.precision == RTT_PRECISION_SPLIT_BF16)
         path = RTT_NUMERIC_TILE;
+    float fill = path == RTT_NUMERIC_TILE ? 0.f : static_cast<float>(sc.tile_fill_threshold);
+    int64_t tile_rows = 0, nonempty = 0;
     if (path == RTT_NUMERIC_AUTO) {
-        const double fill = a->tiles.num_tiles > 0 ? static_cast<double>(a->tiles.payload) /
-                                                         static_cast<double>(a->tiles.num_tiles)
-                                                   : 0.0;
-        path = fill >= sc.tile_fill_threshold && a->tiles.num_tiles > 0 ? RTT_NUMERIC_TILE
-                                                                        : RTT_NUMERIC_VECTOR;
+        RTT_TRY(rtt::count_tile_rows(a->tiles, fill, &a->core.tile_rows, &tile_rows, &nonempty));
+        if (tile_rows == 0)
+            path = RTT_NUMERIC_VECTOR;
+        else if (tile_rows == nonempty)
+            path = RTT_NUMERIC_TILE;
     }
-    rtt::TileColumns bc;
-    if (path == RTT_NUMERIC_TILE) {
+    rtt::TileOpts topt;
+    topt.fill = path == RTT_NUMERIC_TILE ? 0.f : fill;
+    topt.precision = sc.precision;
+    if (path != RTT_NUMERIC_TILE)
+        RTT_TRY(rtt::ensure_record_values(&const_cast<rtt_dynamic_b *>(b)->tiles));
+    if (path != RTT_NUMERIC_VECTOR) {
         rtt_dynamic_b *bm = const_cast<rtt_dynamic_b *>(b);
         auto tc = rtt::Clock::now();
         RTT_TRY(rtt::build_b_columns(&bm->tiles, &bm->build));
         RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
         stats->b_columns_ms = rtt::ms_since(tc);
-        bc = rtt::b_columns(b->tiles);
+        topt.bc = rtt::b_columns(b->tiles);
     }
     RTT_TRY(rtt::spgemm_core(a->tiles, b->tiles, &a->last_c, &a->core, &a->events, &times,
-                            sc.lb_mode, path == RTT_NUMERIC_TILE ? 1 + sc.precision : 0, &bc));
+                            sc.lb_mode, path == RTT_NUMERIC_VECTOR ? nullptr : &topt));
     stats->numeric_path = path;
-    stats->precision = path == RTT_NUMERIC_TILE ? sc.precision : RTT_PRECISION_FP64;
+    stats->precision = path == RTT_NUMERIC_VECTOR ? RTT_PRECISION_FP64 : sc.precision;
     stats->mfma_pairs = times.mfma_pairs;
     a->has_c = true;
     stats->symbolic_ms = times.symbolic_ms;
@@ -724,6 +760,177 @@ int rtt_tilespgemm(const rtt_prepared_a *a_const, const rtt_dynamic_b *b, rtt_sm
         stats->output_copy_ms = rtt::ms_since(te);
         stats->output_materialized = 1;
     }
+    return RTT_OK;
+}
+
+int rtt_batch_extract(const rtt_prepared_a *a, const int64_t *row_offset, const int64_t *col_offset,
+                      int m, int n, rtt_smatrix *out)
+{
+    rtt::clear_error();
+    RTT_TRY(rtt::require_ctx());
+    if (!a || !row_offset || !col_offset || !out || m < 0 || n < 0)
+        return fail(RTT_ERR_INVALID, "rtt_batch_extract: bad argument");
+    if (!a->has_c)
+        return fail(RTT_ERR_INVALID, "rtt_batch_extract: no C has been produced");
+    const int64_t r0 = *row_offset, c0 = *col_offset;
+    if (r0 % rtt::kTileM || c0 % rtt::kTileN || r0 < 0 || c0 < 0 || r0 + m > a->rows ||
+        c0 + n > a->last_c.cols)
+        return fail(RTT_ERR_INVALID,
+                    "rtt_batch_extract: block at (%lld, %lld) of %d x %d is not 8-aligned "
+                    "inside C",
+                    (long long)r0, (long long)c0, m, n);
+    const rtt::DevC &c = a->last_c;
+    const int64_t tr0 = r0 / rtt::kTileM;
+    const int64_t tr_want = (m + rtt::kTileM - 1) / rtt::kTileM;
+    const int64_t tr1 = std::min<int64_t>(c.tilem, tr0 + tr_want);
+    const int64_t nrows = std::max<int64_t>(0, tr1 - tr0);
+    std::vector<int64_t> tp(static_cast<size_t>(nrows) + 1, 0);
+    if (nrows > 0)
+        RTT_TRY(rtt_memcpy_d2h(tp.data(), c.tile_ptr.as<int64_t>() + tr0,
+                               (static_cast<size_t>(nrows) + 1) * sizeof(int64_t)));
+    const int64_t t0 = nrows > 0 ? tp[0] : 0, t1 = nrows > 0 ? tp[nrows] : 0;
+    const int64_t T = t1 - t0;
+    std::vector<int64_t> tn(static_cast<size_t>(T) + 1, 0);
+    if (T > 0)
+        RTT_TRY(rtt_memcpy_d2h(tn.data(), c.tile_nnz.as<int64_t>() + t0,
+                               (static_cast<size_t>(T) + 1) * sizeof(int64_t)));
+    const int64_t z0 = T > 0 ? tn[0] : 0, Z = T > 0 ? tn[T] - z0 : 0;
+    if (T > INT_MAX || Z > INT_MAX)
+        return fail(RTT_ERR_OVERFLOW, "problem block has %lld tiles / %lld nonzeros",
+                    (long long)T, (long long)Z);
+    rtt_smatrix r;
+    std::memset(&r, 0, sizeof(r));
+    r.m = m;
+    r.n = n;
+    r.tilem = static_cast<int>(tr_want);
+    r.tilen = (n + rtt::kTileN - 1) / rtt::kTileN;
+    r.numtile = static_cast<int>(T);
+    r.nnz = static_cast<int>(Z);
+    r.tile_ptr = static_cast<int *>(std::malloc((static_cast<size_t>(tr_want) + 1) * sizeof(int)));
+    r.tile_columnidx = static_cast<int *>(std::malloc(std::max<size_t>(T, 1) * sizeof(int)));
+    r.tile_nnz = static_cast<int *>(std::malloc((static_cast<size_t>(T) + 1) * sizeof(int)));
+    r.tile_csr_Ptr = static_cast<unsigned char *>(std::malloc(std::max<size_t>(8 * T, 1)));
+    r.tile_csr_Col = static_cast<unsigned char *>(std::malloc(std::max<size_t>(Z, 1)));
+    r.tile_csr_Value = static_cast<double *>(std::malloc(std::max<size_t>(Z, 1) * sizeof(double)));
+    if (!r.tile_ptr || !r.tile_columnidx || !r.tile_nnz || !r.tile_csr_Ptr || !r.tile_csr_Col ||
+        !r.tile_csr_Value) {
+        rtt_destroy_smatrix(&r);
+        return fail(RTT_ERR_OOM, "host allocation of the extracted block failed");
+    }
+    int rc = RTT_OK;
+    if (T > 0 && ((rc = rtt_memcpy_d2h(r.tile_columnidx, c.tile_col.as<int>() + t0,
+                                       static_cast<size_t>(T) * sizeof(int))) != RTT_OK ||
+                  (rc = rtt_memcpy_d2h(r.tile_csr_Ptr, c.tile_csr_ptr.as<uint8_t>() + 8 * t0,
+                                       static_cast<size_t>(8 * T))) != RTT_OK)) {
+        rtt_destroy_smatrix(&r);
+        return rc;
+    }
+    if (Z > 0 && ((rc = rtt_memcpy_d2h(r.tile_csr_Col, c.tile_csr_col.as<uint8_t>() + z0,
+                                       static_cast<size_t>(Z))) != RTT_OK ||
+                  (rc = rtt_memcpy_d2h(r.tile_csr_Value, c.tile_csr_val.as<double>() + z0,
+                                       static_cast<size_t>(Z) * sizeof(double))) != RTT_OK)) {
+        rtt_destroy_smatrix(&r);
+        return rc;
+    }
+    const int shift = static_cast<int>(c0 / rtt::kTileN);
+    for (int64_t i = 0; i <= tr_want; ++i)
+        r.tile_ptr[i] = static_cast<int>((i <= nrows ? tp[i] : t1) - t0);
+    for (int64_t t = 0; t < T; ++t)
+        r.tile_columnidx[t] -= shift;
+    for (int64_t t = 0; t <= T; ++t)
+        r.tile_nnz[t] = static_cast<int>(tn[t] - z0);
+    rtt_destroy_smatrix(out);
+    *out = r;
+    return RTT_OK;
+}
+
+int rtt_last_c_row_stats(const rtt_prepared_a *a, int64_t *row_nnz, double *row_sum)
+{
+    rtt::clear_error();
+    RTT_TRY(rtt::require_ctx());
+    if (!a || !row_nnz || !row_sum)
+        return fail(RTT_ERR_INVALID, "rtt_last_c_row_stats: NULL argument");
+    if (!a->has_c)
+        return fail(RTT_ERR_INVALID, "rtt_last_c_row_stats: no C has been produced");
+    const rtt::DevC &c = a->last_c;
+    const int64_t rows = c.tilem * rtt::kTileM;
+    std::fill(row_nnz, row_nnz + a->rows, int64_t(0));
+    std::fill(row_sum, row_sum + a->rows, 0.0);
+    if (rows == 0)
+        return RTT_OK;
+    rtt::DevBuf dn, ds;
+    RTT_TRY(dn.ensure(static_cast<size_t>(rows) * sizeof(int64_t)));
+    RTT_TRY(ds.ensure(static_cast<size_t>(rows) * sizeof(double)));
+    hipStream_t s = rtt::ctx().stream;
+    hipLaunchKernelGGL(c_row_stats_kernel, dim3(rtt::blocks_for(rows, 256)), dim3(256), 0, s, rows,
+                       c.tile_ptr.as<int64_t>(), c.tile_mask.as<uint64_t>(),
+                       c.tile_nnz.as<int64_t>(), c.tile_csr_val.as<double>(), dn.as<int64_t>(),
+                       ds.as<double>());
+    RTT_HIP(hipGetLastError());
+    const int64_t keep = std::min<int64_t>(rows, a->rows);
+    RTT_TRY(rtt_memcpy_d2h(row_nnz, dn.ptr, static_cast<size_t>(keep) * sizeof(int64_t)));
+    RTT_TRY(rtt_memcpy_d2h(row_sum, ds.ptr, static_cast<size_t>(keep) * sizeof(double)));
+    return RTT_OK;
+}
+
+int rtt_autotune(const rtt_prepared_a *a, const rtt_dynamic_b *b, int repeats, rtt_schedule *sched,
+                 rtt_tune_report *report)
+{
+    rtt::clear_error();
+    if (!a || !b || !sched || repeats < 1 || repeats > 64)
+        return fail(RTT_ERR_INVALID, "rtt_autotune: bad argument");
+    // Candidates: the numeric routing threshold (stored values per A tile
+    // above which a tile row takes the FP64 matrix-core tile-pair kernel
+    // instead of the LDS sparse accumulator; 0 = every row, inf = none) and,
+    // for the slab rows of the sparse kernels, the queue assignment.
+    struct Cand {
+        int path;
+        double fill;
+        int lb;
+    };
+    const Cand cands[] = {
+        {RTT_NUMERIC_VECTOR, 0.0, RTT_LB_ADAPTIVE_TAIL}, {RTT_NUMERIC_VECTOR, 0.0, RTT_LB_DYNAMIC},
+        {RTT_NUMERIC_AUTO, 24.0, RTT_LB_ADAPTIVE_TAIL},  {RTT_NUMERIC_AUTO, 16.0, RTT_LB_ADAPTIVE_TAIL},
+        {RTT_NUMERIC_AUTO, 12.0, RTT_LB_ADAPTIVE_TAIL},  {RTT_NUMERIC_AUTO, 8.0, RTT_LB_ADAPTIVE_TAIL},
+        {RTT_NUMERIC_TILE, 0.0, RTT_LB_ADAPTIVE_TAIL},
+    };
+    constexpr int kCands = static_cast<int>(sizeof(cands) / sizeof(cands[0]));
+    static_assert(kCands <= RTT_TUNE_MAX_CANDIDATES, "report size");
+    rtt_schedule base = *sched;
+    base.materialize_output = 0;
+    base.precision = RTT_PRECISION_FP64;
+    rtt_tune_report rep;
+    std::memset(&rep, 0, sizeof(rep));
+    rep.candidates = kCands;
+    int best = 0;
+    for (int i = 0; i < kCands; ++i) {
+        rtt_schedule sc = base;
+        sc.numeric_path = cands[i].path;
+        sc.tile_fill_threshold = cands[i].fill;
+        sc.lb_mode = cands[i].lb;
+        std::vector<double> t;
+        rtt_stats st;
+        for (int r = 0; r <= repeats; ++r) {   // the first run warms up
+            auto t0 = rtt::Clock::now();
+            RTT_TRY(rtt_tilespgemm(a, b, nullptr, &st, &sc));
+            if (r > 0)
+                t.push_back(rtt::ms_since(t0));
+        }
+        std::sort(t.begin(), t.end());
+        rep.ms[i] = t[t.size() / 2];
+        rep.numeric_path[i] = cands[i].path;
+        rep.fill[i] = cands[i].fill;
+        rep.lb_mode[i] = cands[i].lb;
+        if (rep.ms[i] < rep.ms[best])
+            best = i;
+    }
+    rep.chosen = best;
+    sched->numeric_path = cands[best].path;
+    sched->tile_fill_threshold = cands[best].path == RTT_NUMERIC_AUTO ? cands[best].fill
+                                                                      : sched->tile_fill_threshold;
+    sched->lb_mode = cands[best].lb;
+    if (report)
+        *report = rep;
     return RTT_OK;
 }
 

@@ -338,19 +338,6 @@ __device__ __forceinline__ int4 record_part(int part, const int4 &m, const doubl
 }
 
 // B's record planes, one part per thread.
-__global__ void records_kernel(int64_t n, const int4 *__restrict__ meta,
-                               const double *__restrict__ kmajor, int4 *__restrict__ planes)
-{
-    const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gid >= n * kRecParts)
-        return;
-    const int part = static_cast<int>(gid / n);
-    const int64_t t = gid - part * n;
-    const int4 m = meta[t];
-    if (part < record_parts_used(m))
-        planes[gid] = record_part(part, m, kmajor);
-}
-
 // Value parts (1 .. kRecParts-1) of B's record planes; part 0 (meta) is
 // left as it is.
 __global__ void record_values_kernel(int64_t n, const int4 *__restrict__ meta,
@@ -385,7 +372,8 @@ __global__ void scatter_runs_kernel(int64_t nnz, const int2 *__restrict__ slot,
     for (int64_t f = e + 1; f < nnz && slot[f].x == s.x; ++f)
         sum += v[f];
     values[s.x] = sum;
-    kmajor[s.y] = sum;
+    if (kmajor)
+        kmajor[s.y] = sum;
 }
 
 // Offline, per A tile: the record a B tile built as A^T takes over, its meta
@@ -501,23 +489,18 @@ __global__ void group_row_ptr_kernel(int rows, const int *__restrict__ groups,
         trp[r] = groups[static_cast<int64_t>(kClassSlots) * r];
 }
 
-// Gathers meta and record planes of a B built from CSR into class order.
-__global__ void gather_records_kernel(int64_t n, const int *__restrict__ perm,
-                                      const int4 *__restrict__ meta_in,
-                                      const int4 *__restrict__ rec_in,
-                                      int4 *__restrict__ meta_out, int4 *__restrict__ rec_out)
+// Gathers a B's meta into class order (the record value planes are built
+// afterwards, from the ordered meta, when a kernel needs them).
+__global__ void gather_meta_kernel(int64_t n, const int *__restrict__ perm,
+                                   const int4 *__restrict__ meta_in, int4 *__restrict__ meta_out,
+                                   int4 *__restrict__ plane0)
 {
-    const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gid >= n * kRecParts)
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n)
         return;
-    const int part = static_cast<int>(gid / n);
-    const int64_t t = gid - part * n;
-    const int src = perm[t];
-    if (part > 0 && part >= record_parts_used(meta_in[src]))
-        return;
-    rec_out[gid] = rec_in[part * n + src];
-    if (part == 0)
-        meta_out[t] = meta_in[src];
+    const int4 m = meta_in[perm[t]];
+    meta_out[t] = m;
+    plane0[t] = m;
 }
 
 __global__ void entry_rows_kernel(int rows, const int *__restrict__ rp,
@@ -738,15 +721,19 @@ void DevTiles::view(rtt_device_tiles *v) const
     v->csc_tile_ids = has_csc ? csc_tile_ids.as<int>() : nullptr;
 }
 
-int build_records(DevTiles *t)
+int ensure_record_values(DevTiles *b)
 {
-    RTT_TRY(build_colmajor_values(t));
-    RTT_TRY(t->records.ensure(t->num_tiles * sizeof(BRecord)));
-    if (t->num_tiles > 0)
-        hipLaunchKernelGGL(records_kernel, dim3(blocks_for(t->num_tiles * kRecParts, kThreads)),
-                           dim3(kThreads), 0, ctx().stream, t->num_tiles, t->meta.as<int4>(),
-                           t->values_cm.as<double>(), t->records.as<int4>());
+    if (b->transpose_of || b->rec_values_ready)
+        return RTT_OK;
+    RTT_TRY(build_colmajor_values(b));
+    RTT_TRY(b->records.ensure(b->num_tiles * sizeof(BRecord)));
+    if (b->num_tiles > 0)
+        hipLaunchKernelGGL(record_values_kernel,
+                           dim3(blocks_for(b->num_tiles * (kRecParts - 1), kThreads)),
+                           dim3(kThreads), 0, ctx().stream, b->num_tiles, b->meta.as<int4>(),
+                           b->values_cm.as<double>(), b->records.as<int4>());
     RTT_HIP(hipGetLastError());
+    b->rec_values_ready = true;
     return RTT_OK;
 }
 
@@ -783,22 +770,22 @@ int order_by_class(DevTiles *b, BuildWork *w)
         RTT_TRY(w->ids_in.ensure(n * sizeof(int)));
         RTT_TRY(w->ids_out.ensure(n * sizeof(int)));
         RTT_TRY(w->meta2.ensure(n * sizeof(int4)));
-        RTT_TRY(w->records2.ensure(n * sizeof(BRecord)));
+        RTT_TRY(b->records.ensure(n * sizeof(BRecord)));
         hipLaunchKernelGGL(class_keys_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
                            n, b->tile_row_of.as<int>(), b->meta.as<int4>(), w->keys_in.as<int>());
         hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
                            w->ids_in.as<int>(), n);
         RTT_TRY(sort_pairs(w->keys_in.as<int>(), w->keys_out.as<int>(), w->ids_in.as<int>(),
                            w->ids_out.as<int>(), n, bits_for(slots), &w->scan_tmp, s));
-        hipLaunchKernelGGL(gather_records_kernel, dim3(blocks_for(n * kRecParts, kThreads)),
-                           dim3(kThreads), 0, s, n, w->ids_out.as<int>(), b->meta.as<int4>(),
-                           b->records.as<int4>(), w->meta2.as<int4>(),
-                           w->records2.as<int4>());
+        // Meta in class order, also as record plane 0; the value planes
+        // follow from it in ensure_record_values.
+        hipLaunchKernelGGL(gather_meta_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0,
+                           s, n, w->ids_out.as<int>(), b->meta.as<int4>(), w->meta2.as<int4>(),
+                           b->records.as<int4>());
         RTT_HIP(hipMemcpyAsync(b->meta.ptr, w->meta2.ptr, n * sizeof(int4),
                                hipMemcpyDeviceToDevice, s));
-        RTT_HIP(hipMemcpyAsync(b->records.ptr, w->records2.ptr, n * sizeof(BRecord),
-                               hipMemcpyDeviceToDevice, s));
     }
+    b->rec_values_ready = false;
     hipLaunchKernelGGL(row_ptr_from_sorted_kernel, dim3(blocks_for(n + 1, kThreads)),
                        dim3(kThreads), 0, s, w->keys_out.as<int>(), n, static_cast<int>(slots),
                        b->row_groups.as<int>());
@@ -1322,18 +1309,23 @@ int update_b_values(const DevCsr &src, bool clear, DevTiles *b)
     if (b->map_source_id == 0 || b->map_source_id != src.structure_id ||
         b->map_nnz != src.nnz || b->transpose_of)
         return fail(RTT_ERR_INVALID, "B has no source map for this CSR");
+    // The k-major copy and the record value planes are kept current only
+    // once a kernel has asked for them (ensure_record_values); otherwise they
+    // are rebuilt from the payload when next needed.
+    const bool kmajor = b->rec_values_ready;
     if (clear && b->payload > 0) {
         hipLaunchKernelGGL(fill_f64_kernel, dim3(blocks_for(b->payload, kThreads)),
                            dim3(kThreads), 0, s, b->values.as<double>(), b->payload, 0.0);
-        hipLaunchKernelGGL(fill_f64_kernel, dim3(blocks_for(b->payload, kThreads)),
-                           dim3(kThreads), 0, s, b->values_cm.as<double>(), b->payload, 0.0);
+        if (kmajor)
+            hipLaunchKernelGGL(fill_f64_kernel, dim3(blocks_for(b->payload, kThreads)),
+                               dim3(kThreads), 0, s, b->values_cm.as<double>(), b->payload, 0.0);
     }
     if (src.nnz > 0)
         hipLaunchKernelGGL(scatter_runs_kernel, dim3(blocks_for(src.nnz, kThreads)),
                            dim3(kThreads), 0, s, src.nnz, b->src_slot.as<int2>(),
                            src.values.as<double>(), b->values.as<double>(),
-                           b->values_cm.as<double>());
-    if (b->num_tiles > 0)
+                           kmajor ? b->values_cm.as<double>() : nullptr);
+    if (kmajor && b->num_tiles > 0)
         hipLaunchKernelGGL(record_values_kernel,
                            dim3(blocks_for(b->num_tiles * (kRecParts - 1), kThreads)),
                            dim3(kThreads), 0, s, b->num_tiles, b->meta.as<int4>(),

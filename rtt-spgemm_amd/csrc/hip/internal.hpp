@@ -103,6 +103,7 @@ struct DevTiles {
     int64_t num_tiles = 0, payload = 0, dense = 0, sparse = 0;
     bool has_csc = false;       // tile_col_ptr / tile_row_idx / csc_tile_ids valid
     bool cols_ready = false;    // col_ptr / col_ent valid (B built from a CSR)
+    bool rec_values_ready = true;   // B from a CSR: values_cm and record planes 1.. valid
     bool arrays_ready = true;   // tile_col_idx / masks / value_offsets / values valid
     DevBuf tile_row_ptr, tile_col_idx, value_offsets, masks, values;
     DevBuf tile_col_ptr, tile_row_idx, csc_tile_ids;
@@ -170,6 +171,7 @@ struct SpgemmWork {
     uint64_t slab_stage_cap = 0, slab_stage_need = 0;
     DevBuf tile_queue;                      // row-queue heads of the tile-pair kernels
     DevBuf tile_pairs;                      // u64: pairs the tile-pair kernel multiplied
+    DevBuf tile_rows;                       // u64 x 2: rows counted for routing
 };
 
 // Scratch of the tile builders, reused across calls.
@@ -231,8 +233,6 @@ int build_colmajor_values(DevTiles *t);
 // Fills t->transpose_src (A tiles, offline; needs tile_row_of) and
 // t->row_payload_off.
 int build_transpose_source(DevTiles *t);
-// Fills t->values_cm and t->records (meta + first k-major values).
-int build_records(DevTiles *t);
 // b->a_brange[t] = {B.tile_row_ptr[K], heavy tiles, count} and b->a_blight[t]
 // for every A tile t in column K.
 int build_pair_extents(const DevTiles &a, DevTiles *b);
@@ -240,9 +240,14 @@ int build_pair_extents(const DevTiles &a, DevTiles *b);
 // built from a CSR; no-op for B = A^T at tile level, whose columns are A's
 // tile rows (b_columns below).
 int build_b_columns(DevTiles *b, BuildWork *w);
-// Reorders b->meta and b->records of a B built from CSR into class order
-// within every row (stable) and fills b->row_groups.
+// Reorders b->meta of a B built from CSR into class order within every row
+// (stable), writes it as record plane 0 and fills b->row_groups.  The record
+// value planes and the k-major payload are left to ensure_record_values.
 int order_by_class(DevTiles *b, BuildWork *w);
+// Builds values_cm and the record value planes of a B built from a CSR when
+// they are missing (the LDS-accumulator numeric kernels read them; the
+// symbolic and tile-pair kernels do not).
+int ensure_record_values(DevTiles *b);
 
 // ---- tile-pair numeric path (tiletask.hip) ---------------------------------
 // B's tile columns as the tile-pair kernels read them: ptr[J]..ptr[J+1] index
@@ -254,11 +259,24 @@ struct TileColumns {
     const double *vals = nullptr;
 };
 TileColumns b_columns(const DevTiles &b);
-// Values of every C tile from C's finished structure: precision 0 on FP64
-// matrix cores, 1 by BF16 slices (Ozaki scheme) with FP64 recombination.
-// Deterministic: each tile sums its pairs in increasing K.
+// Values of the C tiles of every tile row whose A tiles hold at least `fill`
+// stored values on average (0: every row), from C's finished structure:
+// precision 0 on FP64 matrix cores, 1 by BF16 slices (Ozaki scheme) with
+// FP64 recombination.  Deterministic: each tile sums its pairs in
+// increasing K.
 int tile_numeric(const DevTiles &a, const TileColumns &bc, const DevC &c, int precision,
-                 SpgemmWork *w, hipStream_t s);
+                 float fill, SpgemmWork *w, hipStream_t s);
+// Tile rows of A the tile-pair kernel would take at `fill`, and the tile
+// rows holding any tile (one readback).
+int count_tile_rows(const DevTiles &a, float fill, DevBuf *scratch, int64_t *tile_rows,
+                    int64_t *nonempty_rows);
+// Per-row numeric routing of one call: rows whose A tiles hold at least
+// `fill` stored values on average take the tile-pair kernel in `precision`.
+struct TileOpts {
+    float fill = 0.f;
+    int precision = 0;
+    TileColumns bc;
+};
 
 // ---- SpGEMM core (spgemm.hip) ---------------------------------------------
 struct CoreTimes {
@@ -271,12 +289,12 @@ struct CoreTimes {
 // hold one workgroup per row for many rows: static hands row i to
 // workgroup i mod grid; dynamic takes rows from a work queue in list order;
 // adaptive-tail takes them from the queue heaviest first (most pairs).
-// numeric_path 0 runs the LDS-accumulator kernels; 1 (FP64) and 2 (split
-// BF16) write C's structure for every row first and compute the values with
-// the tile-pair kernel (bc: B's tile columns).
+// tile == nullptr runs the LDS-accumulator kernels for every row; otherwise
+// the rows tile->fill selects have their structure written first and their
+// values computed by the tile-pair kernel, the others as before.
 int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *work,
                 Events *events, CoreTimes *times, int lb_mode = RTT_LB_ADAPTIVE_TAIL,
-                int numeric_path = 0, const TileColumns *bc = nullptr);
+                const TileOpts *tile = nullptr);
 
 // ---- scans (scan.hip) -------------------------------------------------------
 // Exclusive sum of n int64 values into out[0..n] (out[n] = total).

@@ -27,6 +27,7 @@
 //    streamed to HBM.  Rows that exceed the LDS budgets fall back to a
 //    workgroup path with FP64 atomics in HBM.
 #include <climits>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -100,7 +101,16 @@ struct Operands {
     const int *__restrict__ b_groups;       // per B row: start of each class run (kClassSlots)
     const double *__restrict__ b_vals;
     int b_tile_cols;
+    float tile_fill;   // rows whose A tiles hold this many values on average go to the
+                       // tile-pair kernel (+inf: none)
 };
+
+// The tile-pair kernel takes the tile rows whose A tiles hold at least
+// `fill` stored values on average; the LDS-accumulator kernels the others.
+__device__ __forceinline__ bool tile_routed(float fill, int tiles, int payload)
+{
+    return tiles > 0 && static_cast<float>(payload) >= fill * static_cast<float>(tiles);
+}
 
 struct COut {
     const int64_t *__restrict__ tile_ptr;     // tilem + 1
@@ -625,7 +635,9 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
 // packed in increasing order, and each tile row ORs them into the zeroed
 // buffer with at most two 64-bit LDS ORs.
 __global__ __launch_bounds__(kThreads) void symbolic_write_kernel(
-    const int *__restrict__ rows, int count, Stage st, const int *__restrict__ row_level, COut c)
+    const int *__restrict__ rows, int count, Stage st, const int *__restrict__ row_level, COut c,
+    float tile_fill = 0.f, const int *__restrict__ a_trp = nullptr,
+    const int *__restrict__ a_row_off = nullptr)
 {
     __shared__ unsigned long long s_buf[kWavesPerBlock][kWriteBuf / 8];
     __shared__ unsigned long long s_cols[256];
@@ -647,6 +659,10 @@ __global__ __launch_bounds__(kThreads) void symbolic_write_kernel(
         return;
     const int row = rows ? rows[idx] : idx;
     if (row_level[row] != kLevelLds)
+        return;
+    // Without a list: the rows the tile-pair kernel takes.
+    if (!rows && !tile_routed(tile_fill, a_trp[row + 1] - a_trp[row],
+                              a_row_off[row + 1] - a_row_off[row]))
         return;
     const int64_t tb = c.tile_ptr[row];
     const int nt = static_cast<int>(c.tile_ptr[row + 1] - tb);
@@ -1151,7 +1167,7 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     const int a_base = g.a_row_off[row];
     const int a_len = g.a_row_off[row + 1] - a_base;
     if ((level != kLevelLds) | (nt > Cfg::kTiles) | (nnz_row > Cfg::kAcc) |
-        (a_len > Cfg::kAStage)) {
+        (a_len > Cfg::kAStage) | tile_routed(g.tile_fill, a1 - a0, a_len)) {
         if constexpr (TRACE)
             trace_mark(tl, true);
         return;
@@ -1755,34 +1771,55 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
 // Lists rows for a fallback path.  mode 0: symbolic slab rows; mode 1:
 // numeric rows beyond the small LDS instance, split into the large LDS
 // instance's list (out_lds) and the slab list (out_slab).
+// Appends v to out (count holds the length) for the lanes where pred holds:
+// one atomic per wave, positions in lane order within the wave.
+__device__ __forceinline__ void wave_append(bool pred, int v, int *__restrict__ out,
+                                            int *__restrict__ count)
+{
+    const unsigned long long m = __ballot(pred);
+    if (!m)
+        return;
+    const int leader = __ffsll(static_cast<long long>(m)) - 1;
+    int base = 0;
+    if (lane_id() == leader)
+        base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, leader, kWave);
+    if (pred)
+        out[base + __popcll(m & ((1ull << lane_id()) - 1ull))] = v;
+}
+
 __global__ void collect_rows_kernel(int n, int mode, const int *__restrict__ level,
                                     const int64_t *__restrict__ tile_ptr,
                                     const int64_t *__restrict__ nnz_off,
                                     const int *__restrict__ a_trp, const int *__restrict__ a_off,
                                     int *__restrict__ out_slab, int *__restrict__ n_slab,
-                                    int *__restrict__ out_lds, int *__restrict__ n_lds)
+                                    int *__restrict__ out_lds, int *__restrict__ n_lds,
+                                    float tile_fill = INFINITY)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    if (mode == 0 || mode == 2) {
-        if (level[i] == (mode == 0 ? kLevelSlab : kLevelLarge))
-            out_slab[atomicAdd(n_slab, 1)] = i;
-        return;
+    bool to_slab = false, to_lds = false;
+    if (i < n) {
+        const int lv = level[i];
+        if (mode == 0 || mode == 2) {
+            to_slab = lv == (mode == 0 ? kLevelSlab : kLevelLarge);
+        } else if (tile_routed(tile_fill, a_trp[i + 1] - a_trp[i],
+                               a_off[a_trp[i + 1]] - a_off[a_trp[i]])) {
+            // numeric on the tile-pair kernel
+        } else if (lv == kLevelSlab) {
+            to_slab = true;
+        } else if (lv == kLevelLds) {
+            const int64_t nt = tile_ptr[i + 1] - tile_ptr[i];
+            const int64_t nz = nnz_off[i + 1] - nnz_off[i];
+            const int64_t alen = a_off[a_trp[i + 1]] - a_off[a_trp[i]];
+            if (nt > LargeRow::kTiles || nz > LargeRow::kAcc || alen > LargeRow::kAStage)
+                to_slab = true;
+            else if (nt > SmallRow::kTiles || nz > SmallRow::kAcc || alen > SmallRow::kAStage)
+                to_lds = true;
+        }
     }
-    if (level[i] == kLevelSlab) {
-        out_slab[atomicAdd(n_slab, 1)] = i;
-        return;
-    }
-    if (level[i] != kLevelLds)
-        return;
-    const int64_t nt = tile_ptr[i + 1] - tile_ptr[i];
-    const int64_t nz = nnz_off[i + 1] - nnz_off[i];
-    const int64_t alen = a_off[a_trp[i + 1]] - a_off[a_trp[i]];
-    if (nt > LargeRow::kTiles || nz > LargeRow::kAcc || alen > LargeRow::kAStage)
-        out_slab[atomicAdd(n_slab, 1)] = i;
-    else if (nt > SmallRow::kTiles || nz > SmallRow::kAcc || alen > SmallRow::kAStage)
-        out_lds[atomicAdd(n_lds, 1)] = i;
+    wave_append(to_slab, i, out_slab, n_slab);
+    if (mode == 1)
+        wave_append(to_lds, i, out_lds, n_lds);
 }
 
 // Number of (A tile, B tile) pairs: the sum of the per-A-tile extents, one
@@ -1894,10 +1931,12 @@ static int order_rows_by_cost(const int *rows, int n, const Operands &g, SpgemmW
 }
 
 int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Events *ev,
-                CoreTimes *times, int lb_mode, int numeric_path, const TileColumns *bc)
+                CoreTimes *times, int lb_mode, const TileOpts *tile)
 {
-    if (numeric_path != 0 && !bc)
-        return fail(RTT_ERR_INVALID, "tile-pair numeric path without B's tile columns");
+    // tile: rows routed to the tile-pair kernel (all when fill is 0);
+    // vector: rows left to the LDS-accumulator kernels.
+    const bool any_tile = tile != nullptr;
+    const bool any_vector = !tile || tile->fill > 0.f;
     RTT_TRY(require_ctx());
     RTT_TRY(ev->create());
     hipStream_t s = ctx().stream;
@@ -1924,6 +1963,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     g.b_groups = b.row_groups.as<int>();
     g.b_vals = b.payload_values();
     g.b_tile_cols = BC;
+    g.tile_fill = tile ? tile->fill : INFINITY;
 
     const int num_cus = ctx().num_cus > 0 ? ctx().num_cus : 256;
     // Counters read back by the host: [0] pairs, [1] symbolic slab rows,
@@ -2049,13 +2089,14 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     RTT_TRY(exclusive_scan_i64(w->row_nnz.as<int64_t>(), w->c_row_nnz_off.as<int64_t>(), R,
                                &w->scan_tmp, s));
     // Numeric row routing needs only the per-row sizes: list the fallback
-    // rows now and read every count back with one copy.
-    if (R > 0)
+    // rows now and read every count back with one copy.  Rows of the
+    // tile-pair kernel are not listed.
+    if (R > 0 && any_vector)
         hipLaunchKernelGGL(collect_rows_kernel, dim3(blocks_for(R, kThreads)), dim3(kThreads), 0,
                            s, R, 1, w->row_level.as<int>(), c->tile_ptr.as<int64_t>(),
                            w->c_row_nnz_off.as<int64_t>(), g.a_trp, g.a_off,
                            w->num_rows.as<int>(), num_slab_count, w->lds_rows.as<int>(),
-                           num_lds_count);
+                           num_lds_count, g.tile_fill);
     hipLaunchKernelGGL(set_last_kernel, dim3(1), dim3(1), 0, s,
                        reinterpret_cast<int64_t *>(ctr + 4), c->tile_ptr.as<int64_t>() + R);
     hipLaunchKernelGGL(set_last_kernel, dim3(1), dim3(1), 0, s,
@@ -2098,13 +2139,14 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     // staged lists; this pass writes it for the staged rows they cannot hold
     // (the numeric fallback list, whose slab rows it skips).  The tile-pair
     // path needs the whole structure first: every staged row.
-    if (numeric_path != 0 && R > 0 && c->num_tiles > 0)
+    if (any_tile && R > 0 && c->num_tiles > 0)
         hipLaunchKernelGGL(symbolic_write_kernel, dim3(blocks_for(R, kWavesPerBlock)),
-                           dim3(kThreads), 0, s, nullptr, R, st, w->row_level.as<int>(), co);
-    else if (n_num_slab > 0 && c->num_tiles > 0)
+                           dim3(kThreads), 0, s, nullptr, R, st, w->row_level.as<int>(), co,
+                           g.tile_fill, g.a_trp, g.a_row_off);
+    if (n_num_slab > 0 && c->num_tiles > 0)
         hipLaunchKernelGGL(symbolic_write_kernel, dim3(blocks_for(n_num_slab, kWavesPerBlock)),
                            dim3(kThreads), 0, s, w->num_rows.as<int>(), n_num_slab, st,
-                           w->row_level.as<int>(), co);
+                           w->row_level.as<int>(), co, 0.f, nullptr, nullptr);
     RTT_HIP(hipGetLastError());
     RTT_HIP(hipEventRecord(ev->ev[3], s));
     if (n_slab > 0)
@@ -2145,10 +2187,11 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
         }
     }
     RTT_HIP(hipEventRecord(ev->ev[4], s));
-    if (numeric_path != 0) {
-        RTT_TRY(tile_numeric(a, *bc, *c, numeric_path - 1, w, s));
+    if (any_tile) {
+        RTT_TRY(tile_numeric(a, tile->bc, *c, tile->precision, tile->fill, w, s));
         RTT_HIP(hipEventRecord(ev->ev[5], s));
-    } else if (R > 0 && c->nnz > 0) {
+    }
+    if (any_vector && R > 0 && c->nnz > 0) {
         if (trace_slots)
             hipLaunchKernelGGL((numeric_row_kernel<SmallRow, true>), dim3(R), dim3(kThreads), 0,
                                s, g, b.records.as<int4>(), b.num_tiles, nullptr,
@@ -2159,13 +2202,13 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
                                w->row_level.as<int>(), st, co, tl);
     }
     RTT_HIP(hipGetLastError());
-    if (numeric_path == 0)
+    if (!any_tile)
         RTT_HIP(hipEventRecord(ev->ev[5], s));
-    if (numeric_path == 0 && n_num_lds > 0)
+    if (n_num_lds > 0)
         hipLaunchKernelGGL((numeric_row_kernel<LargeRow>), dim3(n_num_lds), dim3(kThreads), 0,
                            s, g, b.records.as<int4>(), b.num_tiles, w->lds_rows.as<int>(),
                            w->row_level.as<int>(), st, co, Timeline{});
-    if (numeric_path == 0 && n_num_slab > 0) {
+    if (n_num_slab > 0) {
         const int blocks = std::min(n_num_slab, 4 * num_cus);
         DevBuf &tab = w->slab;   // reused: the symbolic slab pass has been queued before
         const int64_t ents = static_cast<int64_t>(blocks) * BC;
@@ -2183,7 +2226,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     }
     RTT_HIP(hipEventRecord(ev->ev[7], s));
     RTT_HIP(hipStreamSynchronize(s));
-    if (numeric_path != 0 && c->num_tiles > 0)
+    if (any_tile && c->num_tiles > 0)
         RTT_HIP(hipMemcpy(&times->mfma_pairs, w->tile_pairs.ptr, sizeof(unsigned long long),
                           hipMemcpyDeviceToHost));
     if (trace_slots)

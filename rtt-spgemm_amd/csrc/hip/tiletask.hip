@@ -53,6 +53,8 @@ struct TileTask {
     const unsigned long long *__restrict__ c_mask;
     const int64_t *__restrict__ c_off;   // value offset of each C tile
     double *__restrict__ c_val;
+    const int *__restrict__ a_row_off;   // A payload offset of each tile row
+    float fill;                          // rows taken: A tiles' mean stored values >= fill
     int *__restrict__ queue;             // kXcds row-queue heads, zero at launch
     unsigned long long *__restrict__ mfma_pairs;   // pairs multiplied (one MFMA each)
 };
@@ -422,6 +424,10 @@ __global__ __launch_bounds__(kTThreads) void tile_numeric_kernel(TileTask g)
                 continue;
             const int a0 = g.a_trp[row];
             const int na = g.a_trp[row + 1] - a0;
+            // Rows below the fill threshold belong to the LDS-accumulator kernels.
+            if (static_cast<float>(g.a_row_off[row + 1] - g.a_row_off[row]) <
+                g.fill * static_cast<float>(na))
+                continue;
             const bool lds_keys = na <= kAKeys;
             if (lds_keys) {
                 for (int i = lane; i < na; i += kWave)
@@ -528,8 +534,48 @@ TileColumns b_columns(const DevTiles &b)
     return v;
 }
 
+__global__ void count_tile_rows_kernel(int rows, const int *__restrict__ trp,
+                                       const int *__restrict__ row_off, float fill,
+                                       unsigned long long *__restrict__ count)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    bool take = false;
+    if (r < rows) {
+        const int t = trp[r + 1] - trp[r];
+        take = t > 0 && static_cast<float>(row_off[r + 1] - row_off[r]) >= fill * static_cast<float>(t);
+    }
+    const unsigned long long m = __ballot(take);
+    if (lane_id() == 0 && m)
+        atomicAdd(count, static_cast<unsigned long long>(__popcll(m)));
+}
+
+int count_tile_rows(const DevTiles &a, float fill, DevBuf *scratch, int64_t *tile_rows,
+                    int64_t *nonempty_rows)
+{
+    hipStream_t s = ctx().stream;
+    RTT_TRY(scratch->ensure(2 * sizeof(unsigned long long)));
+    unsigned long long *d = scratch->as<unsigned long long>();
+    hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(2), 0, s, reinterpret_cast<int *>(d), 0, d);
+    hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(2), 0, s, reinterpret_cast<int *>(d), 0,
+                       d + 1);
+    const int R = a.tile_row_count;
+    if (R > 0) {
+        hipLaunchKernelGGL(count_tile_rows_kernel, dim3(blocks_for(R, 256)), dim3(256), 0, s, R,
+                           a.tile_row_ptr.as<int>(), a.row_payload_off.as<int>(), fill, d);
+        hipLaunchKernelGGL(count_tile_rows_kernel, dim3(blocks_for(R, 256)), dim3(256), 0, s, R,
+                           a.tile_row_ptr.as<int>(), a.row_payload_off.as<int>(), 0.f, d + 1);
+    }
+    RTT_HIP(hipGetLastError());
+    unsigned long long h[2] = {0, 0};
+    RTT_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s));
+    RTT_HIP(hipStreamSynchronize(s));
+    *tile_rows = static_cast<int64_t>(h[0]);
+    *nonempty_rows = static_cast<int64_t>(h[1]);
+    return RTT_OK;
+}
+
 int tile_numeric(const DevTiles &a, const TileColumns &bc, const DevC &c, int precision,
-                 SpgemmWork *w, hipStream_t s)
+                 float fill, SpgemmWork *w, hipStream_t s)
 {
     if (c.tilem == 0 || c.num_tiles == 0)
         return RTT_OK;
@@ -550,6 +596,8 @@ int tile_numeric(const DevTiles &a, const TileColumns &bc, const DevC &c, int pr
     g.c_mask = c.tile_mask.as<unsigned long long>();
     g.c_off = c.tile_nnz.as<int64_t>();
     g.c_val = c.tile_csr_val.as<double>();
+    g.a_row_off = a.row_payload_off.as<int>();
+    g.fill = fill;
     g.queue = w->tile_queue.as<int>();
     g.mfma_pairs = w->tile_pairs.as<unsigned long long>();
     hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(kXcds), 0, s, g.queue, kXcds,

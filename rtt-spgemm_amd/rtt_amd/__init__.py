@@ -124,6 +124,12 @@ class Stats(C.Structure):
                 ("mfma_pairs", C.c_ulonglong), ("precision", C.c_int)]
 
 
+class TuneReport(C.Structure):
+    _fields_ = [("candidates", C.c_int), ("chosen", C.c_int), ("ms", C.c_double * 16),
+                ("numeric_path", C.c_int * 16), ("fill", C.c_double * 16),
+                ("lb_mode", C.c_int * 16)]
+
+
 class DeviceC(C.Structure):
     _fields_ = [("tilem", C.c_int64), ("tilen", C.c_int64), ("num_tiles", C.c_int64),
                 ("nnz", C.c_int64), ("tile_ptr", C.c_void_p), ("tile_col", C.c_void_p),
@@ -204,6 +210,10 @@ def load() -> C.CDLL:
                                                       P(SMatrix)]),
         "rtt_csr_transpose": (C.c_int, [P(SMatrix), P(SMatrix)]),
         "rtt_block_diag": (C.c_int, [C.c_int, vp, C.c_int, P(SMatrix), vp, vp]),
+        "rtt_batch_extract": (C.c_int, [vp, P(C.c_int64), P(C.c_int64), C.c_int, C.c_int,
+                                        P(SMatrix)]),
+        "rtt_autotune": (C.c_int, [vp, vp, C.c_int, P(Schedule), P(TuneReport)]),
+        "rtt_last_c_row_stats": (C.c_int, [vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -754,6 +764,39 @@ def tilespgemm(a: PreparedA, b: DynamicB, sched: Schedule | None = None):
         finally:
             load().rtt_destroy_smatrix(C.byref(out))
     return tc, stats
+
+
+def batch_extract(a: PreparedA, row_offset: int, col_offset: int, m: int, n: int) -> TileC:
+    """Problem block (m x n at the 8-aligned offsets) of the last C on `a`."""
+    out = SMatrix()
+    _check(load().rtt_batch_extract(a.handle, C.byref(C.c_int64(int(row_offset))),
+                                    C.byref(C.c_int64(int(col_offset))), m, n, C.byref(out)),
+           "batch_extract")
+    try:
+        return _tilec_from_smatrix(out)
+    finally:
+        load().rtt_destroy_smatrix(C.byref(out))
+
+
+def autotune(a: PreparedA, b: DynamicB, repeats: int = 2, sched: Schedule | None = None):
+    """Fastest schedule for (a, b) among the tuner's candidates, and the table."""
+    sched = sched or schedule(materialize_output=False)
+    rep = TuneReport()
+    _check(load().rtt_autotune(a.handle, b.handle, repeats, C.byref(sched), C.byref(rep)),
+           "autotune")
+    inv = {v: k for k, v in NUMERIC_PATHS.items()}
+    lbinv = {v: k for k, v in LB_MODES.items()}
+    table = [{"numeric_path": inv[rep.numeric_path[i]], "fill": rep.fill[i],
+              "lb_mode": lbinv[rep.lb_mode[i]], "ms": rep.ms[i]} for i in range(rep.candidates)]
+    return sched, table, rep.chosen
+
+
+def last_c_row_stats(a: PreparedA):
+    """(nnz, value sum) of every row of the last C on `a`, computed on the GPU."""
+    n = np.zeros(max(a.m, 1), np.int64)
+    v = np.zeros(max(a.m, 1), np.float64)
+    _check(load().rtt_last_c_row_stats(a.handle, _ptr(n), _ptr(v)), "last_c_row_stats")
+    return n[:a.m], v[:a.m]
 
 
 def last_c(a: PreparedA, with_values: bool = True) -> dict:

@@ -100,12 +100,12 @@ def check_c(rt, tc, ref, positive=False):
     return err
 
 
-def run(rt, a, aat, positive=False, dense_threshold=24):
+def run(rt, a, aat, positive=False, dense_threshold=24, sched=None):
     pa = rt.prepare_identity_a(a, dense_threshold)
     check_a_tiles(rt, a, pa, dense_threshold)
     pb = rt.rebuild_b(pa, aat)
     check_b_tiles(rt, a, pa, pb, aat, dense_threshold)
-    tc, stats = rt.tilespgemm(pa, pb)
+    tc, stats = rt.tilespgemm(pa, pb, sched)
     ref = oracle_product(a, aat)
     err = check_c(rt, tc, ref, positive)
     assert stats["output_nnz"] == ref["nnz"]
@@ -220,7 +220,8 @@ def test_wide_rows_take_the_slab_paths(gpu, aat):
             vals.append(float(rng.uniform(0.5, 1.5)))
     a = csr_from(n, n, rows, cols, vals)
     # merge duplicate (i, i) entries for rows < 3 are fine: duplicates allowed
-    _, _, tc, stats, _ = run(gpu, a, aat, positive=True)
+    _, _, tc, stats, _ = run(gpu, a, aat, positive=True,
+                             sched=gpu.schedule(numeric_path="vector"))
     assert stats["heavy_rows"] > 0
 
 
@@ -247,7 +248,10 @@ def test_slab_row_beyond_the_window_walk(gpu, aat):
     keep = np.unique(rows.astype(np.int64) * n + cols, return_index=True)[1]
     rows, cols = rows[keep], cols[keep]
     a = csr_from(n, n, rows, cols, rng.choice([0.5, 1.0, 1.5], len(rows)))
-    pa, _, _, stats, _ = run(gpu, a, aat, positive=True)
+    # The first tile row's A tiles hold 16 values each, which AUTO would send
+    # to the tile-pair kernel: the LDS-accumulator path is what is tested here.
+    pa, _, _, stats, _ = run(gpu, a, aat, positive=True,
+                             sched=gpu.schedule(numeric_path="vector"))
     assert stats["heavy_rows"] > 0
     assert pa.tiles()["tile_row_ptr"][1] > 8192
 
@@ -258,8 +262,11 @@ def test_rmat_power_law(gpu, scale, aat):
     # C tile rows wider than the LDS tables, so many rows take the slab
     # paths, and B rows mix heavy and light tiles of every class.
     a = gpu.gen_rmat(scale, 16, 0.57, 0.19, 0.19, 20260101)
-    _, _, _, stats, _ = run(gpu, a, aat)
+    _, _, _, stats, _ = run(gpu, a, aat, sched=gpu.schedule(numeric_path="vector"))
     assert stats["heavy_rows"] > 0
+    # AUTO (hub tile rows dense enough take the tile-pair kernel) gives the
+    # oracle's C as well
+    run(gpu, a, aat)
 
 
 def test_repeat_calls_are_deterministic_on_slab_rows(gpu):
@@ -268,9 +275,10 @@ def test_repeat_calls_are_deterministic_on_slab_rows(gpu):
     a = gpu.gen_rmat(14, 16, 0.57, 0.19, 0.19, 7)
     pa = gpu.prepare_identity_a(a)
     pb = gpu.rebuild_b(pa, 0)
-    tc1, s1 = gpu.tilespgemm(pa, pb)
+    vec = gpu.schedule(numeric_path="vector")
+    tc1, s1 = gpu.tilespgemm(pa, pb, vec)
     pb = gpu.rebuild_b(pa, 0, pb)
-    tc2, s2 = gpu.tilespgemm(pa, pb)
+    tc2, s2 = gpu.tilespgemm(pa, pb, vec)
     assert s1["heavy_rows"] > 0
     assert (s1["output_tiles"], s1["output_nnz"]) == (s2["output_tiles"], s2["output_nnz"])
     for k in ("tile_ptr", "tile_columnidx", "tile_nnz", "tile_csr_Ptr", "tile_csr_Col"):
@@ -291,7 +299,7 @@ def test_lb_modes_agree_on_slab_rows(gpu, lb_mode):
     # The first call on a handle learns how much the slab rows' tile lists
     # need; later calls stage them between the symbolic passes.
     for _ in range(2):
-        tc, stats = gpu.tilespgemm(pa, pb, gpu.schedule(lb_mode=lb_mode))
+        tc, stats = gpu.tilespgemm(pa, pb, gpu.schedule(lb_mode=lb_mode, numeric_path="vector"))
         assert stats["heavy_rows"] > 0
         check_c(gpu, tc, ref)
 

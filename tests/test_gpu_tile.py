@@ -201,3 +201,24 @@ def test_split_precision_rejects_the_vector_path(gpu):
     pb = gpu.rebuild_b(pa, 0)
     with pytest.raises(gpu.RttError):
         gpu.tilespgemm(pa, pb, gpu.schedule(numeric_path="vector", precision="split-bf16"))
+
+
+def test_auto_routes_rows_of_one_call_to_both_paths(gpu):
+    """A block-diagonal batch of a block-FEM matrix (16..32 stored values per
+    A tile) and a sparse random one (about 2): AUTO sends the FEM tile rows
+    to the tile-pair kernel and the others to the LDS-accumulator kernels in
+    the same call, and C is the oracle's."""
+    f = gpu.gen_fem_blocks(700, 6, True, 21)
+    r = gpu.gen_random(3000, 3000, 5, 22)
+    big, ro, co = gpu.block_diag([f, r, f], 8)
+    pa = gpu.prepare_identity_a(big)
+    pb = gpu.rebuild_b(pa, 0)
+    tc, st = gpu.tilespgemm(pa, pb)
+    assert st["numeric_path"] == gpu.NUMERIC_PATHS["auto"]
+    assert 0 < st["mfma_pairs"] < st["pair_count"]
+    check_c(gpu, tc, oracle_product(big, 0))
+    # the vector rows' neighbours in AA^T (general transpose path) as well
+    pb2 = gpu.rebuild_b(pa, 1)
+    tc2, st2 = gpu.tilespgemm(pa, pb2)
+    assert st2["numeric_path"] == gpu.NUMERIC_PATHS["auto"]
+    check_c(gpu, tc2, oracle_product(big, 1))
