@@ -161,7 +161,7 @@ def batch_amg(rt, data, steps, warmup, precision, sequential=True):
             ap = rt.tile2csr(tc, canonical=True)
             sap = rt.upload_csr(ap)
             out.append({"pa": pa, "sp": sp_, "pr": pr, "sap": sap, "pb": None, "pb2": None,
-                        "cub": rt.nnz_cub_ab(pa, sp_) + rt.nnz_cub_ab(pr, sap)})
+                        "ap": ap, "cub": rt.nnz_cub_ab(pa, sp_) + rt.nnz_cub_ab(pr, sap)})
         return out
 
     sched = rt.schedule(materialize_output=False, precision=precision)
@@ -181,6 +181,29 @@ def batch_amg(rt, data, steps, warmup, precision, sequential=True):
     big = prep([bA], [bP], [bR])
     ms_b = timed_steps(lambda: run(big), steps, warmup)
     res = {"nnz_cub": big[0]["cub"], "ms_batched": ms_b}
+    if precision != "fp64":
+        # Errors of both products of the step against the FP64 kernels, scaled
+        # per entry by the product of the absolute operands (the same FP64
+        # kernels on |A|, |P| and |R|, |A*P|).
+        it = big[0]
+        got = [rt.last_c(it["pa"])["tile_csr_val"], rt.last_c(it["pr"])["tile_csr_val"]]
+        f64 = rt.schedule(materialize_output=False)
+        rt.tilespgemm(it["pa"], it["pb"], f64)
+        rt.tilespgemm(it["pr"], it["pb2"], f64)
+        ref = [rt.last_c(it["pa"])["tile_csr_val"], rt.last_c(it["pr"])["tile_csr_val"]]
+
+        def absm(m):
+            return rt.CsrMatrix(m.m, m.n, m.rowptr, m.col, np.abs(m.val))
+
+        ap_csr = it["ap"]
+        scales = []
+        for left, right in ((bA, bP), (bR, ap_csr)):
+            pl = rt.prepare_identity_a(absm(left))
+            rt.tilespgemm(pl, rt.rebuild_b_general(pl, rt.upload_csr(absm(right))), f64)
+            scales.append(rt.last_c(pl)["tile_csr_val"])
+        res["split_vs_fp64_errors"] = {
+            "A*P": value_errors_vs(got[0], ref[0], scales[0]),
+            "R*(A*P)": value_errors_vs(got[1], ref[1], scales[1])}
     if sequential:
         items = prep(data["A"], data["P"], data["R"])
         res["ms_sequential"] = timed_steps(lambda: run(items), max(1, steps // 2), 1)
@@ -195,7 +218,8 @@ def batch_main(args):
     if spec["kind"] == "banded":
         r = batch_banded(rt, data, args.steps, args.warmup, args.precision)
     else:
-        r = batch_amg(rt, data, args.steps, args.warmup, args.precision)
+        r = batch_amg(rt, data, args.steps, args.warmup, args.precision,
+                      sequential=not args.no_sequential)
     flop = 2.0 * r["nnz_cub"]
     line = {"workload": f"{args.workload}: {data['desc']}", "precision": args.precision,
             "steps": args.steps, "warmup": args.warmup, "nnz_cub": r["nnz_cub"],
@@ -206,10 +230,87 @@ def batch_main(args):
                               "value": round(flop / (r["ms_sequential"] * 1e6), 3),
                               "unit": "GFLOP/s"}
         line["batched_over_sequential"] = round(r["ms_sequential"] / r["ms_batched"], 3)
-    for k in ("nnz_c", "nnz_c_matches_sequential", "tuned"):
+    for k in ("nnz_c", "nnz_c_matches_sequential", "tuned", "split_vs_fp64_errors"):
         if k in r:
             line[k] = r[k]
     print(json.dumps(line), flush=True)
+
+
+# ---- the split-precision form on the block-FEM matrix ----------------------
+
+def value_errors_vs(got, ref, scale):
+    """max / median of |got - ref| / |ref| (ref != 0) and of |got - ref| / scale."""
+    err = np.abs(got - ref)
+    nz = ref != 0
+    rel = err[nz] / np.abs(ref[nz])
+    fwd = err / np.where(scale > 0, scale, 1.0)
+    return {"rel_max": float(rel.max()) if rel.size else 0.0,
+            "rel_median": float(np.median(rel)) if rel.size else 0.0,
+            "fwd_max": float(fwd.max()) if fwd.size else 0.0,
+            "fwd_median": float(np.median(fwd)) if fwd.size else 0.0}
+
+
+def split_form(args):
+    """C = A*A on the full block-FEM matrix in plain FP64 (tile path) and in
+    the split BF16 form: step times, and the split form's error per C entry
+    against the FP64 result, scaled by |c| and by (|A||A|)_ij (computed by
+    the same FP64 kernels on |A|).  A row sample is also scored against the
+    double-double oracle (both forms)."""
+    import oracle
+    import rtt_amd as rt
+    spec = WORKLOADS[args.workload]
+    a, desc = make_matrix(rt, spec)
+    rt.init(0)
+    pa = rt.prepare_identity_a(a)
+    nnz_cub = rt.nnz_cub(pa, 0)
+    out = {"workload": f"{args.workload}: C = A*A, {desc}", "nnz_cub": nnz_cub}
+    vals = {}
+    for prec in ("fp64", "split-bf16"):
+        sched = rt.schedule(materialize_output=False, precision=prec)
+        state = {}
+
+        def step():
+            state["pb"] = rt.rebuild_b(pa, 0, state.get("pb"))
+            state["st"] = rt.tilespgemm(pa, state["pb"], sched)[1]
+
+        ms = timed_steps(step, args.steps, args.warmup)
+        st = state["st"]
+        vals[prec] = rt.last_c(pa)["tile_csr_val"]
+        out[prec] = {"ms_per_step": round(ms, 4),
+                     "value": round(2.0 * nnz_cub / (ms * 1e6), 3), "unit": "GFLOP/s",
+                     "numeric_kernel_ms": round(st["numeric_kernel_ms"], 4),
+                     "numeric_path": {v: k for k, v in rt.NUMERIC_PATHS.items()}[st["numeric_path"]],
+                     "mfma": {"instruction": "v_mfma_f64_4x4x4f64" if prec == "fp64"
+                              else "v_mfma_f32_16x16x32_bf16 (8 per pair)",
+                              "pairs": st["mfma_pairs"]}}
+    # (|A||A|)_ij with the FP64 kernels: the forward-error scale of every entry.
+    absa = rt.CsrMatrix(a.m, a.n, a.rowptr, a.col, np.abs(a.val))
+    pabs = rt.prepare_identity_a(absa)
+    rt.tilespgemm(pabs, rt.rebuild_b(pabs, 0), rt.schedule(materialize_output=False))
+    scale = rt.last_c(pabs)["tile_csr_val"]
+    out["split_vs_fp64_errors"] = value_errors_vs(vals["split-bf16"], vals["fp64"], scale)
+    out["split_vs_fp64_errors"]["entries"] = int(scale.size)
+    out["split_over_fp64_speed"] = round(out["fp64"]["ms_per_step"] /
+                                         out["split-bf16"]["ms_per_step"], 3)
+    # Row sample against the double-double oracle.
+    rows = 2048
+    r0 = (a.m // 2) // 8 * 8
+    sub = (rows, a.n, (a.rowptr[r0:r0 + rows + 1] - a.rowptr[r0]).astype(np.int32),
+           a.col[a.rowptr[r0]:a.rowptr[r0 + rows]], a.val[a.rowptr[r0]:a.rowptr[r0 + rows]])
+    ref = oracle.spgemm(sub, (a.m, a.n, a.rowptr, a.col, a.val))
+    ct = lambda v: oracle.c_tiles(ref["m"], ref["n"], ref["rowptr"], ref["col"], v)["tile_csr_val"]
+    hi, lo, ab = ct(ref["hi"]), ct(ref["lo"]), ct(ref["absprod"])
+    sample = {}
+    # The sample rows' values are one stretch of C's values: tile rows
+    # r0/8 .. (r0 + rows)/8 in tile order, as the oracle's tiles of the block.
+    c = rt.last_c(pa, with_values=False)
+    t0, t1 = int(c["tile_ptr"][r0 // 8]), int(c["tile_ptr"][(r0 + rows) // 8])
+    z0, z1 = int(c["tile_nnz"][t0]), int(c["tile_nnz"][t1])
+    for prec in ("fp64", "split-bf16"):
+        sample[prec] = oracle.value_errors(vals[prec][z0:z1], hi, lo, ab)
+    out["oracle_sample"] = {"rows": f"{r0}..{r0 + rows - 1}", "entries": int(z1 - z0),
+                            "errors": sample}
+    print(json.dumps(out), flush=True)
 
 
 def csr_bytes(m, nnz):
@@ -217,11 +318,13 @@ def csr_bytes(m, nnz):
     return 4 * (m + 1) + 12 * nnz
 
 
-def cpu_baseline(a, aat, nnz_cub, budget_s=12.0):
+def cpu_baseline(a, aat, nnz_cub, budget_s=2.5, repeats=5):
     """Times the oracle's row SPA (structure + values in reference order) on a
-    contiguous block of rows, sized to roughly budget_s seconds."""
+    contiguous block of rows sized to about budget_s seconds, `repeats`
+    times; reports the median.  Threads: OMP_NUM_THREADS when set (the host's
+    CPU share on a GPU box), else every core."""
     import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(threads)
     A = (a.m, a.n, a.rowptr, a.col, a.val)
     if aat:
@@ -244,12 +347,16 @@ def cpu_baseline(a, aat, nnz_cub, budget_s=12.0):
 
     dt, prod, rows = run(max(1024, min(a.m, int(2e6 / max(per_row, 1)))))
     rows = int(min(a.m, rows * max(1.0, budget_s / max(dt, 1e-3))))
-    dt, prod, rows = run(rows)
+    runs = [run(rows) for _ in range(repeats)]
+    dts = sorted(r[0] for r in runs)
+    dt = dts[len(dts) // 2]
+    prod, rows = runs[0][1], runs[0][2]
     return {"value": 2.0 * prod / dt / 1e9, "unit": "GFLOP/s", "cores": threads,
             "kind": "port",
             "sample": f"row SPA (oracle, structure + FP64 values) on {rows} contiguous rows "
                       f"({rows / a.m:.1%} of the rows, {prod:.3g} products) of the same "
-                      f"workload, {dt:.2f} s, OpenMP {threads} threads"}
+                      f"workload, median of {repeats} runs ({dt:.2f} s), OpenMP {threads} "
+                      f"threads"}
 
 
 # Kernel names in the rocprofv3 counter summaries, per breakdown key.
@@ -286,6 +393,60 @@ def profile_traffic(workload, kernel):
     return None, None
 
 
+# The precision and batching forms the metric compares with plain FP64, each
+# measured in a child process (this script with its own arguments) before the
+# main process touches the GPU; a failure there does not end the benchmark.
+FORMS = {
+    "split_block_fem": ["--workload", "fem-500k-aa", "--split-form", "--steps", "5",
+                        "--warmup", "2"],
+    "batched_banded": ["--workload", "banded-batch-512", "--steps", "4", "--warmup", "1"],
+    "batched_amg_fp64": ["--workload", "amg-batch-64", "--steps", "3", "--warmup", "1"],
+    "batched_amg_split": ["--workload", "amg-batch-64", "--steps", "3", "--warmup", "1",
+                          "--precision", "split-bf16", "--no-sequential"],
+}
+
+
+def run_forms():
+    out = {}
+    for name, extra in FORMS.items():
+        cmd = [sys.executable, os.path.abspath(__file__)] + extra
+        try:
+            p = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+        except subprocess.TimeoutExpired:
+            out[name] = {"error": "timed out after 900 s"}
+            continue
+        lines = [l for l in p.stdout.splitlines() if l.startswith("{")]
+        if p.returncode != 0 or not lines:
+            out[name] = {"error": f"exit {p.returncode}: {p.stderr[-300:]}"}
+            continue
+        out[name] = json.loads(lines[-1])
+        print(f"form {name} done", file=sys.stderr, flush=True)
+    # Headline ratios of the metric: each form's GFLOP/s over plain FP64 on
+    # the same workload.
+    ratios = {}
+    f = out.get("split_block_fem", {})
+    if "fp64" in f and "split-bf16" in f:
+        ratios["split_bf16_over_fp64_block_fem"] = {
+            "ratio": round(f["split-bf16"]["value"] / f["fp64"]["value"], 3),
+            "split_fwd_max_err": f["split_vs_fp64_errors"]["fwd_max"],
+            "split_rel_max_err": f["split_vs_fp64_errors"]["rel_max"],
+            "split_rel_median_err": f["split_vs_fp64_errors"]["rel_median"]}
+    b = out.get("batched_banded", {})
+    if "batched_over_sequential" in b:
+        ratios["batched_over_sequential_banded_512"] = b["batched_over_sequential"]
+    c = out.get("batched_amg_fp64", {})
+    if "batched_over_sequential" in c:
+        ratios["batched_over_sequential_amg_64"] = c["batched_over_sequential"]
+    d = out.get("batched_amg_split", {})
+    if "batched" in c and "batched" in d:
+        ratios["split_batched_over_fp64_sequential_amg_64"] = round(
+            d["batched"]["value"] / c["sequential"]["value"], 3) if "sequential" in c else None
+        ratios["split_batched_over_fp64_batched_amg_64"] = round(
+            d["batched"]["value"] / c["batched"]["value"], 3)
+    out["ratios_vs_plain_fp64"] = ratios
+    return out
+
+
 # The other product mode of the same benchmark configuration, reported
 # beside the main line: C = A*A on the R-MAT 2^20 matrix, in the reference's
 # default (BGRF-v1) layout.
@@ -299,7 +460,7 @@ def second_mode():
     m = SECOND_MODE
     cmd = [sys.executable, os.path.abspath(__file__), "--workload", m["workload"], "--layout",
            m["layout"], "--steps", str(m["steps"]), "--warmup", str(m["warmup"]),
-           "--no-cpu-baseline", "--no-control", "--no-second-mode"]
+           "--no-cpu-baseline", "--no-control", "--no-second-mode", "--no-forms"]
     try:
         p = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
     except subprocess.TimeoutExpired:
@@ -407,6 +568,12 @@ def main():
     ap.add_argument("--no-control", action="store_true",
                     help="skip the rocSPARSE SpGEMM control on the same workload")
     ap.add_argument("--control-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batched workloads: skip the one-call-per-problem comparison")
+    ap.add_argument("--split-form", action="store_true",
+                    help="plain FP64 vs split-BF16 on --workload (block FEM), with errors")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the split-precision and batched forms reported beside the line")
     ap.add_argument("--no-second-mode", action="store_true",
                     help="skip the R-MAT 2^20 C = A*A measurement reported beside the main line")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "split-bf16"],
@@ -430,6 +597,9 @@ def main():
     if args.workload in BATCHES:
         batch_main(args)
         return
+    if args.split_form:
+        split_form(args)
+        return
 
     import rtt_amd as rt
     spec = WORKLOADS[args.workload]
@@ -438,6 +608,7 @@ def main():
     # touches the GPU.
     control = None if args.no_control else rocsparse_control(args.workload)
     other = None if args.no_second_mode else second_mode()
+    forms = None if args.no_forms else run_forms()
     a, desc = make_matrix(rt, spec)
     rt.init(0)
     pa = rt.prepare_bgrf_a(a) if args.layout == "bgrf" else rt.prepare_identity_a(a)
@@ -543,6 +714,7 @@ def main():
         "cpu_baseline": cpu,
         "control": control,
         "second_mode": other,
+        "forms": forms,
     }
     print(json.dumps(line), flush=True)
 

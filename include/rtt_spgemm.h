@@ -377,7 +377,8 @@ typedef struct rtt_schedule {    /* DmmaNumericScheduleConfig subset */
     int split_max_chunks;        /* 8; valid 2..32 */
     double max_heavy_fraction;   /* 0.01; valid 0..1 */
     int tail_record_capacity;    /* 1 << 20 */
-    int force_tail_split;        /* admit every multi-step task (tests) */
+    int force_tail_split;        /* split the admitted tiles whatever their share
+                                    (skips the heavy-fraction gate) */
     int numeric_path;            /* enum rtt_numeric_path; default AUTO */
     double tile_fill_threshold;  /* AUTO: stored values per A tile for TILE (12) */
     int precision;               /* enum rtt_precision; default FP64 */
@@ -401,7 +402,8 @@ typedef struct rtt_stats {       /* DmmaSpGemmStats subset */
     unsigned long long output_nnz;     /* nnz(C) */
     unsigned long long heavy_rows;     /* VECTOR: tile rows beyond the small LDS
                                           budget (larger LDS kernel or HBM slab) */
-    unsigned long long heavy_chunks;   /* chunks of split heavy C tiles */
+    unsigned long long heavy_chunks;   /* TILE, adaptive-tail: chunks the split heavy
+                                          C tiles were cut into (0 when not split) */
     int output_materialized;
     int numeric_path;                  /* VECTOR, TILE, or AUTO when rows took both */
     double b_columns_ms;               /* TILE: B's tile columns built for the call */
@@ -409,6 +411,11 @@ typedef struct rtt_stats {       /* DmmaSpGemmStats subset */
                                           (4 blocks x 4x4x4, 512 flops) each, or for
                                           SPLIT_BF16 eight BF16 16x16x32 MFMAs */
     int precision;                     /* enum rtt_precision that ran */
+    int tail_reason;                   /* TILE, adaptive-tail: tail gate decision --
+                                          1 split, 4 no heavy tile, 5 record capacity
+                                          exceeded, 6 heavy fraction exceeded, 7
+                                          workspace limit exceeded; 0 no admission */
+    unsigned long long tail_records;   /* C tiles admitted as heavy */
 } rtt_stats;
 
 /* dmma_tilespgemm (src/dmma_spgemm.h:6490): C = A * B with A 8x4 and B 4x8
@@ -421,6 +428,19 @@ typedef struct rtt_stats {       /* DmmaSpGemmStats subset */
 int rtt_tilespgemm(const rtt_prepared_a *a, const rtt_dynamic_b *b,
                    rtt_smatrix *output, rtt_stats *stats,
                    const rtt_schedule *sched);
+
+/* Tail queue of the TILE path under adaptive-tail (the reference's
+ * admission, gate, chunk descriptors and chunk workspace reduction,
+ * src/dmma_spgemm.h:2628-2776, 4902-5254, 6003-6049, 6981-7209): a C tile
+ * whose K-merge costs scans + 8 matches > split_threshold (FP32) is admitted
+ * with chunks = ceil(cost / split_chunk_target) clamped to
+ * [2, split_max_chunks] and to its scans; when the gate passes, each chunk
+ * (a contiguous range of the tile's B column) is multiplied by its own wave
+ * into a 64-value workspace slot and the chunks are summed in order.  This
+ * copies the admitted records of the last call on `a` (slot order): tile
+ * index, merge scans, matches, chunks; *count receives how many. */
+int rtt_last_tail_records(const rtt_prepared_a *a, int max_records, long long *task,
+                          int *scans, int *matches, int *chunks, int *count);
 
 /* Device view of the last C produced on handle `a` (valid until the next
  * rtt_tilespgemm on `a`): 64-bit offsets, 64-bit tile masks. */

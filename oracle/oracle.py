@@ -167,6 +167,18 @@ def transpose(m, n, rp, ci, val):
     return trp, tci[:nnz], tv[:nnz]
 
 
+def spgemm_row_counts(a, b) -> np.ndarray:
+    """Row pointer (int64, m + 1) of C = A*B's structure: the structure pass
+    of the row SPA alone (orc_spgemm_count), for products too large to fill."""
+    ma, na, rpa, cia = a[:4]
+    mb, nb, rpb, cib = b[:4]
+    rpa, cia, rpb, cib = _i32(rpa), _i32(cia), _i32(rpb), _i32(cib)
+    rpc = np.zeros(ma + 1, np.int64)
+    if lib().orc_spgemm_count(ma, na, _p(rpa), _p(cia), mb, nb, _p(rpb), _p(cib), _p(rpc)) < 0:
+        raise ValueError("shape mismatch")
+    return rpc
+
+
 def spgemm(a, b, with_dd: bool = True):
     """C = A*B.  a, b = (m, n, rowptr, col, val).  Returns a dict with the
     reference-order values and, optionally, the double-double truth and the

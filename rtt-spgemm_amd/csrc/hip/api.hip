@@ -727,6 +727,15 @@ int rtt_tilespgemm(const rtt_prepared_a *a_const, const rtt_dynamic_b *b, rtt_sm
     rtt::TileOpts topt;
     topt.fill = path == RTT_NUMERIC_TILE ? 0.f : fill;
     topt.precision = sc.precision;
+    // The adaptive-tail schedule splits heavy C tiles of the tile-pair path
+    // (src/main.cu:132-166: the tail queue belongs to --lb-mode adaptive-tail).
+    topt.tail = sc.lb_mode == RTT_LB_ADAPTIVE_TAIL;
+    topt.threshold = static_cast<float>(sc.split_threshold);
+    topt.target = static_cast<float>(sc.split_chunk_target);
+    topt.max_chunks = sc.split_max_chunks;
+    topt.max_heavy_fraction = sc.max_heavy_fraction;
+    topt.force = sc.force_tail_split;
+    topt.record_capacity = sc.tail_record_capacity;
     if (path != RTT_NUMERIC_TILE)
         RTT_TRY(rtt::ensure_record_values(&const_cast<rtt_dynamic_b *>(b)->tiles));
     if (path != RTT_NUMERIC_VECTOR) {
@@ -742,6 +751,9 @@ int rtt_tilespgemm(const rtt_prepared_a *a_const, const rtt_dynamic_b *b, rtt_sm
     stats->numeric_path = path;
     stats->precision = path == RTT_NUMERIC_VECTOR ? RTT_PRECISION_FP64 : sc.precision;
     stats->mfma_pairs = times.mfma_pairs;
+    stats->tail_reason = times.tail_reason;
+    stats->tail_records = times.tail_records;
+    stats->heavy_chunks = times.tail_chunks;
     a->has_c = true;
     stats->symbolic_ms = times.symbolic_ms;
     stats->allocation_ms = times.allocation_ms;
@@ -932,6 +944,21 @@ int rtt_autotune(const rtt_prepared_a *a, const rtt_dynamic_b *b, int repeats, r
     if (report)
         *report = rep;
     return RTT_OK;
+}
+
+int rtt_last_tail_records(const rtt_prepared_a *a, int max_records, long long *task, int *scans,
+                          int *matches, int *chunks, int *count)
+{
+    rtt::clear_error();
+    RTT_TRY(rtt::require_ctx());
+    if (!a || !count || max_records < 0 || (max_records > 0 && (!task || !scans || !matches ||
+                                                                 !chunks)))
+        return fail(RTT_ERR_INVALID, "rtt_last_tail_records: bad argument");
+    *count = 0;
+    rtt_prepared_a *am = const_cast<rtt_prepared_a *>(a);
+    if (!am->core.tail_state.ptr)
+        return RTT_OK;
+    return rtt::tail_records(&am->core, max_records, task, scans, matches, chunks, count);
 }
 
 int rtt_last_c_view(const rtt_prepared_a *a, rtt_device_c *v)

@@ -172,6 +172,9 @@ struct SpgemmWork {
     DevBuf tile_queue;                      // row-queue heads of the tile-pair kernels
     DevBuf tile_pairs;                      // u64: pairs the tile-pair kernel multiplied
     DevBuf tile_rows;                       // u64 x 2: rows counted for routing
+    // Tail queue of the tile-pair path: records, state, chunk prefix, heavy
+    // tile bitset and the chunks' partial tiles.
+    DevBuf tail_records, tail_state, tail_chunk_base, tail_heavy, tail_workspace;
 };
 
 // Scratch of the tile builders, reused across calls.
@@ -264,18 +267,33 @@ TileColumns b_columns(const DevTiles &b);
 // precision 0 on FP64 matrix cores, 1 by BF16 slices (Ozaki scheme) with
 // FP64 recombination.  Deterministic: each tile sums its pairs in
 // increasing K.
-int tile_numeric(const DevTiles &a, const TileColumns &bc, const DevC &c, int precision,
-                 float fill, SpgemmWork *w, hipStream_t s);
+struct TileOpts;
+int tile_numeric(const DevTiles &a, const TileColumns &bc, const DevC &c, const TileOpts &opt,
+                 SpgemmWork *w, hipStream_t s);
+// Gate decision, admitted records and chunks of the last tail admission.
+int tail_stats(SpgemmWork *w, int *reason, unsigned long long *records,
+               unsigned long long *chunks);
+// The admitted records (slot order): tile, merge scans and matches, chunks.
+int tail_records(SpgemmWork *w, int max_records, long long *task, int *scans, int *matches,
+                 int *chunks, int *count);
 // Tile rows of A the tile-pair kernel would take at `fill`, and the tile
 // rows holding any tile (one readback).
 int count_tile_rows(const DevTiles &a, float fill, DevBuf *scratch, int64_t *tile_rows,
                     int64_t *nonempty_rows);
 // Per-row numeric routing of one call: rows whose A tiles hold at least
 // `fill` stored values on average take the tile-pair kernel in `precision`.
+// With `tail`, C tiles whose K-merge costs more than `threshold` (FP32
+// scans + 8 matches) are split into chunks (the adaptive-tail queue).
 struct TileOpts {
     float fill = 0.f;
     int precision = 0;
     TileColumns bc;
+    bool tail = false;
+    float threshold = 16384.f, target = 8192.f;
+    int max_chunks = 8;
+    double max_heavy_fraction = 0.01;
+    int force = 0;
+    int record_capacity = 1 << 20;
 };
 
 // ---- SpGEMM core (spgemm.hip) ---------------------------------------------
@@ -284,6 +302,8 @@ struct CoreTimes {
     double sym_count_kernel_ms = 0, sym_fill_kernel_ms = 0, numeric_kernel_ms = 0;
     unsigned long long pairs = 0, heavy_rows = 0;
     unsigned long long mfma_pairs = 0;   // tile path: (A tile, B tile) products on MFMA
+    int tail_reason = 0;                 // tile path: tail gate (0: no admission ran)
+    unsigned long long tail_records = 0, tail_chunks = 0;
 };
 // lb_mode (enum rtt_lb_mode) assigns the rows of the slab kernels, which
 // hold one workgroup per row for many rows: static hands row i to

@@ -2188,7 +2188,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     }
     RTT_HIP(hipEventRecord(ev->ev[4], s));
     if (any_tile) {
-        RTT_TRY(tile_numeric(a, tile->bc, *c, tile->precision, tile->fill, w, s));
+        RTT_TRY(tile_numeric(a, tile->bc, *c, *tile, w, s));
         RTT_HIP(hipEventRecord(ev->ev[5], s));
     }
     if (any_vector && R > 0 && c->nnz > 0) {
@@ -2226,9 +2226,13 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     }
     RTT_HIP(hipEventRecord(ev->ev[7], s));
     RTT_HIP(hipStreamSynchronize(s));
-    if (any_tile && c->num_tiles > 0)
+    if (any_tile && c->num_tiles > 0) {
         RTT_HIP(hipMemcpy(&times->mfma_pairs, w->tile_pairs.ptr, sizeof(unsigned long long),
                           hipMemcpyDeviceToHost));
+        if (tile->tail)
+            RTT_TRY(tail_stats(w, &times->tail_reason, &times->tail_records,
+                               &times->tail_chunks));
+    }
     if (trace_slots)
         RTT_TRY(write_timeline(trace_path, tl, trace_slots, static_cast<uint64_t>(R),
                                static_cast<uint64_t>(c->num_tiles), num_cus));

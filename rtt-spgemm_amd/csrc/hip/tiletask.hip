@@ -25,6 +25,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <vector>
 
 #include "device.hpp"
 #include "internal.hpp"
@@ -57,6 +59,32 @@ struct TileTask {
     float fill;                          // rows taken: A tiles' mean stored values >= fill
     int *__restrict__ queue;             // kXcds row-queue heads, zero at launch
     unsigned long long *__restrict__ mfma_pairs;   // pairs multiplied (one MFMA each)
+    // Heavy C tiles split into chunks (bit t of heavy: C tile t), used when
+    // tail->enabled; null when no admission ran.
+    const uint32_t *__restrict__ heavy;
+    const struct TailState *__restrict__ tail;
+};
+
+// Tail queue (the reference's heavy-task split, src/dmma_spgemm.h:2534-2776,
+// 4902-5254, 6003-6049): C tiles whose K-merge costs more than the admission
+// threshold are split into chunks of their B-column range; each chunk's
+// partial tile goes to a workspace slot, and a reduction sums the chunks of a
+// tile in chunk order, so the split tiles are as reproducible as the rest.
+struct TailRecord {          // DmmaTailRecord {id, scans, matches, chunks} + the tile row
+    long long task;
+    int row;
+    int scans;
+    int matches;
+    int chunks;
+};
+struct TailState {
+    unsigned int count;              // records admitted (may exceed capacity)
+    int overflow;                    // an admission found the record list full
+    unsigned long long chunks;       // chunks of the admitted records
+    int reason;                      // gate decision (rtt_stats.tail_reason)
+    int enabled;                     // 1: heavy tiles take the chunk kernels
+    int chunk_head;                  // work queue of the chunk kernel
+    int reduce_head;                 // work queue of the reduction
 };
 
 constexpr uint32_t kDense = 0x80000000u;
@@ -125,6 +153,12 @@ struct Fp64Product {
         const int bit = ln.roles.c_bit;
         if ((m >> bit) & 1ull)
             g.c_val[off + __popcll(m & ((1ull << bit) - 1ull))] = acc.v;
+    }
+
+    // Lane's entry of a finished tile into a 64-slot partial (bit order).
+    __device__ static void dump(const Lane &ln, const Acc &acc, double *slot64)
+    {
+        slot64[ln.roles.c_bit] = acc.v;
     }
 
     // Runs buffered pairs [0, n) in order; a change of C tile stores the
@@ -264,6 +298,15 @@ struct SplitProduct {
             if ((m >> bit) & 1ull)
                 g.c_val[off + __popcll(m & ((1ull << bit) - 1ull))] = acc.v[v];
         }
+    }
+
+    __device__ static void dump(const Lane &ln, const Acc &acc, double *slot64)
+    {
+        if (!ln.out_live)
+            return;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            slot64[(ln.out_row0 + v) * 8 + ln.out_col] = acc.v[v];
     }
 
     // Pairs [p0, p0 + n) of one C tile, n <= kSplitBatch, added into acc.
@@ -407,6 +450,7 @@ __global__ __launch_bounds__(kTThreads) void tile_numeric_kernel(TileTask g)
     int *cbeg = s_cb[w];
     const typename P::Lane ln(lane);
     unsigned long long done = 0;   // pairs this wave multiplied
+    const bool heavy_on = g.heavy && g.tail->enabled;
     const int home = blockIdx.x % kXcds;
     for (int probe = 0; probe < kXcds; ++probe) {
         const int x = (home + probe) % kXcds;
@@ -446,6 +490,10 @@ __global__ __launch_bounds__(kTThreads) void tile_numeric_kernel(TileTask g)
                     co = g.c_off[cc + lane];
                     cb = g.b_cptr[j];
                     nb = g.b_cptr[j + 1] - cb;
+                    // Heavy tiles are computed by the chunk kernels instead.
+                    const int64_t t = cc + lane;
+                    if (heavy_on && ((g.heavy[t >> 5] >> (t & 31)) & 1u))
+                        nb = 0;
                 }
                 const int incl = wave_inclusive_scan(nb);
                 const int total = __shfl(incl, kWave - 1, kWave);
@@ -505,6 +553,314 @@ __global__ __launch_bounds__(kTThreads) void tile_numeric_kernel(TileTask g)
     }
     if (lane == 0 && done)
         atomicAdd(g.mfma_pairs, done);
+}
+
+// Chunks of a task whose merge takes `scans` steps with `matches` matched
+// pairs: the reference's FP32 admission (cost = scans + 8 matches, strictly
+// above the threshold) and chunk count (ceil(cost / target), clamped to
+// [2, max_chunks] and to at most one chunk per scan step); 0 = not admitted
+// (dmma_tail_chunk_count_from_counts, src/dmma_spgemm.h:2628-2647).
+__device__ __forceinline__ int tail_chunks(int scans, int matches, float threshold, float target,
+                                           int max_chunks)
+{
+    const float work = 1.0f * static_cast<float>(scans) + 8.0f * static_cast<float>(matches);
+    if (!(work > threshold) || !(target > 0.0f) || scans <= 1 || max_chunks <= 1)
+        return 0;
+    const float want = ceilf(work / target);
+    int c = want >= static_cast<float>(max_chunks) ? max_chunks : static_cast<int>(want);
+    c = max(c, 2);
+    c = min(c, scans);
+    return c >= 2 ? c : 0;
+}
+
+struct TailArgs {
+    float threshold, target;
+    int max_chunks;
+    double max_heavy_fraction;
+    int force;
+    unsigned int capacity;
+    unsigned long long workspace_limit;   // bytes
+    long long tiles;                      // C tiles of the call
+    TailRecord *__restrict__ records;
+    TailState *__restrict__ state;
+    uint32_t *__restrict__ heavy;
+    int *__restrict__ chunk_base;         // capacity + 1
+    double *__restrict__ workspace;       // 64 doubles per chunk
+};
+
+__global__ void tail_reset_kernel(TailState *st)
+{
+    if (threadIdx.x == 0) {
+        st->count = 0u;
+        st->overflow = 0;
+        st->chunks = 0ull;
+        st->reason = 0;
+        st->enabled = 0;
+        st->chunk_head = 0;
+        st->reduce_head = 0;
+    }
+}
+
+// One wave per tile row of the tile-pair kernel: the exact merge counts of
+// every C tile whose upper bound (all of both lists scanned, every pair of
+// the shorter list matched) clears the threshold, and a record for each
+// admitted tile.
+__global__ __launch_bounds__(kTThreads) void tail_admit_kernel(TileTask g, TailArgs t)
+{
+    const int row = blockIdx.x * kTWaves + static_cast<int>(threadIdx.x) / kWave;
+    const int lane = lane_id();
+    if (row >= g.rows)
+        return;
+    const int64_t c0 = g.c_ptr[row], c1 = g.c_ptr[row + 1];
+    const int a0 = g.a_trp[row];
+    const int na = g.a_trp[row + 1] - a0;
+    if (c0 == c1 || static_cast<float>(g.a_row_off[row + 1] - g.a_row_off[row]) <
+                        g.fill * static_cast<float>(na))
+        return;
+    for (int64_t cc = c0; cc < c1; cc += kWave) {
+        const int64_t tile = cc + lane;
+        int cb = 0, nb = 0;
+        bool cand = false;
+        if (tile < c1) {
+            const int j = g.c_col[tile];
+            cb = g.b_cptr[j];
+            nb = g.b_cptr[j + 1] - cb;
+            const float ub = 1.0f * static_cast<float>(na + nb) +
+                             8.0f * static_cast<float>(min(na, nb));
+            cand = ub > t.threshold;
+        }
+        unsigned long long todo = __ballot(cand);
+        while (todo) {
+            const int l = __ffsll(static_cast<long long>(todo)) - 1;
+            todo &= todo - 1;
+            const int tcb = read_lane(cb, l), tnb = read_lane(nb, l);
+            // Intersection of A row's K list with B column's K list: lanes
+            // take the shorter list, binary search in the longer.
+            const bool a_short = na <= tnb;
+            const int ns = a_short ? na : tnb, nl = a_short ? tnb : na;
+            int matches = 0;
+            for (int x0 = 0; x0 < ns; x0 += kWave) {
+                const int x = x0 + lane;
+                bool hit = false;
+                if (x < ns) {
+                    const int key = a_short ? g.a_meta[a0 + x].x : g.b_cent[tcb + x].x;
+                    int lo = 0, hi = nl;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        const int k = a_short ? g.b_cent[tcb + mid].x : g.a_meta[a0 + mid].x;
+                        if (k < key)
+                            lo = mid + 1;
+                        else
+                            hi = mid;
+                    }
+                    hit = lo < nl &&
+                          (a_short ? g.b_cent[tcb + lo].x : g.a_meta[a0 + lo].x) == key;
+                }
+                matches += __popcll(__ballot(hit));
+            }
+            const int scans = na + tnb - matches;
+            const int chunks = tail_chunks(scans, matches, t.threshold, t.target, t.max_chunks);
+            if (lane == 0 && chunks > 0) {
+                const unsigned int slot = atomicAdd(&t.state->count, 1u);
+                if (slot < t.capacity) {
+                    TailRecord r;
+                    r.task = cc + l;
+                    r.row = row;
+                    r.scans = scans;
+                    r.matches = matches;
+                    r.chunks = chunks;
+                    t.records[slot] = r;
+                    atomicAdd(&t.state->chunks, static_cast<unsigned long long>(chunks));
+                } else {
+                    atomicExch(&t.state->overflow, 1);
+                }
+            }
+        }
+    }
+}
+
+// The gate (one thread): split only when some tile was admitted, the
+// records fit, the heavy fraction is at most max_heavy_fraction (unless
+// forced) and the chunk workspace fits its limit.  Reason codes: 1 split,
+// 4 no heavy tile, 5 record capacity exceeded, 6 heavy fraction exceeded,
+// 7 workspace limit exceeded.
+__global__ void tail_gate_kernel(TailArgs t)
+{
+    if (threadIdx.x != 0)
+        return;
+    TailState *st = t.state;
+    int reason;
+    if (st->overflow || st->count > t.capacity)
+        reason = 5;
+    else if (st->count == 0u)
+        reason = 4;
+    else if (!t.force && static_cast<double>(st->count) / static_cast<double>(t.tiles) >
+                             t.max_heavy_fraction)
+        reason = 6;
+    else if (st->chunks * 64ull * sizeof(double) > t.workspace_limit)
+        reason = 7;
+    else
+        reason = 1;
+    st->reason = reason;
+    st->enabled = reason == 1 ? 1 : 0;
+}
+
+__global__ void tail_mark_kernel(TailArgs t)
+{
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const TailState *st = t.state;
+    const unsigned int n = min(st->count, t.capacity);
+    if (i == 0)
+        t.chunk_base[0] = 0;
+    if (i >= n)
+        return;
+    const TailRecord r = t.records[i];
+    if (st->enabled)
+        atomicOr(&t.heavy[r.task >> 5], 1u << (r.task & 31));
+}
+
+// Exclusive prefix of the records' chunk counts (records in slot order; a
+// tile's chunks are contiguous).  Single workgroup, sequential over the
+// workgroup's partial sums: the record list is short.
+__global__ __launch_bounds__(kTThreads) void tail_chunk_base_kernel(TailArgs t)
+{
+    __shared__ int s_sum[kTThreads];
+    const TailState *st = t.state;
+    const int n = static_cast<int>(min(st->count, t.capacity));
+    const int per = (n + kTThreads - 1) / kTThreads;
+    const int b = threadIdx.x * per, e = min(n, b + per);
+    int sum = 0;
+    for (int i = b; i < e; ++i)
+        sum += t.records[i].chunks;
+    s_sum[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int i = 0; i < kTThreads; ++i) {
+            const int v = s_sum[i];
+            s_sum[i] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    int run = s_sum[threadIdx.x];
+    for (int i = b; i < e; ++i) {
+        t.chunk_base[i] = run;
+        run += t.records[i].chunks;
+    }
+    if (e == n && b < e)
+        t.chunk_base[n] = run;
+    if (n == 0 && threadIdx.x == 0)
+        t.chunk_base[0] = 0;
+}
+
+// Persistent waves over the chunks: chunk j of record r covers B column
+// entries [cb + nb j / chunks, cb + nb (j+1) / chunks) of its tile; the
+// pairs found there are multiplied like the tile kernel's and the lane's
+// partial entries go to the chunk's 64-slot workspace.
+template <class P>
+__global__ __launch_bounds__(kTThreads) void tail_chunk_kernel(TileTask g, TailArgs t)
+{
+    __shared__ int4 s_pm[kTWaves][kPairCap];
+    __shared__ unsigned char s_pc[kTWaves][kPairCap];
+    const int w = threadIdx.x / kWave;
+    const int lane = lane_id();
+    int4 *pm = s_pm[w];
+    unsigned char *pc = s_pc[w];
+    const typename P::Lane ln(lane);
+    const TailState *st = t.state;
+    if (!st->enabled)
+        return;
+    const int n = static_cast<int>(min(st->count, t.capacity));
+    const int total = t.chunk_base[n];
+    for (;;) {
+        int q = 0;
+        if (lane == 0)
+            q = atomicAdd(&t.state->chunk_head, 1);
+        q = read_lane(q, 0);
+        if (q >= total)
+            break;
+        const int r = upper_index(t.chunk_base, n, q);
+        const TailRecord rec = t.records[r];
+        const int j = q - t.chunk_base[r];
+        const int row = rec.row;
+        const int a0 = g.a_trp[row];
+        const int na = g.a_trp[row + 1] - a0;
+        const int jj = g.c_col[rec.task];
+        const int cb = g.b_cptr[jj];
+        const int nb = g.b_cptr[jj + 1] - cb;
+        const int e0 = cb + static_cast<int>(static_cast<long long>(nb) * j / rec.chunks);
+        const int e1 = cb + static_cast<int>(static_cast<long long>(nb) * (j + 1) / rec.chunks);
+        int npairs = 0, cur = -1;
+        typename P::Acc acc;
+        for (int p0 = e0; p0 < e1; p0 += kWave) {
+            const int p = p0 + lane;
+            int ai = -1;
+            int4 be = make_int4(0, 0, 0, 0);
+            if (p < e1) {
+                be = g.b_cent[p];
+                int lo = 0, hi = na;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (g.a_meta[a0 + mid].x < be.x)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                if (lo < na && g.a_meta[a0 + lo].x == be.x)
+                    ai = a0 + lo;
+            }
+            const unsigned long long match = __ballot(ai >= 0);
+            if (ai >= 0) {
+                const int4 am = g.a_meta[ai];
+                const int slot = npairs + __popcll(match & ((1ull << lane) - 1ull));
+                pm[slot] = make_int4(am.y, am.w, be.y, be.w);
+                pc[slot] = 0;
+            }
+            npairs += __popcll(match);
+            if (npairs + kWave > kPairCap || p0 + kWave >= e1) {
+                wave_sync();
+                P::run(g, ln, pm, pc, npairs, cur, acc, 0ull, 0ll);
+                npairs = 0;
+                wave_sync();
+            }
+        }
+        P::dump(ln, acc, t.workspace + static_cast<size_t>(q) * 64);
+    }
+}
+
+// One wave per admitted tile: its chunks' partials summed in chunk order,
+// stored at the tile's entries.
+__global__ __launch_bounds__(kTThreads) void tail_reduce_kernel(TileTask g, TailArgs t)
+{
+    const TailState *st = t.state;
+    if (!st->enabled)
+        return;
+    const int n = static_cast<int>(min(st->count, t.capacity));
+    const int lane = lane_id();
+    for (;;) {
+        int r = 0;
+        if (lane == 0)
+            r = atomicAdd(&t.state->reduce_head, 1);
+        r = read_lane(r, 0);
+        if (r >= n)
+            break;
+        const TailRecord rec = t.records[r];
+        const double *ws = t.workspace + static_cast<size_t>(t.chunk_base[r]) * 64;
+        double sum = 0.0;
+        for (int j = 0; j < rec.chunks; ++j)
+            sum += ws[static_cast<size_t>(j) * 64 + lane];
+        const unsigned long long m = g.c_mask[rec.task];
+        if ((m >> lane) & 1ull)
+            g.c_val[g.c_off[rec.task] + __popcll(m & ((1ull << lane) - 1ull))] = sum;
+    }
+}
+
+__global__ void zero_u32_kernel(uint32_t *p, int64_t n)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = 0u;
 }
 
 __global__ void zero_queue_kernel(int *q, int n, unsigned long long *pairs)
@@ -574,11 +930,12 @@ int count_tile_rows(const DevTiles &a, float fill, DevBuf *scratch, int64_t *til
     return RTT_OK;
 }
 
-int tile_numeric(const DevTiles &a, const TileColumns &bc, const DevC &c, int precision,
-                 float fill, SpgemmWork *w, hipStream_t s)
+int tile_numeric(const DevTiles &a, const TileColumns &bc, const DevC &c, const TileOpts &opt,
+                 SpgemmWork *w, hipStream_t s)
 {
     if (c.tilem == 0 || c.num_tiles == 0)
         return RTT_OK;
+    const int precision = opt.precision;
     if (precision != 0 && precision != 1)
         return fail(RTT_ERR_INVALID, "unknown tile numeric precision %d", precision);
     RTT_TRY(w->tile_queue.ensure(kXcds * sizeof(int)));
@@ -597,12 +954,58 @@ int tile_numeric(const DevTiles &a, const TileColumns &bc, const DevC &c, int pr
     g.c_off = c.tile_nnz.as<int64_t>();
     g.c_val = c.tile_csr_val.as<double>();
     g.a_row_off = a.row_payload_off.as<int>();
-    g.fill = fill;
+    g.fill = opt.fill;
     g.queue = w->tile_queue.as<int>();
     g.mfma_pairs = w->tile_pairs.as<unsigned long long>();
+    g.heavy = nullptr;
+    g.tail = nullptr;
     hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(kXcds), 0, s, g.queue, kXcds,
                        g.mfma_pairs);
     const int cus = ctx().num_cus > 0 ? ctx().num_cus : 256;
+    TailArgs t{};
+    if (opt.tail) {
+        // Record capacity as the reference sizes it: one past the admissible
+        // heavy count (floor(tiles * max_fraction)), or every tile when the
+        // split is forced, and never more than tail_record_capacity
+        // (src/dmma_spgemm.h:7000-7016).
+        const int64_t T = c.num_tiles;
+        const int64_t admissible = static_cast<int64_t>(
+            std::floor(static_cast<double>(T) * opt.max_heavy_fraction));
+        const int64_t cap = std::max<int64_t>(
+            1, std::min<int64_t>(opt.force ? T : admissible + 1, opt.record_capacity));
+        const unsigned long long limit = 1ull << 30;   // src/main.cu:89
+        const unsigned long long ws_bytes = std::min<unsigned long long>(
+            limit, static_cast<unsigned long long>(cap) * opt.max_chunks * 64 * sizeof(double));
+        RTT_TRY(w->tail_records.ensure(static_cast<size_t>(cap) * sizeof(TailRecord)));
+        RTT_TRY(w->tail_state.ensure(sizeof(TailState)));
+        RTT_TRY(w->tail_chunk_base.ensure((static_cast<size_t>(cap) + 1) * sizeof(int)));
+        RTT_TRY(w->tail_heavy.ensure(static_cast<size_t>((T + 31) / 32) * sizeof(uint32_t)));
+        RTT_TRY(w->tail_workspace.ensure(ws_bytes));
+        t.threshold = opt.threshold;
+        t.target = opt.target;
+        t.max_chunks = opt.max_chunks;
+        t.max_heavy_fraction = opt.max_heavy_fraction;
+        t.force = opt.force;
+        t.capacity = static_cast<unsigned int>(cap);
+        t.workspace_limit = ws_bytes;
+        t.tiles = T;
+        t.records = w->tail_records.as<TailRecord>();
+        t.state = w->tail_state.as<TailState>();
+        t.heavy = w->tail_heavy.as<uint32_t>();
+        t.chunk_base = w->tail_chunk_base.as<int>();
+        t.workspace = w->tail_workspace.as<double>();
+        hipLaunchKernelGGL(tail_reset_kernel, dim3(1), dim3(1), 0, s, t.state);
+        hipLaunchKernelGGL(zero_u32_kernel, dim3(blocks_for((T + 31) / 32, 256)), dim3(256), 0, s,
+                           t.heavy, (T + 31) / 32);
+        hipLaunchKernelGGL(tail_admit_kernel, dim3(blocks_for(g.rows, kTWaves)), dim3(kTThreads),
+                           0, s, g, t);
+        hipLaunchKernelGGL(tail_gate_kernel, dim3(1), dim3(1), 0, s, t);
+        hipLaunchKernelGGL(tail_mark_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, s, t);
+        hipLaunchKernelGGL(tail_chunk_base_kernel, dim3(1), dim3(kTThreads), 0, s, t);
+        RTT_HIP(hipGetLastError());
+        g.heavy = t.heavy;
+        g.tail = t.state;
+    }
     // Persistent: enough workgroups to fill every CU several times over.
     const int blocks = static_cast<int>(std::min<int64_t>(
         static_cast<int64_t>(cus) * 8, (static_cast<int64_t>(g.rows) + kTWaves - 1) / kTWaves));
@@ -613,6 +1016,49 @@ int tile_numeric(const DevTiles &a, const TileColumns &bc, const DevC &c, int pr
         hipLaunchKernelGGL(tile_numeric_kernel<SplitProduct>, dim3(std::max(blocks, 1)),
                            dim3(kTThreads), 0, s, g);
     RTT_HIP(hipGetLastError());
+    if (opt.tail) {
+        if (precision == 0)
+            hipLaunchKernelGGL(tail_chunk_kernel<Fp64Product>, dim3(cus * 4), dim3(kTThreads), 0,
+                               s, g, t);
+        else
+            hipLaunchKernelGGL(tail_chunk_kernel<SplitProduct>, dim3(cus * 4), dim3(kTThreads), 0,
+                               s, g, t);
+        hipLaunchKernelGGL(tail_reduce_kernel, dim3(cus * 2), dim3(kTThreads), 0, s, g, t);
+        RTT_HIP(hipGetLastError());
+    }
+    return RTT_OK;
+}
+
+int tail_stats(SpgemmWork *w, int *reason, unsigned long long *records,
+               unsigned long long *chunks)
+{
+    TailState st{};
+    RTT_HIP(hipMemcpy(&st, w->tail_state.ptr, sizeof(st), hipMemcpyDeviceToHost));
+    *reason = st.reason;
+    *records = st.count;
+    *chunks = st.enabled ? st.chunks : 0ull;
+    return RTT_OK;
+}
+
+int tail_records(SpgemmWork *w, int max_records, long long *task, int *scans, int *matches,
+                 int *chunks, int *count)
+{
+    TailState st{};
+    RTT_HIP(hipMemcpy(&st, w->tail_state.ptr, sizeof(st), hipMemcpyDeviceToHost));
+    const int n = static_cast<int>(std::min<unsigned long long>(
+        std::min<unsigned long long>(st.count, w->tail_records.bytes / sizeof(TailRecord)),
+        static_cast<unsigned long long>(max_records)));
+    std::vector<TailRecord> recs(static_cast<size_t>(n));
+    if (n > 0)
+        RTT_HIP(hipMemcpy(recs.data(), w->tail_records.ptr, n * sizeof(TailRecord),
+                          hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+        task[i] = recs[i].task;
+        scans[i] = recs[i].scans;
+        matches[i] = recs[i].matches;
+        chunks[i] = recs[i].chunks;
+    }
+    *count = n;
     return RTT_OK;
 }
 

@@ -121,7 +121,8 @@ class Stats(C.Structure):
                 ("output_nnz", C.c_ulonglong), ("heavy_rows", C.c_ulonglong),
                 ("heavy_chunks", C.c_ulonglong), ("output_materialized", C.c_int),
                 ("numeric_path", C.c_int), ("b_columns_ms", C.c_double),
-                ("mfma_pairs", C.c_ulonglong), ("precision", C.c_int)]
+                ("mfma_pairs", C.c_ulonglong), ("precision", C.c_int),
+                ("tail_reason", C.c_int), ("tail_records", C.c_ulonglong)]
 
 
 class TuneReport(C.Structure):
@@ -214,6 +215,7 @@ def load() -> C.CDLL:
                                         P(SMatrix)]),
         "rtt_autotune": (C.c_int, [vp, vp, C.c_int, P(Schedule), P(TuneReport)]),
         "rtt_last_c_row_stats": (C.c_int, [vp, vp, vp]),
+        "rtt_last_tail_records": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, P(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -789,6 +791,26 @@ def autotune(a: PreparedA, b: DynamicB, repeats: int = 2, sched: Schedule | None
     table = [{"numeric_path": inv[rep.numeric_path[i]], "fill": rep.fill[i],
               "lb_mode": lbinv[rep.lb_mode[i]], "ms": rep.ms[i]} for i in range(rep.candidates)]
     return sched, table, rep.chosen
+
+
+def last_tail_records(a: PreparedA, max_records: int = 1 << 20) -> dict:
+    """Heavy C tiles admitted by the last call's tail queue (slot order)."""
+    task = np.zeros(max(max_records, 1), np.int64)
+    sc, mt, ch = (np.zeros(max(max_records, 1), np.int32) for _ in range(3))
+    n = C.c_int()
+    _check(load().rtt_last_tail_records(a.handle, max_records, _ptr(task), _ptr(sc), _ptr(mt),
+                                        _ptr(ch), C.byref(n)), "last_tail_records")
+    k = n.value
+    return {"task": task[:k], "scans": sc[:k], "matches": mt[:k], "chunks": ch[:k]}
+
+
+def last_c_tile_ptr(a: PreparedA) -> np.ndarray:
+    """C's 64-bit tile row pointer (tilem + 1) of the last product on `a`."""
+    v = DeviceC()
+    _check(load().rtt_last_c_view(a.handle, C.byref(v)), "last_c_view")
+    out = np.zeros(v.tilem + 1, np.int64)
+    _check(load().rtt_memcpy_d2h(_ptr(out), C.c_void_p(v.tile_ptr), out.nbytes), "memcpy_d2h")
+    return out
 
 
 def last_c_row_stats(a: PreparedA):
