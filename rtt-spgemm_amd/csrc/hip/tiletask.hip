@@ -217,7 +217,11 @@ struct Fp64Product {
 //   C[r][c] += 2^(E_r + E_c) sum_d P_d 2^(-8(d+2)),
 // and batches and C tiles accumulate in FP64.  The only approximation is the
 // truncation of every operand 64 bits below its row (column) maximum, a
-// relative error of at most 2^-63 of that maximum per entry.
+// relative error of at most 2^-63 of that maximum per entry.  A batch whose
+// A rows or B columns span more than kSpreadBits binades (where the smallest
+// entries would keep fewer than 43 bits) runs on the FP64 matrix cores
+// instead, so every product keeps a relative error below 2^-42 and the
+// forward error of C stays below about 2^-41 sum |a||b| whatever the data.
 // Lane roles of v_mfma_f32_16x16x32_bf16 (measured on gfx950): A[i][k'] in
 // lane i + 16(k' >> 3), element k' & 7; B[k'][j] in lane j + 16(k' >> 3),
 // element k' & 7; D[i][j] in lane j + 16(i >> 2), element i & 3.
@@ -259,6 +263,11 @@ __device__ __forceinline__ uint32_t byte_bf16(uint32_t v, int byte)
     return __float_as_uint(static_cast<float>((v >> (8 * byte)) & 0xFFu)) >> 16;
 }
 
+// A batch whose A row (B column) exponents span more than this many binades
+// is multiplied on the FP64 matrix cores instead: its smallest entries would
+// keep fewer than 64 - 1 - kSpreadBits = 43 bits of the slices.
+constexpr int kSpreadBits = 20;
+
 struct SplitProduct {
     struct Lane {
         int a_phys;      // A tile bit this lane supplies (rows 0..7 only)
@@ -268,7 +277,9 @@ struct SplitProduct {
         int out_row0;    // first of the four C rows this lane's outputs cover
         int out_col;     // C column of this lane's outputs
         bool out_live;   // lane holds C outputs (rows 0..7, h = 0) after the halves combine
-        __device__ Lane(int lane)
+        LaneRoles f64;   // roles in the FP64 fallback (v_mfma_f64_4x4x4f64)
+        int f64_lane[4]; // lane whose FP64 entry is this lane's output v
+        __device__ Lane(int lane) : f64(lane)
         {
             const int i = lane & 15, k = lane >> 4;
             a_live = i < 8;
@@ -278,35 +289,62 @@ struct SplitProduct {
             out_row0 = 4 * (lane >> 4);
             out_col = lane & 7;
             out_live = lane < 32 && h == 0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = (out_row0 + v) & 7, c = out_col;
+                const int b = ((r >> 2) << 1) | (c >> 2);
+                f64_lane[v] = (c & 3) + 4 * b + 16 * (r & 3);
+            }
         }
     };
     struct Acc {
         double v[4] = {0.0, 0.0, 0.0, 0.0};
+        double f64 = 0.0;   // FP64-fallback partial, v_mfma_f64_4x4x4f64 layout
     };
     static constexpr unsigned long long kMfmaPerPair = 8;
+
+    // Lane's four outputs with the FP64-fallback partials added (every lane
+    // takes part in the shuffles).
+    __device__ static void finish(const Lane &ln, const Acc &acc, double out[4])
+    {
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            out[v] = acc.v[v] + __shfl(acc.f64, ln.f64_lane[v], kWave);
+    }
 
     __device__ static void store(const TileTask &g, const Lane &ln, int c_local,
                                  unsigned long long cm, long long co, const Acc &acc)
     {
         const unsigned long long m = read_lane64(cm, c_local);
         const long long off = static_cast<long long>(read_lane64(static_cast<unsigned long long>(co), c_local));
+        double out[4];
+        finish(ln, acc, out);
         if (!ln.out_live)
             return;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int bit = (ln.out_row0 + v) * 8 + ln.out_col;
             if ((m >> bit) & 1ull)
-                g.c_val[off + __popcll(m & ((1ull << bit) - 1ull))] = acc.v[v];
+                g.c_val[off + __popcll(m & ((1ull << bit) - 1ull))] = out[v];
         }
     }
 
     __device__ static void dump(const Lane &ln, const Acc &acc, double *slot64)
     {
+        double out[4];
+        finish(ln, acc, out);
         if (!ln.out_live)
             return;
 #pragma unroll
         for (int v = 0; v < 4; ++v)
-            slot64[(ln.out_row0 + v) * 8 + ln.out_col] = acc.v[v];
+            slot64[(ln.out_row0 + v) * 8 + ln.out_col] = out[v];
+    }
+
+    // Smallest biased exponent of the nonzero entries (0x7FF when none).
+    __device__ static int min_exp(double x)
+    {
+        const int e = align_exp(x);
+        return e ? e : 0x7FF;
     }
 
     // Pairs [p0, p0 + n) of one C tile, n <= kSplitBatch, added into acc.
@@ -314,7 +352,7 @@ struct SplitProduct {
                                  Acc &acc)
     {
         double av[kSplitBatch], bv[kSplitBatch];
-        int ea = 0, eb = 0;
+        int ea = 0, eb = 0, ea_lo = 0x7FF, eb_lo = 0x7FF;
 #pragma unroll
         for (int i = 0; i < kSplitBatch; ++i) {
             av[i] = 0.0;
@@ -329,13 +367,36 @@ struct SplitProduct {
             }
             ea = max(ea, align_exp(av[i]));
             eb = max(eb, align_exp(bv[i]));
+            ea_lo = min(ea_lo, min_exp(av[i]));
+            eb_lo = min(eb_lo, min_exp(bv[i]));
         }
-        // Row (column) maxima over the four k of the tile: lanes x, x+16,
-        // x+32, x+48 hold one row (column).
+        // Row (column) maxima and minima over the four k of the tile: lanes
+        // x, x+16, x+32, x+48 hold one row (column).
         ea = max(ea, __shfl_xor(ea, 16, kWave));
         ea = max(ea, __shfl_xor(ea, 32, kWave));
         eb = max(eb, __shfl_xor(eb, 16, kWave));
         eb = max(eb, __shfl_xor(eb, 32, kWave));
+        ea_lo = min(ea_lo, __shfl_xor(ea_lo, 16, kWave));
+        ea_lo = min(ea_lo, __shfl_xor(ea_lo, 32, kWave));
+        eb_lo = min(eb_lo, __shfl_xor(eb_lo, 16, kWave));
+        eb_lo = min(eb_lo, __shfl_xor(eb_lo, 32, kWave));
+        const bool wide = (ea_lo != 0x7FF && ea - ea_lo > kSpreadBits) ||
+                          (eb_lo != 0x7FF && eb - eb_lo > kSpreadBits);
+        if (__ballot(wide)) {
+            // Too wide a spread for the slices: these pairs on the FP64 cores.
+#pragma unroll
+            for (int i = 0; i < kSplitBatch; ++i) {
+                if (i >= n)
+                    break;
+                const int4 m = pm[p0 + i];
+                const double a = tile_entry(g.a_vals, static_cast<uint32_t>(m.x),
+                                            static_cast<uint32_t>(m.y), ln.f64.a_phys);
+                const double b = tile_entry(g.b_vals, static_cast<uint32_t>(m.z),
+                                            static_cast<uint32_t>(m.w), ln.f64.b_phys);
+                acc.f64 = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, acc.f64, 0, 0, 0);
+            }
+            return;
+        }
         ea = max(ea, 1);
         eb = max(eb, 1);
         f32x4 acc_t[8];

@@ -222,3 +222,24 @@ def test_auto_routes_rows_of_one_call_to_both_paths(gpu):
     tc2, st2 = gpu.tilespgemm(pa, pb2)
     assert st2["numeric_path"] == gpu.NUMERIC_PATHS["auto"]
     check_c(gpu, tc2, oracle_product(big, 1))
+
+
+def test_split_precision_on_wide_spread_operands(gpu):
+    """The Galerkin product R (A P) of a Poisson operator with 10^+-3
+    coefficient jumps: A P's tile columns span far more than 20 binades, so
+    the split form multiplies those batches on the FP64 cores, and the
+    forward error stays at most 1e-12 everywhere (the slices alone would
+    lose up to 2e-10 there)."""
+    g = 10
+    A = gpu.gen_poisson7_jumps(g, 3.0, 91)
+    P = gpu.gen_aggregation_prolongator(A, g, 2.0 / 3.0)
+    R = gpu.csr_transpose(P)
+    ap = oracle.spgemm(as_tuple(A), as_tuple(P))
+    AP = gpu.CsrMatrix(ap["m"], ap["n"], ap["rowptr"].astype(np.int32), ap["col"], ap["val"])
+    pr = gpu.prepare_identity_a(R)
+    pb = gpu.rebuild_b_general(pr, gpu.upload_csr(AP))
+    tc, st = gpu.tilespgemm(pr, pb, gpu.schedule(precision="split-bf16"))
+    ref = oracle.spgemm(as_tuple(R), as_tuple(AP))
+    err = split_bound_ok(gpu, R, AP, tc, ref)
+    print("R(AP) split", err, flush=True)
+    assert err["fwd_max"] <= SPLIT_FWD_TOL, err
