@@ -429,6 +429,21 @@ int rtt_tilespgemm(const rtt_prepared_a *a, const rtt_dynamic_b *b,
                    rtt_smatrix *output, rtt_stats *stats,
                    const rtt_schedule *sched);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Recomputes the values of the last C produced on `a`
+ * (by rtt_tilespgemm with B = b, every row on the TILE path, b not rebuilt
+ * since; a values-only rtt_update_b_values in between is what it is for)
+ * into c_values, nnz(C) doubles of device memory the caller owns, in C's
+ * value order, with `precision` (enum rtt_precision).  Allocates nothing:
+ * its scratch (8 row-queue heads and a pair counter, 40 bytes) lives in a
+ * fixed 256-byte device array per GPU, cleared by a kernel on every call.
+ * Calls that overlap on one GPU race on that array.  Heavy tiles run whole
+ * (no tail split).  Queues the kernels on the library stream and returns;
+ * rtt_sync() waits for them. */
+int rtt_tile_values_noalloc(const rtt_prepared_a *a, const rtt_dynamic_b *b, int precision,
+                            double *c_values);
+
 /* Tail queue of the TILE path under adaptive-tail (the reference's
  * admission, gate, chunk descriptors and chunk workspace reduction,
  * src/dmma_spgemm.h:2628-2776, 4902-5254, 6003-6049, 6981-7209): a C tile

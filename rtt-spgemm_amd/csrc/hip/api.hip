@@ -13,6 +13,17 @@
 
 #include "internal.hpp"
 
+// Scratch of rtt_tile_values_noalloc: a fixed device array instead of an
+// allocation (each GPU holds its own copy until the process ends).  Host code
+// uses only the per-device address hipGetSymbolAddress returned for it.
+constexpr size_t kNoAllocScratchBytes = 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void *g_scratch_dev[kMaxDevices] = {};   // address of `scratch` on each device
+} // namespace
+
 struct rtt_prepared_a {
     rtt::DevCsr csr;          // input CSR, rows column-sorted (B source)
     rtt::DevCsr csr_tiled;    // reordered copy when a permutation is given
@@ -31,6 +42,11 @@ struct rtt_prepared_a {
     rtt::DevC last_c;
     bool has_c = false;
     uint64_t id = 0;          // unique per preparation (never reused in a process)
+    // The last product: B handle and build generation, and whether every row
+    // ran on the tile-pair kernel (C's whole structure was written first).
+    const rtt_dynamic_b *last_b = nullptr;
+    uint64_t last_b_gen = 0;
+    bool last_all_tile = false;
 };
 
 // A B is built for one prepared A: its per-A-tile pair extents are indexed by
@@ -40,6 +56,7 @@ struct rtt_dynamic_b {
     rtt::BuildWork build;
     int aat = 0;
     uint64_t a_id = 0;        // rtt_prepared_a::id of the A it was built for
+    uint64_t gen = 0;         // structure builds of this handle (values updates keep it)
 };
 
 struct rtt_device_csr {
@@ -215,6 +232,11 @@ int rtt_init(int device)
         return fail(RTT_ERR_INVALID, "rtt_init: device %d not present (%d devices)", device,
                     count);
     RTT_HIP(hipSetDevice(device));
+    if (device < kMaxDevices && !g_scratch_dev[device]) {
+        void *p = nullptr;
+        RTT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[device] = p;
+    }
     RTT_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     RTT_HIP(hipDeviceGetAttribute(&c.num_cus, hipDeviceAttributeMultiprocessorCount, device));
     c.device = device;
@@ -471,6 +493,7 @@ int rtt_rebuild_b(const rtt_prepared_a *a_const, int aat, rtt_dynamic_b **inout,
     RTT_TRY(rtt::build_pair_extents(a->tiles, &b->tiles));
     RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
     b->a_id = a->id;
+    ++b->gen;
     st.total_ms = rtt::ms_since(t0);
     st.csr2tile_ms = st.total_ms - st.csr_row_permute_ms;
     if (stats)
@@ -547,6 +570,7 @@ int rtt_rebuild_b_general(const rtt_prepared_a *a_const, const rtt_device_csr *s
     RTT_TRY(rtt::build_pair_extents(a_const->tiles, &b->tiles));
     RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
     b->a_id = a_const->id;
+    ++b->gen;
     st.total_ms = rtt::ms_since(t0);
     st.csr2tile_ms = st.total_ms;
     if (stats)
@@ -750,6 +774,9 @@ int rtt_tilespgemm(const rtt_prepared_a *a_const, const rtt_dynamic_b *b, rtt_sm
                             sc.lb_mode, path == RTT_NUMERIC_VECTOR ? nullptr : &topt));
     stats->numeric_path = path;
     stats->precision = path == RTT_NUMERIC_VECTOR ? RTT_PRECISION_FP64 : sc.precision;
+    a->last_b = b;
+    a->last_b_gen = b->gen;
+    a->last_all_tile = path == RTT_NUMERIC_TILE;
     stats->mfma_pairs = times.mfma_pairs;
     stats->tail_reason = times.tail_reason;
     stats->tail_records = times.tail_records;
@@ -944,6 +971,29 @@ int rtt_autotune(const rtt_prepared_a *a, const rtt_dynamic_b *b, int repeats, r
     if (report)
         *report = rep;
     return RTT_OK;
+}
+
+int rtt_tile_values_noalloc(const rtt_prepared_a *a, const rtt_dynamic_b *b, int precision,
+                            double *c_values)
+{
+    rtt::clear_error();
+    RTT_TRY(rtt::require_ctx());
+    if (!a || !b || !c_values)
+        return fail(RTT_ERR_INVALID, "rtt_tile_values_noalloc: NULL argument");
+    RTT_TRY(check_binding(a, b, "rtt_tile_values_noalloc"));
+    if (!a->has_c || !a->last_all_tile || a->last_b != b || a->last_b_gen != b->gen)
+        return fail(RTT_ERR_INVALID,
+                    "rtt_tile_values_noalloc: needs C's structure from an rtt_tilespgemm on this "
+                    "(A, B) whose rows all took the TILE path, with B not rebuilt since");
+    if (!b->tiles.transpose_of && !b->tiles.cols_ready)
+        return fail(RTT_ERR_INVALID, "rtt_tile_values_noalloc: B's tile columns are not built");
+    const int dev = rtt::ctx().device;
+    if (dev < 0 || dev >= kMaxDevices || !g_scratch_dev[dev])
+        return fail(RTT_ERR_INVALID, "rtt_tile_values_noalloc: no scratch address for device %d",
+                    dev);
+    return rtt::tile_values_noalloc(a->tiles, rtt::b_columns(b->tiles), a->last_c, precision,
+                                    g_scratch_dev[dev], kNoAllocScratchBytes, c_values,
+                                    rtt::ctx().stream);
 }
 
 int rtt_last_tail_records(const rtt_prepared_a *a, int max_records, long long *task, int *scans,

@@ -270,6 +270,12 @@ TileColumns b_columns(const DevTiles &b);
 struct TileOpts;
 int tile_numeric(const DevTiles &a, const TileColumns &bc, const DevC &c, const TileOpts &opt,
                  SpgemmWork *w, hipStream_t s);
+// C's values for C's existing structure, every row on the tile-pair kernel,
+// written to c_values; the row-queue heads and pair counter live in the
+// caller's `scratch` (device memory of scratch_bytes).  Launches kernels
+// only: no allocation, no host synchronisation; no tail split.
+int tile_values_noalloc(const DevTiles &a, const TileColumns &bc, const DevC &c, int precision,
+                        void *scratch, size_t scratch_bytes, double *c_values, hipStream_t s);
 // Gate decision, admitted records and chunks of the last tail admission.
 int tail_stats(SpgemmWork *w, int *reason, unsigned long long *records,
                unsigned long long *chunks);

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -1087,6 +1088,57 @@ int tile_numeric(const DevTiles &a, const TileColumns &bc, const DevC &c, const 
         hipLaunchKernelGGL(tail_reduce_kernel, dim3(cus * 2), dim3(kTThreads), 0, s, g, t);
         RTT_HIP(hipGetLastError());
     }
+    return RTT_OK;
+}
+
+int tile_values_noalloc(const DevTiles &a, const TileColumns &bc, const DevC &c, int precision,
+                        void *scratch, size_t scratch_bytes, double *c_values, hipStream_t s)
+{
+    // Scratch: the kXcds row-queue heads, then the pair counter.
+    const size_t need = kXcds * sizeof(int) + sizeof(unsigned long long);
+    if (need > scratch_bytes)
+        return fail(RTT_ERR_INVALID, "tile values need %zu scratch bytes, %zu available", need,
+                    scratch_bytes);
+    if (c.tilem > INT_MAX)
+        return fail(RTT_ERR_INVALID, "C has %lld tile rows: beyond the row queues' range",
+                    (long long)c.tilem);
+    if (precision != 0 && precision != 1)
+        return fail(RTT_ERR_INVALID, "unknown tile numeric precision %d", precision);
+    if (c.tilem == 0 || c.num_tiles == 0)
+        return RTT_OK;
+    TileTask g;
+    g.rows = static_cast<int>(c.tilem);
+    g.a_trp = a.tile_row_ptr.as<int>();
+    g.a_meta = a.meta.as<int4>();
+    g.a_vals = a.values.as<double>();
+    g.b_cptr = bc.ptr;
+    g.b_cent = bc.ent;
+    g.b_vals = bc.vals;
+    g.c_ptr = c.tile_ptr.as<int64_t>();
+    g.c_col = c.tile_col.as<int>();
+    g.c_mask = c.tile_mask.as<unsigned long long>();
+    g.c_off = c.tile_nnz.as<int64_t>();
+    g.c_val = c_values;
+    g.a_row_off = a.row_payload_off.as<int>();
+    g.fill = 0.f;
+    g.queue = static_cast<int *>(scratch);
+    g.mfma_pairs = reinterpret_cast<unsigned long long *>(static_cast<char *>(scratch) +
+                                                          kXcds * sizeof(int));
+    g.heavy = nullptr;
+    g.tail = nullptr;
+    // Cleared by a kernel (the scratch keeps the previous call's contents).
+    hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(kXcds), 0, s, g.queue, kXcds,
+                       g.mfma_pairs);
+    const int cus = ctx().num_cus > 0 ? ctx().num_cus : 256;
+    const int blocks = static_cast<int>(std::min<int64_t>(
+        static_cast<int64_t>(cus) * 8, (static_cast<int64_t>(g.rows) + kTWaves - 1) / kTWaves));
+    if (precision == 0)
+        hipLaunchKernelGGL(tile_numeric_kernel<Fp64Product>, dim3(std::max(blocks, 1)),
+                           dim3(kTThreads), 0, s, g);
+    else
+        hipLaunchKernelGGL(tile_numeric_kernel<SplitProduct>, dim3(std::max(blocks, 1)),
+                           dim3(kTThreads), 0, s, g);
+    RTT_HIP(hipGetLastError());
     return RTT_OK;
 }
 

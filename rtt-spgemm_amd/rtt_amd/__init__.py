@@ -216,6 +216,7 @@ def load() -> C.CDLL:
         "rtt_autotune": (C.c_int, [vp, vp, C.c_int, P(Schedule), P(TuneReport)]),
         "rtt_last_c_row_stats": (C.c_int, [vp, vp, vp]),
         "rtt_last_tail_records": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, P(C.c_int)]),
+        "rtt_tile_values_noalloc": (C.c_int, [vp, vp, C.c_int, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -791,6 +792,13 @@ def autotune(a: PreparedA, b: DynamicB, repeats: int = 2, sched: Schedule | None
     table = [{"numeric_path": inv[rep.numeric_path[i]], "fill": rep.fill[i],
               "lb_mode": lbinv[rep.lb_mode[i]], "ms": rep.ms[i]} for i in range(rep.candidates)]
     return sched, table, rep.chosen
+
+
+def tile_values_noalloc(a: PreparedA, b: DynamicB, precision: str, c_values_ptr: int) -> None:
+    """Recomputes the last C's values into device memory at c_values_ptr
+    (nnz doubles the caller owns), allocating nothing; returns after queuing."""
+    _check(load().rtt_tile_values_noalloc(a.handle, b.handle, PRECISIONS[precision],
+                                          C.c_void_p(int(c_values_ptr))), "tile_values_noalloc")
 
 
 def last_tail_records(a: PreparedA, max_records: int = 1 << 20) -> dict:
