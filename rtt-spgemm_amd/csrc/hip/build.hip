@@ -99,6 +99,11 @@ __global__ void validate_csr_kernel(int rows, int cols, int64_t nnz,
     }
 }
 
+// Columns of its 64 rows a wave stages in LDS before the walk (when the rows
+// are contiguous in the CSR and fit), so that the lanes' entry-by-entry walk
+// reads LDS instead of issuing one dependent global load per entry.
+constexpr int kBuildStage = 3072;
+
 // One group of TR lanes per tile row.  COUNT pass: per tile row the number of
 // tiles, payload slots and dense tiles.  FILL pass: tile columns, masks,
 // value offsets and payload.  Duplicates (adjacent equal columns in a sorted
@@ -116,7 +121,27 @@ __global__ __launch_bounds__(kThreads) void tile_build_kernel(
     int2 *__restrict__ src_slot, int *__restrict__ dup_flag)
 {
     static_assert(TR * TC == 32, "tiles hold 32 elements");
+    static_assert(kWave % TR == 0, "a wave holds whole tile rows");
+    __shared__ int s_col[kThreads / kWave][kBuildStage];
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    // Row gid (= tile row gid / TR, row gid % TR of it).  The wave's rows are
+    // 64 consecutive rows; without a row map their entries are one stretch
+    // of the CSR, staged in LDS when it fits.
+    const int64_t wave_row0 = gid - lane_id();
+    const int64_t lim = min<int64_t>(static_cast<int64_t>(tile_row_count) * TR, active_rows);
+    int e0 = 0, e1 = 0;
+    if (!row_map && wave_row0 < lim) {
+        e0 = rp[wave_row0];
+        e1 = rp[min<int64_t>(wave_row0 + kWave, lim)];
+    }
+    const bool staged = !row_map && wave_row0 < lim && e1 - e0 <= kBuildStage;
+    int *stage = s_col[threadIdx.x / kWave];
+    if (staged) {
+        for (int x = e0 + lane_id(); x < e1; x += kWave)
+            stage[x - e0] = ci[x];
+    }
+    wave_sync();
+    auto col = [&](int x) { return staged ? stage[x - e0] : ci[x]; };
     const int64_t tr = gid / TR;
     const int r = static_cast<int>(gid % TR);
     if (tr >= tile_row_count)
@@ -129,14 +154,14 @@ __global__ __launch_bounds__(kThreads) void tile_build_kernel(
         end = rp[src + 1];
     }
     // Columns at or beyond active_cols are outside the tiled extent.
-    while (end > p && ci[end - 1] >= active_cols)
+    while (end > p && col(end - 1) >= active_cols)
         --end;
 
     int64_t ntiles = 0, payload = 0, ndense = 0;
     int64_t t_out = FILL ? tile_off[tr] : 0;
     int64_t p_out = FILL ? payload_off[tr] : 0;
     for (;;) {
-        int head = p < end ? ci[p] / TC : INT_MAX;
+        int head = p < end ? col(p) / TC : INT_MAX;
 #pragma unroll
         for (int o = TR / 2; o > 0; o >>= 1)
             head = min(head, __shfl_xor(head, o, TR));
@@ -145,7 +170,7 @@ __global__ __launch_bounds__(kThreads) void tile_build_kernel(
         uint32_t bits = 0;
         int q = p;
         while (q < end) {
-            const int c = ci[q];
+            const int c = col(q);
             if (c / TC != head)
                 break;
             const int lc = c % TC;
@@ -181,11 +206,11 @@ __global__ __launch_bounds__(kThreads) void tile_build_kernel(
             }
             int x = p;
             while (x < q) {
-                const int c = ci[x];
+                const int c = col(x);
                 const int x0 = x;
                 double s = val[x];
                 ++x;
-                while (x < q && ci[x] == c) {
+                while (x < q && col(x) == c) {
                     s += val[x];
                     ++x;
                 }

@@ -41,6 +41,8 @@ constexpr int kTWaves = kTThreads / kWave;
 constexpr int kAKeys = 256;     // A tile columns of a row staged in LDS
 constexpr int kPairCap = 256;   // pairs a wave buffers before running them
 constexpr int kBatch = 8;       // pairs whose operand loads are issued together
+constexpr int kRowGrab = 16;    // tile rows a wave takes from a row queue at once (one atomic:
+                                // per-row atomics on 8 heads serialise at the L2)
 static_assert(kPairCap % kBatch == 0 && kPairCap >= 2 * kWave, "pair buffer");
 
 struct TileTask {
@@ -521,10 +523,12 @@ __global__ __launch_bounds__(kTThreads) void tile_numeric_kernel(TileTask g)
         for (;;) {
             int r = 0;
             if (lane == 0)
-                r = atomicAdd(&g.queue[x], 1);
-            const int row = lo + read_lane(r, 0);
-            if (row >= hi)
+                r = atomicAdd(&g.queue[x], kRowGrab);
+            const int base = lo + read_lane(r, 0);
+            if (base >= hi)
                 break;
+            const int rend = min(base + kRowGrab, hi);
+          for (int row = base; row < rend; ++row) {
             const int64_t c0 = g.c_ptr[row], c1 = g.c_ptr[row + 1];
             if (c0 == c1)
                 continue;
@@ -611,6 +615,7 @@ __global__ __launch_bounds__(kTThreads) void tile_numeric_kernel(TileTask g)
             }
             if (lds_keys)
                 wave_sync();
+          }
         }
     }
     if (lane == 0 && done)
@@ -952,19 +957,26 @@ TileColumns b_columns(const DevTiles &b)
     return v;
 }
 
+// count[0]: tile rows routed to the tile-pair kernel at `fill`; count[1]:
+// tile rows holding any tile.  One atomic pair per wave.
 __global__ void count_tile_rows_kernel(int rows, const int *__restrict__ trp,
                                        const int *__restrict__ row_off, float fill,
                                        unsigned long long *__restrict__ count)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    bool take = false;
+    bool take = false, any = false;
     if (r < rows) {
         const int t = trp[r + 1] - trp[r];
-        take = t > 0 && static_cast<float>(row_off[r + 1] - row_off[r]) >= fill * static_cast<float>(t);
+        any = t > 0;
+        take = any && static_cast<float>(row_off[r + 1] - row_off[r]) >= fill * static_cast<float>(t);
     }
-    const unsigned long long m = __ballot(take);
-    if (lane_id() == 0 && m)
-        atomicAdd(count, static_cast<unsigned long long>(__popcll(m)));
+    const unsigned long long m = __ballot(take), n = __ballot(any);
+    if (lane_id() == 0) {
+        if (m)
+            atomicAdd(count, static_cast<unsigned long long>(__popcll(m)));
+        if (n)
+            atomicAdd(count + 1, static_cast<unsigned long long>(__popcll(n)));
+    }
 }
 
 int count_tile_rows(const DevTiles &a, float fill, DevBuf *scratch, int64_t *tile_rows,
@@ -977,12 +989,9 @@ int count_tile_rows(const DevTiles &a, float fill, DevBuf *scratch, int64_t *til
     hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(2), 0, s, reinterpret_cast<int *>(d), 0,
                        d + 1);
     const int R = a.tile_row_count;
-    if (R > 0) {
+    if (R > 0)
         hipLaunchKernelGGL(count_tile_rows_kernel, dim3(blocks_for(R, 256)), dim3(256), 0, s, R,
                            a.tile_row_ptr.as<int>(), a.row_payload_off.as<int>(), fill, d);
-        hipLaunchKernelGGL(count_tile_rows_kernel, dim3(blocks_for(R, 256)), dim3(256), 0, s, R,
-                           a.tile_row_ptr.as<int>(), a.row_payload_off.as<int>(), 0.f, d + 1);
-    }
     RTT_HIP(hipGetLastError());
     unsigned long long h[2] = {0, 0};
     RTT_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s));
