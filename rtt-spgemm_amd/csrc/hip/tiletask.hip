@@ -36,11 +36,13 @@ namespace rtt {
 
 namespace {
 
-constexpr int kTThreads = 256;
+constexpr int kTThreads = 128;
 constexpr int kTWaves = kTThreads / kWave;
-constexpr int kAKeys = 256;     // A tile columns of a row staged in LDS
-constexpr int kPairCap = 256;   // pairs a wave buffers before running them
+constexpr int kAKeys = 128;     // A tiles of a row staged in LDS (keys, masks, offsets)
+constexpr int kAVals = 512;     // A payload values of a row staged in LDS (0: read from HBM)
+constexpr int kPairCap = 128;   // pairs a wave buffers before running them
 constexpr int kBatch = 8;       // pairs whose operand loads are issued together
+constexpr int kEnumRounds = 4;  // rounds of 64 B column entries whose K are loaded together
 constexpr int kRowGrab = 16;    // tile rows a wave takes from a row queue at once (one atomic:
                                 // per-row atomics on 8 heads serialise at the L2)
 static_assert(kPairCap % kBatch == 0 && kPairCap >= 2 * kWave, "pair buffer");
@@ -147,6 +149,10 @@ struct Fp64Product {
         double v = 0.0;
     };
     static constexpr unsigned long long kMfmaPerPair = 1;
+    // Waves per SIMD the tile kernel is compiled for: the kernel is bound by
+    // memory latency, and 4 waves (at most 128 VGPRs) ran it 20 % faster
+    // than the 3 the compiler picks unconstrained.
+    static constexpr int kOccupancy = 4;
 
     __device__ static void store(const TileTask &g, const Lane &ln, int c_local,
                                  unsigned long long cm, long long co, const Acc &acc)
@@ -164,23 +170,43 @@ struct Fp64Product {
         slot64[ln.roles.c_bit] = acc.v;
     }
 
+    // B entries of buffered pairs [p0, p0 + kBatch) (zero past n).
+    __device__ static void load_b(const TileTask &g, const Lane &ln, const int4 *pm, int p0, int n,
+                                  double (&bv)[kBatch])
+    {
+#pragma unroll
+        for (int i = 0; i < kBatch; ++i) {
+            bv[i] = 0.0;
+            if (p0 + i < n) {
+                const int4 m = pm[p0 + i];
+                bv[i] = tile_entry(g.b_vals, static_cast<uint32_t>(m.z), static_cast<uint32_t>(m.w),
+                                   ln.roles.b_phys);
+            }
+        }
+    }
+
     // Runs buffered pairs [0, n) in order; a change of C tile stores the
-    // finished one.
-    __device__ static void run(const TileTask &g, const Lane &ln, const int4 *pm,
-                               const unsigned char *pc, int n, int &cur, Acc &acc,
+    // finished one.  The B loads of the next batch are issued before the
+    // products of the current one, so one batch's loads are always in flight
+    // behind the matrix-core work.
+    __device__ static void run(const TileTask &g, const Lane &ln, const double *a_vals,
+                               const int4 *pm, const unsigned char *pc, int n, int &cur, Acc &acc,
                                unsigned long long cm, long long co)
     {
+        double bv[kBatch];
+        load_b(g, ln, pm, 0, n, bv);
         for (int p0 = 0; p0 < n; p0 += kBatch) {
+            double bn[kBatch];
+            load_b(g, ln, pm, p0 + kBatch, n, bn);
             const int nbat = min(kBatch, n - p0);
-            double av[kBatch], bv[kBatch];
+            double av[kBatch];
 #pragma unroll
             for (int i = 0; i < kBatch; ++i) {
+                av[i] = 0.0;
                 if (i < nbat) {
                     const int4 m = pm[p0 + i];
-                    av[i] = tile_entry(g.a_vals, static_cast<uint32_t>(m.x),
+                    av[i] = tile_entry(a_vals, static_cast<uint32_t>(m.x),
                                        static_cast<uint32_t>(m.y), ln.roles.a_phys);
-                    bv[i] = tile_entry(g.b_vals, static_cast<uint32_t>(m.z),
-                                       static_cast<uint32_t>(m.w), ln.roles.b_phys);
                 }
             }
 #pragma unroll
@@ -196,6 +222,9 @@ struct Fp64Product {
                     acc.v = __builtin_amdgcn_mfma_f64_4x4x4f64(av[i], bv[i], acc.v, 0, 0, 0);
                 }
             }
+#pragma unroll
+            for (int i = 0; i < kBatch; ++i)
+                bv[i] = bn[i];
         }
     }
 };
@@ -305,6 +334,10 @@ struct SplitProduct {
         double f64 = 0.0;   // FP64-fallback partial, v_mfma_f64_4x4x4f64 layout
     };
     static constexpr unsigned long long kMfmaPerPair = 8;
+    // No register cap: held to 4 waves per SIMD this product spills about 240
+    // VGPRs to scratch, and that build gave wrong values; unconstrained it
+    // needs about 207 VGPRs (2 waves) and spills nothing.
+    static constexpr int kOccupancy = 1;
 
     // Lane's four outputs with the FP64-fallback partials added (every lane
     // takes part in the shuffles).
@@ -351,8 +384,8 @@ struct SplitProduct {
     }
 
     // Pairs [p0, p0 + n) of one C tile, n <= kSplitBatch, added into acc.
-    __device__ static void batch(const TileTask &g, const Lane &ln, const int4 *pm, int p0, int n,
-                                 Acc &acc)
+    __device__ static void batch(const TileTask &g, const Lane &ln, const double *a_vals,
+                                 const int4 *pm, int p0, int n, Acc &acc)
     {
         double av[kSplitBatch], bv[kSplitBatch];
         int ea = 0, eb = 0, ea_lo = 0x7FF, eb_lo = 0x7FF;
@@ -363,7 +396,7 @@ struct SplitProduct {
             if (i < n) {
                 const int4 m = pm[p0 + i];
                 if (ln.a_live)
-                    av[i] = tile_entry(g.a_vals, static_cast<uint32_t>(m.x),
+                    av[i] = tile_entry(a_vals, static_cast<uint32_t>(m.x),
                                        static_cast<uint32_t>(m.y), ln.a_phys);
                 bv[i] = tile_entry(g.b_vals, static_cast<uint32_t>(m.z),
                                    static_cast<uint32_t>(m.w), ln.b_phys);
@@ -392,7 +425,7 @@ struct SplitProduct {
                 if (i >= n)
                     break;
                 const int4 m = pm[p0 + i];
-                const double a = tile_entry(g.a_vals, static_cast<uint32_t>(m.x),
+                const double a = tile_entry(a_vals, static_cast<uint32_t>(m.x),
                                             static_cast<uint32_t>(m.y), ln.f64.a_phys);
                 const double b = tile_entry(g.b_vals, static_cast<uint32_t>(m.z),
                                             static_cast<uint32_t>(m.w), ln.f64.b_phys);
@@ -475,8 +508,8 @@ struct SplitProduct {
         }
     }
 
-    __device__ static void run(const TileTask &g, const Lane &ln, const int4 *pm,
-                               const unsigned char *pc, int n, int &cur, Acc &acc,
+    __device__ static void run(const TileTask &g, const Lane &ln, const double *a_vals,
+                               const int4 *pm, const unsigned char *pc, int n, int &cur, Acc &acc,
                                unsigned long long cm, long long co)
     {
         int p = 0;
@@ -491,16 +524,139 @@ struct SplitProduct {
             int q = p + 1;
             while (q < n && q - p < kSplitBatch && pc[q] == t)
                 ++q;
-            batch(g, ln, pm, p, q - p, acc);
+            batch(g, ln, a_vals, pm, p, q - p, acc);
             p = q;
         }
     }
 };
 
+// The C tiles of tile row `row` ([c0, c1)), up to 64 at a time: their B
+// column entries are matched against A row I's K list and the pairs run.
+// kStaged: A row I's keys, {mask, offset} and payload are in LDS (offsets
+// relative to the row's first value); otherwise they are read from HBM.
+template <class P, bool kStaged>
+__device__ __forceinline__ unsigned long long tile_row(const TileTask &g,
+                                                       const typename P::Lane &ln, int64_t c0,
+                                                       int64_t c1, int a0, int na, const int *keys,
+                                                       const int2 *am, const double *avals,
+                                                       int4 *pm, unsigned char *pc, int *start,
+                                                       int *cbeg, bool heavy_on)
+{
+    const int lane = lane_id();
+    const double *a_vals = (kStaged && kAVals > 0) ? avals : g.a_vals;
+    unsigned long long done = 0;
+    for (int64_t cc = c0; cc < c1; cc += kWave) {
+        const int nc = static_cast<int>(min<int64_t>(kWave, c1 - cc));
+        // Lane c of the chunk holds C tile cc + c: mask, offset, B column run.
+        unsigned long long cm = 0;
+        long long co = 0;
+        int nb = 0, cb = 0;
+        if (lane < nc) {
+            const int j = g.c_col[cc + lane];
+            cm = g.c_mask[cc + lane];
+            co = g.c_off[cc + lane];
+            cb = g.b_cptr[j];
+            nb = g.b_cptr[j + 1] - cb;
+            // Heavy tiles are computed by the chunk kernels instead.
+            const int64_t t = cc + lane;
+            if (heavy_on && ((g.heavy[t >> 5] >> (t & 31)) & 1u))
+                nb = 0;
+        }
+        const int incl = wave_inclusive_scan(nb);
+        const int total = __shfl(incl, kWave - 1, kWave);
+        start[lane] = incl - nb;
+        cbeg[lane] = cb;
+        if (lane == 0)
+            start[kWave] = total;
+        wave_sync();
+        int npairs = 0;
+        int cur = -1;
+        typename P::Acc acc;
+        const int rounds = (total + kWave - 1) / kWave;
+        for (int r0 = 0; r0 < rounds; r0 += kEnumRounds) {
+            // The K of kEnumRounds rounds of B column entries, loaded at once.
+            int key[kEnumRounds], bix[kEnumRounds], cl[kEnumRounds];
+#pragma unroll
+            for (int i = 0; i < kEnumRounds; ++i) {
+                const int q = (r0 + i) * kWave + lane;
+                key[i] = -1;
+                bix[i] = 0;
+                cl[i] = 0;
+                if (r0 + i < rounds && q < total) {
+                    const int c = upper_index(start, nc, q);
+                    cl[i] = c;
+                    bix[i] = cbeg[c] + (q - start[c]);
+                    key[i] = g.b_cent[bix[i]].x;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < kEnumRounds; ++i) {
+                if (r0 + i >= rounds)
+                    break;
+                int ai = -1;
+                if (key[i] >= 0) {
+                    int l = 0, h = na;
+                    while (l < h) {
+                        const int mid = (l + h) >> 1;
+                        const int k = kStaged ? keys[mid] : g.a_meta[a0 + mid].x;
+                        if (k < key[i])
+                            l = mid + 1;
+                        else
+                            h = mid;
+                    }
+                    if (l < na && (kStaged ? keys[l] : g.a_meta[a0 + l].x) == key[i])
+                        ai = l;
+                }
+                const unsigned long long match = __ballot(ai >= 0);
+                if (ai >= 0) {
+                    int amask, aoff;
+                    if (kStaged) {
+                        const int2 m = am[ai];
+                        amask = m.x;
+                        aoff = m.y;
+                    } else {
+                        const int4 m = g.a_meta[a0 + ai];
+                        amask = m.y;
+                        aoff = m.w;
+                    }
+                    const int slot = npairs + __popcll(match & ((1ull << lane) - 1ull));
+                    pm[slot] = make_int4(amask, aoff, bix[i], 0);
+                    pc[slot] = static_cast<unsigned char>(cl[i]);
+                }
+                npairs += __popcll(match);
+                // Run the buffered pairs once another round might not fit.
+                const bool last = r0 + i + 1 >= rounds;
+                if (npairs + kWave > kPairCap || last) {
+                    wave_sync();
+                    // The matched B entries' {mask, offset}: one round of
+                    // independent loads for the whole buffer.
+                    for (int p = lane; p < npairs; p += kWave) {
+                        const int4 b = g.b_cent[pm[p].z];
+                        pm[p].z = b.y;
+                        pm[p].w = b.w;
+                    }
+                    wave_sync();
+                    P::run(g, ln, a_vals, pm, pc, npairs, cur, acc, cm, co);
+                    done += static_cast<unsigned long long>(npairs);
+                    npairs = 0;
+                    wave_sync();
+                }
+            }
+        }
+        if (cur >= 0)
+            P::store(g, ln, cur, cm, co, acc);
+        wave_sync();
+    }
+    return done;
+}
+
 template <class P>
-__global__ __launch_bounds__(kTThreads) void tile_numeric_kernel(TileTask g)
+__global__ __launch_bounds__(kTThreads) __attribute__((amdgpu_waves_per_eu(P::kOccupancy)))
+void tile_numeric_kernel(TileTask g)
 {
     __shared__ int s_keys[kTWaves][kAKeys];
+    __shared__ int2 s_am[kTWaves][kAKeys];              // {mask, offset | dense} from the row's first value
+    __shared__ double s_av[kTWaves][kAVals > 0 ? kAVals : 1];   // A row's payload
     __shared__ int4 s_pm[kTWaves][kPairCap];            // {A mask, A offset, B mask, B offset}
     __shared__ unsigned char s_pc[kTWaves][kPairCap];   // C tile (chunk-local) of each pair
     __shared__ int s_start[kTWaves][kWave + 1];
@@ -508,10 +664,8 @@ __global__ __launch_bounds__(kTThreads) void tile_numeric_kernel(TileTask g)
     const int w = threadIdx.x / kWave;
     const int lane = lane_id();
     int *keys = s_keys[w];
-    int4 *pm = s_pm[w];
-    unsigned char *pc = s_pc[w];
-    int *start = s_start[w];
-    int *cbeg = s_cb[w];
+    int2 *am = s_am[w];
+    double *av = s_av[w];
     const typename P::Lane ln(lane);
     unsigned long long done = 0;   // pairs this wave multiplied
     const bool heavy_on = g.heavy && g.tail->enabled;
@@ -528,94 +682,38 @@ __global__ __launch_bounds__(kTThreads) void tile_numeric_kernel(TileTask g)
             if (base >= hi)
                 break;
             const int rend = min(base + kRowGrab, hi);
-          for (int row = base; row < rend; ++row) {
-            const int64_t c0 = g.c_ptr[row], c1 = g.c_ptr[row + 1];
-            if (c0 == c1)
-                continue;
-            const int a0 = g.a_trp[row];
-            const int na = g.a_trp[row + 1] - a0;
-            // Rows below the fill threshold belong to the LDS-accumulator kernels.
-            if (static_cast<float>(g.a_row_off[row + 1] - g.a_row_off[row]) <
-                g.fill * static_cast<float>(na))
-                continue;
-            const bool lds_keys = na <= kAKeys;
-            if (lds_keys) {
-                for (int i = lane; i < na; i += kWave)
-                    keys[i] = g.a_meta[a0 + i].x;
-                wave_sync();
-            }
-            for (int64_t cc = c0; cc < c1; cc += kWave) {
-                const int nc = static_cast<int>(min<int64_t>(kWave, c1 - cc));
-                // Lane c of the chunk holds C tile cc + c: mask, offset, B column run.
-                unsigned long long cm = 0;
-                long long co = 0;
-                int nb = 0, cb = 0;
-                if (lane < nc) {
-                    const int j = g.c_col[cc + lane];
-                    cm = g.c_mask[cc + lane];
-                    co = g.c_off[cc + lane];
-                    cb = g.b_cptr[j];
-                    nb = g.b_cptr[j + 1] - cb;
-                    // Heavy tiles are computed by the chunk kernels instead.
-                    const int64_t t = cc + lane;
-                    if (heavy_on && ((g.heavy[t >> 5] >> (t & 31)) & 1u))
-                        nb = 0;
+            for (int row = base; row < rend; ++row) {
+                const int64_t c0 = g.c_ptr[row], c1 = g.c_ptr[row + 1];
+                if (c0 == c1)
+                    continue;
+                const int a0 = g.a_trp[row];
+                const int na = g.a_trp[row + 1] - a0;
+                const int v0 = g.a_row_off[row];
+                const int nv = g.a_row_off[row + 1] - v0;
+                // Rows below the fill threshold belong to the LDS-accumulator kernels.
+                if (static_cast<float>(nv) < g.fill * static_cast<float>(na))
+                    continue;
+                if (na <= kAKeys && (kAVals == 0 || nv <= kAVals)) {
+                    // Offsets relative to the row's first value when the
+                    // payload is staged too.
+                    const uint32_t base = kAVals > 0 ? static_cast<uint32_t>(v0) : 0u;
+                    for (int i = lane; i < na; i += kWave) {
+                        const int4 m = g.a_meta[a0 + i];
+                        keys[i] = m.x;
+                        const uint32_t ow = static_cast<uint32_t>(m.w);
+                        am[i] = make_int2(m.y, static_cast<int>(((ow & ~kDense) - base) | (ow & kDense)));
+                    }
+                    if (kAVals > 0)
+                        for (int i = lane; i < nv; i += kWave)
+                            av[i] = g.a_vals[v0 + i];
+                    wave_sync();
+                    done += tile_row<P, true>(g, ln, c0, c1, a0, na, keys, am, av, s_pm[w], s_pc[w],
+                                              s_start[w], s_cb[w], heavy_on);
+                } else {
+                    done += tile_row<P, false>(g, ln, c0, c1, a0, na, keys, am, av, s_pm[w],
+                                               s_pc[w], s_start[w], s_cb[w], heavy_on);
                 }
-                const int incl = wave_inclusive_scan(nb);
-                const int total = __shfl(incl, kWave - 1, kWave);
-                start[lane] = incl - nb;
-                cbeg[lane] = cb;
-                if (lane == 0)
-                    start[kWave] = total;
-                wave_sync();
-                int npairs = 0;
-                int cur = -1;
-                typename P::Acc acc;
-                for (int q0 = 0; q0 < total; q0 += kWave) {
-                    const int q = q0 + lane;
-                    int ai = -1;
-                    int4 be = make_int4(0, 0, 0, 0);
-                    int c_local = 0;
-                    if (q < total) {
-                        c_local = upper_index(start, nc, q);
-                        be = g.b_cent[cbeg[c_local] + (q - start[c_local])];
-                        int l = 0, h = na;
-                        while (l < h) {
-                            const int mid = (l + h) >> 1;
-                            const int k = lds_keys ? keys[mid] : g.a_meta[a0 + mid].x;
-                            if (k < be.x)
-                                l = mid + 1;
-                            else
-                                h = mid;
-                        }
-                        if (l < na && (lds_keys ? keys[l] : g.a_meta[a0 + l].x) == be.x)
-                            ai = a0 + l;
-                    }
-                    const unsigned long long match = __ballot(ai >= 0);
-                    if (ai >= 0) {
-                        const int4 am = g.a_meta[ai];
-                        const int slot = npairs + __popcll(match & ((1ull << lane) - 1ull));
-                        pm[slot] = make_int4(am.y, am.w, be.y, be.w);
-                        pc[slot] = static_cast<unsigned char>(c_local);
-                    }
-                    npairs += __popcll(match);
-                    // Run the buffered pairs once another round might not fit.
-                    const bool last = q0 + kWave >= total;
-                    if (npairs + kWave > kPairCap || last) {
-                        wave_sync();
-                        P::run(g, ln, pm, pc, npairs, cur, acc, cm, co);
-                        done += static_cast<unsigned long long>(npairs);
-                        npairs = 0;
-                        wave_sync();
-                    }
-                }
-                if (cur >= 0)
-                    P::store(g, ln, cur, cm, co, acc);
-                wave_sync();
             }
-            if (lds_keys)
-                wave_sync();
-          }
         }
     }
     if (lane == 0 && done)
@@ -887,7 +985,7 @@ __global__ __launch_bounds__(kTThreads) void tail_chunk_kernel(TileTask g, TailA
             npairs += __popcll(match);
             if (npairs + kWave > kPairCap || p0 + kWave >= e1) {
                 wave_sync();
-                P::run(g, ln, pm, pc, npairs, cur, acc, 0ull, 0ll);
+                P::run(g, ln, g.a_vals, pm, pc, npairs, cur, acc, 0ull, 0ll);
                 npairs = 0;
                 wave_sync();
             }
