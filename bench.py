@@ -680,6 +680,16 @@ def main():
     print(json.dumps({"breakdown": breakdown, "nnz_cub": nnz_cub, "nnz_c": nnz_c,
                       "tiles_c": st["output_tiles"], "pairs": st["pair_count"], "mfma": mfma}),
           file=sys.stderr, flush=True)
+    # A*A^T with identity rows: the B update copies each B tile's record from
+    # A's transpose_src records, built with A's column-major values at A's
+    # (untimed) preparation.  The reference does the corresponding transpose
+    # inside its timed B rebuild, so the step is also reported with it.
+    offline_tr = None
+    if aat and args.layout == "identity":
+        tr_ms = float(pa.stats["transpose_src_ms"])
+        offline_tr = {"transpose_src_ms": round(tr_ms, 4),
+                      "ms_per_step_with_it": round(ms_per_step + tr_ms, 4),
+                      "value_with_it": round(2.0 * nnz_cub / ((ms_per_step + tr_ms) * 1e6), 3)}
     control = finish_control(control, nnz_cub, nnz_c)
     cpu = None
     if not args.no_cpu_baseline:
@@ -699,8 +709,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: C = A*A{'^T' if aat else ''}, {desc}, "
                                f"{a.m} rows, {a.nnz} nnz, nnzCub {nnz_cub}, nnzC {nnz_c}",
-                   "precision": "plain FP64", "lb_mode": args.lb_mode,
-                   "numeric_path": "tile" if tile_path else "vector",
+                   "precision": "plain FP64" if args.precision == "fp64" else
+                                "split BF16 slices, FP64 recombination",
+                   "lb_mode": args.lb_mode,
+                   "numeric_path": {rt.NUMERIC_PATHS["auto"]: "auto (vector and tile rows)",
+                                    rt.NUMERIC_PATHS["vector"]: "vector",
+                                    rt.NUMERIC_PATHS["tile"]: "tile"}[st["numeric_path"]],
                    "layout": args.layout,
                    "pipeline_hbm_frac": round(b_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                               5)},
@@ -712,6 +726,7 @@ def main():
                      "traffic_source": traffic_src,
                      "mfma": mfma},
         "cpu_baseline": cpu,
+        "offline_transpose": offline_tr,
         "control": control,
         "second_mode": other,
         "forms": forms,

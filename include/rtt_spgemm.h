@@ -138,6 +138,10 @@ typedef struct rtt_offline_a_stats {            /* DmmaOfflineAStats */
     double tile_build_ms;
     double total_ms;
     int rows_were_sorted;     /* 1 if the input rows were already column-sorted */
+    double transpose_src_ms;  /* part of tile_build_ms: the column-major values and the
+                               * per-tile records that the A*A^T B update (identity rows)
+                               * copies from; the reference builds the corresponding
+                               * transpose inside its timed B rebuild */
 } rtt_offline_a_stats;
 
 /* gpu_prepare_identity_a (src/gpu_dmma_tiles.h:2443-2502): upload and

@@ -75,7 +75,7 @@ class OfflineAStats(C.Structure):
     _fields_ = [("h2d_ms", C.c_double), ("validation_ms", C.c_double),
                 ("canonicalize_ms", C.c_double), ("reorder_ms", C.c_double),
                 ("tile_build_ms", C.c_double), ("total_ms", C.c_double),
-                ("rows_were_sorted", C.c_int)]
+                ("rows_were_sorted", C.c_int), ("transpose_src_ms", C.c_double)]
 
 
 class BUpdateStats(C.Structure):

@@ -354,3 +354,27 @@ def test_reorder_dumps_match_the_plan(tmp_path):
     assert _read_kv(prefix + "_id_unified_stats.csv")["algorithm"] == "identity_baseline"
     with pytest.raises(rtt_amd.RttError):
         rtt_amd.dump_reorder(prefix, a, np.zeros(a.m, np.int32), i, st)
+
+
+def test_b_values_clear_policy_matches_reference_vectors():
+    """rtt_choose_b_values_clear against the committed outputs of the
+    reference's dmma_choose_b_values_clear (tests/golden/reference_vectors.json,
+    written from the reference sources compiled into oracle/_ref): every
+    policy value times every combination of the eight preconditions, and the
+    reason and policy names."""
+    import json
+    path = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")
+    with open(path) as f:
+        g = json.load(f)["b_values_clear"]
+    cols = g["columns"]
+    assert len(g["rows"]) == 5 * 256
+    lib = rtt_amd.load()
+    for row in g["rows"]:
+        policy, flags, want = row[0], row[1:9], tuple(row[9:])
+        got = rtt_amd.choose_b_values_clear(policy, **dict(zip(cols[1:9], flags)))
+        assert (got["clear_payload"], got["skip_clear"], got["fallback"],
+                got["reason"]) == want, (policy, flags)
+    for r, name in g["names"]["reason"].items():
+        assert lib.rtt_b_values_clear_reason_name(int(r)).decode() == name
+    for p, name in g["names"]["policy"].items():
+        assert lib.rtt_b_values_clear_policy_name(int(p)).decode() == name

@@ -7,6 +7,7 @@ answer of the reference's design doc, the reference's own tail planner and
 Matrix Market reader compiled from its sources (oracle/_ref), and exact
 rational arithmetic for the double-double truth.
 """
+import os
 from fractions import Fraction
 
 import numpy as np
@@ -317,3 +318,32 @@ def test_mmio_reader_matches_reference(tmp_path, kind, sym):
     ref = oracle.ref_mmio_read(p)
     for g, r in zip(got, ref):
         assert np.array_equal(np.asarray(g), np.asarray(r))
+
+
+# ---- committed vectors of the reference's functions ----------------------------
+# tests/golden/reference_vectors.json was written by
+# tests/golden/make_reference_vectors.py from the reference sources compiled
+# into oracle/_ref; these checks need no reference build.
+
+def golden():
+    import json
+    path = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+def test_tail_chunk_count_matches_reference_vectors():
+    g = golden()["tail_chunk_count"]
+    assert len(g["rows"]) > 100
+    for scans, matches, thr, tgt, mx, want in g["rows"]:
+        assert oracle.tail_chunk_count(int(scans), int(matches), thr, tgt, int(mx)) == want
+
+
+def test_tail_plan_matches_reference_vectors():
+    for p in golden()["plan_tail_queue"]:
+        got = oracle.plan_tail_queue(np.array(p["scans"]), np.array(p["matches"]),
+                                     np.array(p["output_nnz"]), **p["settings"])
+        for k in ("reason", "heavy", "chunks", "workspace"):
+            assert got[k] == p[k], (p["settings"], k)
+        assert got["heavy_ids"].tolist() == p["heavy_ids"]
+        assert got["heavy_chunks"].tolist() == p["heavy_chunks"]
