@@ -41,6 +41,32 @@ def test_tile_path_matches_oracle(gpu, name, make, aat, positive):
     tile_run(gpu, make(gpu), aat, positive=positive)
 
 
+def test_tile_path_rows_beyond_the_lds_staging(gpu):
+    """Tile rows whose A tiles (more than 128) or A payload (more than 512
+    values) do not fit the kernel's LDS staging read A from HBM instead; rows
+    of both kinds and rows that fit run in one call."""
+    import scipy.sparse as sp
+    n = 4000
+    wide = gpu.gen_random(16, n, 300, 51).to_scipy()              # ~900 A tiles per tile row
+    deep = gpu.gen_random(16, 480, 100, 52).to_scipy()            # <= 120 A tiles, ~800 values
+    deep = sp.hstack([deep, sp.csr_matrix((16, n - 480))])
+    small = gpu.gen_random(32, n, 4, 53).to_scipy()
+    a = sp.vstack([wide, deep, small]).tocsr()
+    a.sort_indices()
+    m = gpu.CsrMatrix(a.shape[0], a.shape[1], a.indptr.astype(np.int32),
+                      a.indices.astype(np.int32), a.data.astype(np.float64))
+    at = gpu.prepare_identity_a(m).tiles()
+    per_row = np.diff(at["tile_row_ptr"])
+    pay = np.diff(at["value_offsets"][at["tile_row_ptr"]])
+    assert per_row.max() > 128 and ((per_row <= 128) & (pay > 512)).any()
+    b = gpu.gen_random(4000, 300, 3, 54)
+    pa = gpu.prepare_identity_a(m)
+    pb = gpu.rebuild_b_general(pa, gpu.upload_csr(b))
+    tc, _ = gpu.tilespgemm(pa, pb, gpu.schedule(numeric_path="tile"))
+    check_c(gpu, tc, oracle.spgemm(as_tuple(m), as_tuple(b)))
+    tile_run(gpu, m, 1)
+
+
 @pytest.mark.parametrize("dense_threshold", [1, 24, 32])
 def test_tile_path_dense_thresholds(gpu, dense_threshold):
     """Dense (zero-filled, 32-slot) and sparse payloads decode the same."""
