@@ -362,6 +362,7 @@ def cpu_baseline(a, aat, nnz_cub, budget_s=2.5, repeats=5):
 # Kernel names in the rocprofv3 counter summaries, per breakdown key.
 PROFILE_KERNELS = {
     "numeric": "numeric_row_kernel<SmallRow",
+    "numeric_tile": "tile_numeric_kernel<Fp64Product>",
     "symbolic_count": "symbolic_wave_kernel",
     "symbolic_fill": "symbolic_write_kernel",
 }
@@ -369,22 +370,28 @@ PROFILE_KERNELS = {
 
 def profile_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary of this workload: FETCH_SIZE and WRITE_SIZE (KiB per dispatch),
-    each scaled by what the counter reads for a device copy of known size.
-    None when no summary covers the kernel."""
+    summary of this workload: FETCH_SIZE and WRITE_SIZE (KiB per dispatch,
+    from the table whose first counter columns they are), each scaled by what
+    the counter reads for a device copy of known size.  None when no summary
+    covers the kernel."""
     import re
     path = os.path.join(ROOT, "profiles", f"{workload}_counters.md")
     if not os.path.exists(path):
         return None, None
     text = open(path).read()
-    m = re.search(r"FETCH_SIZE x 1024 x ([0-9]+(?:\.[0-9]+)?) and WRITE_SIZE x 1024 x ([0-9]+(?:\.[0-9]+)?)", text)
+    m = re.search(r"FETCH_SIZE\s+x\s+1024\s+x\s+([0-9]+(?:\.[0-9]+)?)\s+and\s+WRITE_SIZE\s+x\s+"
+                  r"1024\s+x\s+([0-9]+(?:\.[0-9]+)?)", text)
     if not m:
         return None, None
     sf, sw = float(m.group(1)), float(m.group(2))
     name = PROFILE_KERNELS.get(kernel)
+    in_table = False
     for line in text.splitlines():
         cells = [c.strip(" `") for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 3 and name and cells[0].startswith(name):
+        if len(cells) >= 3 and cells[0] == "kernel":
+            in_table = cells[1].startswith("FETCH_SIZE") and cells[2].startswith("WRITE_SIZE")
+            continue
+        if in_table and len(cells) >= 3 and name and cells[0].startswith(name):
             try:
                 fetch_kib, write_kib = float(cells[1]), float(cells[2])
             except ValueError:
@@ -660,7 +667,9 @@ def main():
     }
     dom = max(kern, key=kern.get)
     achieved = alg[dom] / (kern[dom] * 1e-3) / 1e9
-    traffic, traffic_src = profile_traffic(args.workload, dom)
+    traffic, traffic_src = profile_traffic(
+        args.workload, "numeric_tile" if dom == "numeric" and tile_path and
+        args.precision == "fp64" else dom)
     b_alg = csr_bytes(a.m, a.nnz) + csr_bytes(m_b, a.nnz) + csr_bytes(a.m, nnz_c)
     breakdown = {
         "b_update_ms": float(np.median([s[1]["total_ms"] for s in steps])),
@@ -681,8 +690,7 @@ def main():
                       "tiles_c": st["output_tiles"], "pairs": st["pair_count"], "mfma": mfma}),
           file=sys.stderr, flush=True)
     # A*A^T with identity rows: the B update copies each B tile's record from
-    # A's transpose_src records, built with A's column-major values at A's
-    # (untimed) preparation.  The reference does the corresponding transpose
+    # A's transpose_src records, built at A's (untimed) preparation.  The reference does the corresponding transpose
     # inside its timed B rebuild, so the step is also reported with it.
     offline_tr = None
     if aat and args.layout == "identity":

@@ -138,10 +138,10 @@ typedef struct rtt_offline_a_stats {            /* DmmaOfflineAStats */
     double tile_build_ms;
     double total_ms;
     int rows_were_sorted;     /* 1 if the input rows were already column-sorted */
-    double transpose_src_ms;  /* part of tile_build_ms: the column-major values and the
-                               * per-tile records that the A*A^T B update (identity rows)
-                               * copies from; the reference builds the corresponding
-                               * transpose inside its timed B rebuild */
+    double transpose_src_ms;  /* part of tile_build_ms: the per-tile records that the
+                               * A*A^T B update (identity rows) copies from; the reference
+                               * builds the corresponding transpose inside its timed B
+                               * rebuild */
 } rtt_offline_a_stats;
 
 /* gpu_prepare_identity_a (src/gpu_dmma_tiles.h:2443-2502): upload and
@@ -477,6 +477,14 @@ typedef struct rtt_device_c {
     const double *tile_csr_val;  /* nnz */
 } rtt_device_c;
 int rtt_last_c_view(const rtt_prepared_a *a, rtt_device_c *view);
+
+/* No counterpart in the reference (its calls run on CUDA's default stream):
+ * the HIP stream (a hipStream_t) every call of this library queues its work
+ * on, after rtt_init.  A caller orders its own work after the library's
+ * with it, or captures an allocation-free call (rtt_tile_values_noalloc) on
+ * it into a HIP graph with hipStreamBeginCapture / hipStreamEndCapture and
+ * replays the graph. */
+int rtt_stream(void **stream);
 
 /* Copies device memory to host (test and tool helper). */
 int rtt_memcpy_d2h(void *dst, const void *src, size_t bytes);

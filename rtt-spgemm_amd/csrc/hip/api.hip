@@ -250,6 +250,16 @@ int rtt_sync(void)
     return RTT_OK;
 }
 
+int rtt_stream(void **stream)
+{
+    rtt::clear_error();
+    RTT_TRY(rtt::require_ctx());
+    if (!stream)
+        return fail(RTT_ERR_INVALID, "rtt_stream: NULL argument");
+    *stream = rtt::ctx().stream;
+    return RTT_OK;
+}
+
 int rtt_memcpy_d2h(void *dst, const void *src, size_t bytes)
 {
     RTT_TRY(rtt::require_ctx());
@@ -392,9 +402,9 @@ int rtt_prepare_external_a(const rtt_smatrix *host, const int *row_new_to_old,
     RTT_TRY(rtt::build_tiles(*tiled, nullptr, a->active_rows, a->active_inner, host->m, host->n,
                              false, dense_threshold, false, &a->tiles, &a->build));
     RTT_TRY(rtt::fill_tile_row_of(&a->tiles));
+    RTT_TRY(rtt::build_colmajor_values(&a->tiles));
     RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
     auto t_tr = rtt::Clock::now();
-    RTT_TRY(rtt::build_colmajor_values(&a->tiles));
     RTT_TRY(rtt::build_transpose_source(&a->tiles));
     RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
     st.transpose_src_ms = rtt::ms_since(t_tr);

@@ -244,6 +244,13 @@ __global__ __launch_bounds__(kThreads) void tile_build_kernel(
     }
 }
 
+__global__ void gather3_kernel(const int64_t *a, const int64_t *b, const int64_t *c, int64_t *out)
+{
+    out[0] = *a;
+    out[1] = *b;
+    out[2] = *c;
+}
+
 __global__ void narrow_i64_kernel(const int64_t *in, int *out, int64_t n)
 {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -686,10 +693,13 @@ int run_tile_build(const DevCsr &src, const int *row_map, int active_rows,
     RTT_TRY(exclusive_scan_i64(w->row_tiles.as<int64_t>(), tile_pre, R, &w->scan_tmp, s));
     RTT_TRY(exclusive_scan_i64(w->row_payload.as<int64_t>(), pay_pre, R, &w->scan_tmp, s));
     RTT_TRY(exclusive_scan_i64(w->row_dense.as<int64_t>(), dense_pre, R, &w->scan_tmp, s));
+    // The three totals in one readback.
     int64_t totals[3] = {0, 0, 0};
-    RTT_TRY(read_i64(tile_pre + R, &totals[0], s));
-    RTT_TRY(read_i64(pay_pre + R, &totals[1], s));
-    RTT_TRY(read_i64(dense_pre + R, &totals[2], s));
+    RTT_TRY(w->totals.ensure(3 * sizeof(int64_t)));
+    hipLaunchKernelGGL(gather3_kernel, dim3(1), dim3(1), 0, s, tile_pre + R, pay_pre + R,
+                       dense_pre + R, w->totals.as<int64_t>());
+    RTT_HIP(hipMemcpyAsync(totals, w->totals.ptr, sizeof(totals), hipMemcpyDeviceToHost, s));
+    RTT_HIP(hipStreamSynchronize(s));
     if (totals[0] > INT_MAX || totals[1] > INT_MAX)
         return fail(RTT_ERR_OVERFLOW,
                     "tile count %lld or payload %lld exceeds the int32 tile ABI",

@@ -185,6 +185,7 @@ struct BuildWork {
     DevBuf tcsr_row_ptr, tcsr_col, tcsr_val;   // transposed CSR (reordered AAT)
     DevBuf flag;                                // one int (duplicate detection)
     DevBuf staging;                             // uploaded values before the gather
+    DevBuf totals;                              // tile build: tiles, payload, dense
 };
 
 struct Events {

@@ -1930,6 +1930,24 @@ static int order_rows_by_cost(const int *rows, int n, const Operands &g, SpgemmW
     return RTT_OK;
 }
 
+// Workgroups of a slab pass whose per-workgroup HBM table holds `cols`
+// entries of `entry` bytes: up to four per CU, fewer when the tables would
+// pass a quarter of the free device memory (plus what `table` already holds).
+// The passes take rows from a work queue (or round robin over the grid), so
+// any number of workgroups covers every row; one table that does not fit
+// fails the allocation with RTT_ERR_OOM.
+static int slab_grid(int rows, int num_cus, int64_t cols, size_t entry, const DevBuf &table)
+{
+    int blocks = std::max(1, std::min(rows, 4 * num_cus));
+    size_t free_b = 0, total_b = 0;
+    if (cols > 0 && hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+        const uint64_t per = static_cast<uint64_t>(cols) * entry;
+        const uint64_t budget = free_b / 4 + table.bytes;
+        blocks = static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(blocks, budget / per)));
+    }
+    return blocks;
+}
+
 int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Events *ev,
                 CoreTimes *times, int lb_mode, const TileOpts *tile)
 {
@@ -2069,7 +2087,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
             ss.m = w->slab_stage_m.as<unsigned long long>();
             ss.capacity = w->slab_stage_cap;
         }
-        slab_blocks = std::min(n_slab, 4 * num_cus);
+        slab_blocks = slab_grid(n_slab, num_cus, BC, sizeof(uint64_t), w->slab);
         slab_rows = w->l3_rows.as<int>();
         if (lb_mode == RTT_LB_ADAPTIVE_TAIL)
             RTT_TRY(order_rows_by_cost(w->l3_rows.as<int>(), n_slab, g, w, s, &slab_rows));
@@ -2209,8 +2227,8 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
                            s, g, b.records.as<int4>(), b.num_tiles, w->lds_rows.as<int>(),
                            w->row_level.as<int>(), st, co, Timeline{});
     if (n_num_slab > 0) {
-        const int blocks = std::min(n_num_slab, 4 * num_cus);
         DevBuf &tab = w->slab;   // reused: the symbolic slab pass has been queued before
+        const int blocks = slab_grid(n_num_slab, num_cus, BC, sizeof(int4), tab);
         const int64_t ents = static_cast<int64_t>(blocks) * BC;
         RTT_TRY(tab.ensure(static_cast<size_t>(ents) * sizeof(int4)));
         hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(4 * ents, kThreads)), dim3(kThreads), 0,

@@ -217,6 +217,7 @@ def load() -> C.CDLL:
         "rtt_last_c_row_stats": (C.c_int, [vp, vp, vp]),
         "rtt_last_tail_records": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, P(C.c_int)]),
         "rtt_tile_values_noalloc": (C.c_int, [vp, vp, C.c_int, vp]),
+        "rtt_stream": (C.c_int, [P(vp)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -799,6 +800,14 @@ def tile_values_noalloc(a: PreparedA, b: DynamicB, precision: str, c_values_ptr:
     (nnz doubles the caller owns), allocating nothing; returns after queuing."""
     _check(load().rtt_tile_values_noalloc(a.handle, b.handle, PRECISIONS[precision],
                                           C.c_void_p(int(c_values_ptr))), "tile_values_noalloc")
+
+
+def stream() -> int:
+    """The HIP stream (hipStream_t, as an integer) the library queues its
+    work on (rtt_stream)."""
+    h = C.c_void_p()
+    _check(load().rtt_stream(C.byref(h)), "stream")
+    return h.value or 0
 
 
 def last_tail_records(a: PreparedA, max_records: int = 1 << 20) -> dict:

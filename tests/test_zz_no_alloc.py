@@ -100,3 +100,55 @@ def test_noalloc_refuses_without_a_tile_structure(gpu):
     with pytest.raises(rt.RttError):
         rt.tile_values_noalloc(pa, pb, "fp64", out.ptr.value)
     rt.sync()
+
+
+def test_noalloc_call_replays_from_a_hip_graph(gpu):
+    """The allocation-free call queues kernels only (its scratch is cleared
+    by a kernel, not a memset) and does not synchronise, so it can be
+    captured on the library's stream (rtt_stream) into a HIP graph.  The
+    graph is replayed three times, with the program's own eager work and a
+    synchronisation between replays, and every replay must give the eager
+    call's values bit for bit."""
+    rt = gpu
+    a = rt.gen_fem_blocks(1200, 6, True, 51)
+    pa = rt.prepare_identity_a(a)
+    pb = rt.rebuild_b(pa, 0)
+    rt.tilespgemm(pa, pb, rt.schedule(materialize_output=False, numeric_path="tile"))
+    want = rt.last_c(pa)["tile_csr_val"]
+    n = len(want)
+    assert n > 0
+    out = DeviceBuffer(8 * n)
+    other = DeviceBuffer(8 * n)
+    hip = out.hip
+    vp = C.c_void_p
+    hip.hipStreamBeginCapture.argtypes = [vp, C.c_int]
+    hip.hipStreamEndCapture.argtypes = [vp, C.POINTER(vp)]
+    hip.hipGraphInstantiate.argtypes = [C.POINTER(vp), vp, vp, vp, C.c_size_t]
+    hip.hipGraphLaunch.argtypes = [vp, vp]
+    hip.hipGraphExecDestroy.argtypes = [vp]
+    hip.hipGraphDestroy.argtypes = [vp]
+    stream = vp(rt.stream())
+    graph, exe = vp(), vp()
+    rt.sync()
+    assert hip.hipStreamBeginCapture(stream, 2) == 0   # hipStreamCaptureModeRelaxed
+    try:
+        rt.tile_values_noalloc(pa, pb, "fp64", out.ptr.value)
+    finally:
+        end = hip.hipStreamEndCapture(stream, C.byref(graph))
+    assert end == 0 and graph.value
+    assert hip.hipGraphInstantiate(C.byref(exe), graph, None, None, 0) == 0
+    try:
+        for replay in range(3):
+            # Eager work between the replays: the output poisoned with NaN
+            # bytes, and the same call run eagerly into another buffer.
+            assert out.hip.hipMemset(out.ptr, 0xFF, C.c_size_t(8 * n)) == 0
+            rt.tile_values_noalloc(pa, pb, "fp64", other.ptr.value)
+            rt.sync()
+            assert hip.hipGraphLaunch(exe, stream) == 0
+            rt.sync()
+            got = out.download(n)
+            assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), replay
+            assert np.array_equal(other.download(n).view(np.uint64), want.view(np.uint64))
+    finally:
+        hip.hipGraphExecDestroy(exe)
+        hip.hipGraphDestroy(graph)
