@@ -377,20 +377,27 @@ int64_t orc_spgemm_count(int m_a, int n_a, const int *rp_a, const int *ci_a,
     const int words = n_b / 32 + 1;
 #pragma omp parallel
     {
-        std::vector<uint32_t> flag(words);
+        // Column bitmap of the row, and the words the row set: only those
+        // are counted and cleared, so a row costs its products, not n_b.
+        std::vector<uint32_t> flag(words, 0u);
+        std::vector<int> touched;
 #pragma omp for schedule(dynamic, 64)
         for (int i = 0; i < m_a; ++i) {
-            std::fill(flag.begin(), flag.end(), 0u);
             for (int e = rp_a[i]; e < rp_a[i + 1]; ++e) {
                 const int k = ci_a[e];
                 for (int l = rp_b[k]; l < rp_b[k + 1]; ++l) {
                     const int key = ci_b[l];
+                    if (!flag[key / 32])
+                        touched.push_back(key / 32);
                     flag[key / 32] |= 1u << (key % 32);
                 }
             }
             int64_t cnt = 0;
-            for (int w = 0; w < words; ++w)
+            for (const int w : touched) {
                 cnt += __builtin_popcount(flag[w]);
+                flag[w] = 0u;
+            }
+            touched.clear();
             rp_c[i + 1] = cnt;
         }
     }
@@ -416,13 +423,13 @@ int orc_spgemm_fill(int m_a, int n_a, const int *rp_a, const int *ci_a,
     int status = 0;
 #pragma omp parallel
     {
-        std::vector<uint32_t> flag(words);
+        std::vector<uint32_t> flag(words, 0u);
+        std::vector<int> touched;   // words the row set, emitted in ascending order
         std::vector<double> acc(v_c ? n_b : 0);
         std::vector<double> hi(v_c_hi ? n_b : 0), lo(v_c_hi ? n_b : 0);
         std::vector<double> ab(abs_c ? n_b : 0);
 #pragma omp for schedule(dynamic, 64)
         for (int i = 0; i < m_a; ++i) {
-            std::fill(flag.begin(), flag.end(), 0u);
             for (int e = rp_a[i]; e < rp_a[i + 1]; ++e) {
                 const int k = ci_a[e];
                 const double a = v_a ? v_a[e] : 0.0;
@@ -431,6 +438,8 @@ int orc_spgemm_fill(int m_a, int n_a, const int *rp_a, const int *ci_a,
                     const uint32_t bit = 1u << (j % 32);
                     const double b = v_b ? v_b[l] : 0.0;
                     if (!(flag[j / 32] & bit)) {
+                        if (!flag[j / 32])
+                            touched.push_back(j / 32);
                         flag[j / 32] |= bit;
                         if (v_c)
                             acc[j] = 0.0;
@@ -450,8 +459,10 @@ int orc_spgemm_fill(int m_a, int n_a, const int *rp_a, const int *ci_a,
                 }
             }
             int64_t o = rp_c[i];
-            for (int w = 0; w < words; ++w) {
+            std::sort(touched.begin(), touched.end());
+            for (const int w : touched) {
                 uint32_t f = flag[w];
+                flag[w] = 0u;
                 while (f) {
                     const int bitpos = __builtin_ctz(f);
                     f &= f - 1;
@@ -473,6 +484,7 @@ int orc_spgemm_fill(int m_a, int n_a, const int *rp_a, const int *ci_a,
                     ++o;
                 }
             }
+            touched.clear();
         }
     }
     return status;

@@ -109,6 +109,24 @@ __device__ __forceinline__ double tile_entry(const double *__restrict__ vals, ui
 
 __device__ __forceinline__ int read_lane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
+// tile_entry for a tile that is the same in every lane (the pair loops'
+// operands): its mask and offset move to scalar registers, so the dense test
+// is a scalar branch and a lane's own work is its mask bit and the popcount
+// below it (`below` = (1 << phys) - 1, fixed per lane).
+__device__ __forceinline__ double tile_entry_uniform(const double *__restrict__ vals,
+                                                     uint32_t mask, uint32_t offw, int phys,
+                                                     uint32_t below)
+{
+    mask = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(mask)));
+    offw = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(offw)));
+    const uint32_t off = offw & ~kDense;
+    if (offw & kDense)
+        return vals[off + phys];
+    if (!((mask >> phys) & 1u))
+        return 0.0;
+    return vals[off + __popc(mask & below)];
+}
+
 // Lane roles of v_mfma_f64_4x4x4f64 (four independent 4x4x4 blocks; block b,
 // measured on gfx950): A[b][i][k] in lane i + 4b + 16k, B[b][k][j] in lane
 // j + 4b + 16k, D[b][i][j] in lane j + 4b + 16i.  Block b = 2h + v multiplies
@@ -118,12 +136,15 @@ struct LaneRoles {
     int a_phys;   // A tile bit r*4+k this lane supplies
     int b_phys;   // B tile bit c*4+k
     int c_bit;    // C tile bit r*8+c this lane accumulates
+    uint32_t a_below, b_below;   // (1 << phys) - 1
     __device__ LaneRoles(int lane)
     {
         const int blk = (lane >> 2) & 3, lo = lane & 3, hi = lane >> 4;
         a_phys = ((blk >> 1) * 4 + lo) * 4 + hi;
         b_phys = ((blk & 1) * 4 + lo) * 4 + hi;
         c_bit = ((blk >> 1) * 4 + hi) * 8 + (blk & 1) * 4 + lo;
+        a_below = (1u << a_phys) - 1u;
+        b_below = (1u << b_phys) - 1u;
     }
 };
 
@@ -179,8 +200,9 @@ struct Fp64Product {
             bv[i] = 0.0;
             if (p0 + i < n) {
                 const int4 m = pm[p0 + i];
-                bv[i] = tile_entry(g.b_vals, static_cast<uint32_t>(m.z), static_cast<uint32_t>(m.w),
-                                   ln.roles.b_phys);
+                bv[i] = tile_entry_uniform(g.b_vals, static_cast<uint32_t>(m.z),
+                                           static_cast<uint32_t>(m.w), ln.roles.b_phys,
+                                           ln.roles.b_below);
             }
         }
     }
@@ -205,14 +227,15 @@ struct Fp64Product {
                 av[i] = 0.0;
                 if (i < nbat) {
                     const int4 m = pm[p0 + i];
-                    av[i] = tile_entry(a_vals, static_cast<uint32_t>(m.x),
-                                       static_cast<uint32_t>(m.y), ln.roles.a_phys);
+                    av[i] = tile_entry_uniform(a_vals, static_cast<uint32_t>(m.x),
+                                               static_cast<uint32_t>(m.y), ln.roles.a_phys,
+                                               ln.roles.a_below);
                 }
             }
 #pragma unroll
             for (int i = 0; i < kBatch; ++i) {
                 if (i < nbat) {
-                    const int t = pc[p0 + i];
+                    const int t = __builtin_amdgcn_readfirstlane(pc[p0 + i]);
                     if (t != cur) {
                         if (cur >= 0)
                             store(g, ln, cur, cm, co, acc);
