@@ -109,6 +109,25 @@ __device__ __forceinline__ double tile_entry(const double *__restrict__ vals, ui
 
 __device__ __forceinline__ int read_lane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
+// Entry `phys` of a tile that is the same in every lane, split so that the
+// load is unconditional: its mask and offset move to scalar registers, the
+// load reads a slot inside the tile (the tile's first slot when the entry is
+// absent), and *has says whether the value counts.  The caller applies `has`
+// where it uses the value, so that the wait for the load lands there and not
+// right after the load.
+__device__ __forceinline__ double tile_entry_raw(const double *vals, uint32_t mask, uint32_t offw,
+                                                 int phys, uint32_t below, bool *has)
+{
+    mask = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(mask)));
+    offw = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(offw)));
+    const uint32_t off = offw & ~kDense;
+    const bool dense = (offw & kDense) != 0;
+    const bool bit = ((mask >> phys) & 1u) != 0;
+    *has = dense || bit;
+    const uint32_t idx = dense ? static_cast<uint32_t>(phys) : (bit ? __popc(mask & below) : 0u);
+    return vals[off + idx];
+}
+
 // tile_entry for a tile that is the same in every lane (the pair loops'
 // operands): its mask and offset move to scalar registers, so the dense test
 // is a scalar branch and a lane's own work is its mask bit and the popcount
@@ -191,20 +210,22 @@ struct Fp64Product {
         slot64[ln.roles.c_bit] = acc.v;
     }
 
-    // B entries of buffered pairs [p0, p0 + kBatch) (zero past n).
+    // B entries of buffered pairs [p0, p0 + kBatch): one unconditional load
+    // each (pairs past n repeat pair n - 1; their values are not used); bit
+    // i of *has: the entry counts.
     __device__ static void load_b(const TileTask &g, const Lane &ln, const int4 *pm, int p0, int n,
-                                  double (&bv)[kBatch])
+                                  double (&bv)[kBatch], uint32_t *has)
     {
+        uint32_t h = 0;
 #pragma unroll
         for (int i = 0; i < kBatch; ++i) {
-            bv[i] = 0.0;
-            if (p0 + i < n) {
-                const int4 m = pm[p0 + i];
-                bv[i] = tile_entry_uniform(g.b_vals, static_cast<uint32_t>(m.z),
-                                           static_cast<uint32_t>(m.w), ln.roles.b_phys,
-                                           ln.roles.b_below);
-            }
+            const int4 m = pm[min(p0 + i, n - 1)];
+            bool hi;
+            bv[i] = tile_entry_raw(g.b_vals, static_cast<uint32_t>(m.z), static_cast<uint32_t>(m.w),
+                                   ln.roles.b_phys, ln.roles.b_below, &hi);
+            h |= hi ? (1u << i) : 0u;
         }
+        *has = h;
     }
 
     // Runs buffered pairs [0, n) in order; a change of C tile stores the
@@ -215,22 +236,25 @@ struct Fp64Product {
                                const int4 *pm, const unsigned char *pc, int n, int &cur, Acc &acc,
                                unsigned long long cm, long long co)
     {
+        if (n <= 0)
+            return;
         double bv[kBatch];
-        load_b(g, ln, pm, 0, n, bv);
+        uint32_t bh = 0;
+        load_b(g, ln, pm, 0, n, bv, &bh);
         for (int p0 = 0; p0 < n; p0 += kBatch) {
             double bn[kBatch];
-            load_b(g, ln, pm, p0 + kBatch, n, bn);
+            uint32_t bnh = 0;
+            load_b(g, ln, pm, p0 + kBatch, n, bn, &bnh);
             const int nbat = min(kBatch, n - p0);
             double av[kBatch];
+            uint32_t ah = 0;
 #pragma unroll
             for (int i = 0; i < kBatch; ++i) {
-                av[i] = 0.0;
-                if (i < nbat) {
-                    const int4 m = pm[p0 + i];
-                    av[i] = tile_entry_uniform(a_vals, static_cast<uint32_t>(m.x),
-                                               static_cast<uint32_t>(m.y), ln.roles.a_phys,
-                                               ln.roles.a_below);
-                }
+                const int4 m = pm[min(p0 + i, n - 1)];
+                bool hi;
+                av[i] = tile_entry_raw(a_vals, static_cast<uint32_t>(m.x), static_cast<uint32_t>(m.y),
+                                       ln.roles.a_phys, ln.roles.a_below, &hi);
+                ah |= hi ? (1u << i) : 0u;
             }
 #pragma unroll
             for (int i = 0; i < kBatch; ++i) {
@@ -242,12 +266,15 @@ struct Fp64Product {
                         cur = t;
                         acc.v = 0.0;
                     }
-                    acc.v = __builtin_amdgcn_mfma_f64_4x4x4f64(av[i], bv[i], acc.v, 0, 0, 0);
+                    const double a = ((ah >> i) & 1u) ? av[i] : 0.0;
+                    const double b = ((bh >> i) & 1u) ? bv[i] : 0.0;
+                    acc.v = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, acc.v, 0, 0, 0);
                 }
             }
 #pragma unroll
             for (int i = 0; i < kBatch; ++i)
                 bv[i] = bn[i];
+            bh = bnh;
         }
     }
 };
