@@ -359,6 +359,53 @@ def cpu_baseline(a, aat, nnz_cub, budget_s=2.5, repeats=5):
                       f"threads"}
 
 
+def c1_line(rt, steps=20, warmup=5, repeats=5):
+    """The reference's CPU-runnable configuration: C = A*A on the 64^2
+    5-point Laplacian (4096 rows, nnzCub 100,104).  The GPU step (B update +
+    symbolic + allocate + numeric, launch-bound at this size) beside the
+    cpu_baseline legs the plan names for it: the legacy 16x16 tile CPU
+    SpGEMM and the row SPA, both restated in the oracle, median of
+    `repeats` runs each."""
+    import oracle
+    a = rt.gen_laplace2d(64, 0.1, 20260101)
+    pa = rt.prepare_identity_a(a)
+    nnz_cub = rt.nnz_cub(pa, 0)
+    sched = rt.schedule(materialize_output=False)
+    state = {}
+
+    def step():
+        state["pb"] = rt.rebuild_b(pa, 0, state.get("pb"))
+        state["st"] = rt.tilespgemm(pa, state["pb"], sched)[1]
+
+    ms = timed_steps(step, steps, warmup)
+    A = (a.m, a.n, a.rowptr, a.col, a.val)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+
+    def med(f):
+        ts = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            f()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[len(ts) // 2]
+
+    t_legacy = med(lambda: oracle.legacy_tile_spgemm(A, A))
+    t_spa = med(lambda: oracle.spgemm(A, A, with_dd=False))
+    gf = lambda t: round(2.0 * nnz_cub / t / 1e9, 4)
+    return {"workload": "laplace2d-64-aa: C = A*A, 2D 5-point Laplacian 64^2, off-diagonals "
+                        f"-1+U[-0.1,0.1], seed 20260101, {a.m} rows, {a.nnz} nnz, nnzCub "
+                        f"{nnz_cub}, nnzC {state['st']['output_nnz']}",
+            "gpu": {"ms_per_step": round(ms, 4), "value": gf(ms * 1e-3), "unit": "GFLOP/s",
+                    "steps": steps, "warmup": warmup},
+            "cpu_legacy_tile16": {"value": gf(t_legacy), "unit": "GFLOP/s", "ms": round(t_legacy * 1e3, 3),
+                                  "cores": threads, "kind": "port",
+                                  "sample": f"whole product, median of {repeats} runs (the "
+                                            "legacy 16x16 tile CPU algorithm, oracle)"},
+            "cpu_spa": {"value": gf(t_spa), "unit": "GFLOP/s", "ms": round(t_spa * 1e3, 3),
+                        "cores": threads, "kind": "port",
+                        "sample": f"whole product, median of {repeats} runs (row SPA, oracle)"}}
+
+
 # Kernel names in the rocprofv3 counter summaries, per breakdown key.
 PROFILE_KERNELS = {
     "numeric": "numeric_row_kernel<SmallRow",
@@ -581,6 +628,8 @@ def main():
                     help="plain FP64 vs split-BF16 on --workload (block FEM), with errors")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the split-precision and batched forms reported beside the line")
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the 64^2 Laplacian line (GPU step and its CPU baselines)")
     ap.add_argument("--no-second-mode", action="store_true",
                     help="skip the R-MAT 2^20 C = A*A measurement reported beside the main line")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "split-bf16"],
@@ -702,6 +751,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(a, aat, nnz_cub)
+    c1 = None if args.no_cpu_baseline or args.no_c1 else c1_line(rt)
     line = {
         "metric": METRIC,
         "value": round(gflops, 3),
@@ -735,6 +785,7 @@ def main():
                      "mfma": mfma},
         "cpu_baseline": cpu,
         "offline_transpose": offline_tr,
+        "c1": c1,
         "control": control,
         "second_mode": other,
         "forms": forms,

@@ -46,6 +46,9 @@ def lib() -> C.CDLL:
         _lib.orc_transpose.argtypes = [i32, i32, vp, vp, vp, vp, vp, vp]
         _lib.orc_spgemm_count.argtypes = [i32, i32, vp, vp, i32, i32, vp, vp, vp]
         _lib.orc_spgemm_count.restype = i64
+        _lib.orc_legacy_tile_spgemm.argtypes = [i32, i32, vp, vp, vp, i32, i32, vp, vp, vp, vp,
+                                                vp, vp]
+        _lib.orc_legacy_tile_spgemm.restype = i64
         _lib.orc_spgemm_fill.argtypes = [i32, i32, vp, vp, vp, i32, i32, vp, vp, vp, vp, vp,
                                          vp, vp, vp, vp]
         _lib.orc_hybrid_tiles_count.argtypes = [i32, i32, vp, vp, i32, i32, i32, i32, vp]
@@ -177,6 +180,26 @@ def spgemm_row_counts(a, b) -> np.ndarray:
     if lib().orc_spgemm_count(ma, na, _p(rpa), _p(cia), mb, nb, _p(rpb), _p(cib), _p(rpc)) < 0:
         raise ValueError("shape mismatch")
     return rpc
+
+
+def legacy_tile_spgemm(a, b):
+    """C = A*B by the reference's legacy 16x16 tile CPU algorithm
+    (orc_legacy_tile_spgemm, src/spgemm-cpu.h:3-455), as CSR:
+    (rowptr int64, col int32, val float64).  a, b = (m, n, rowptr, col, val)."""
+    ma, na, rpa, cia, va = a
+    mb, nb, rpb, cib, vb = b
+    rpa, cia, va, rpb, cib, vb = _i32(rpa), _i32(cia), _f64(va), _i32(rpb), _i32(cib), _f64(vb)
+    rpc = np.zeros(ma + 1, np.int64)
+    f = lib().orc_legacy_tile_spgemm
+    nnz = f(ma, na, _p(rpa), _p(cia), _p(va), mb, nb, _p(rpb), _p(cib), _p(vb), _p(rpc), None,
+            None)
+    if nnz < 0:
+        raise ValueError("shape mismatch")
+    cic = np.zeros(max(nnz, 1), np.int32)
+    vc = np.zeros(max(nnz, 1), np.float64)
+    f(ma, na, _p(rpa), _p(cia), _p(va), mb, nb, _p(rpb), _p(cib), _p(vb), _p(rpc), _p(cic),
+      _p(vc))
+    return rpc, cic[:nnz], vc[:nnz]
 
 
 def spgemm(a, b, with_dd: bool = True):

@@ -867,3 +867,199 @@ int orc_plan_tail_queue(int n, const int *scans, const int *matches,
 }
 
 } // extern "C"
+
+/* ------------------------------------------------------------------------ */
+/* Legacy 16x16 tile SpGEMM on the CPU (src/spgemm-cpu.h:3-455)             */
+/* ------------------------------------------------------------------------ */
+
+namespace {
+
+constexpr int kLegacyTile = 16;   /* BLOCK_SIZE, src/common.h:31-32 */
+
+/* A matrix in 16x16 tiles, each tile a small CSR: tile rows of tiles (tile
+ * CSR), per tile its first entry, per tile and local row the entry offset
+ * (u8), per entry its local column (u8) and value.  With by_column the tiles
+ * are also listed per tile column (tile CSC: ids in increasing tile row). */
+struct Tiles16 {
+    int tm = 0, tn = 0;
+    std::vector<int> row_ptr, col_idx;        /* tile CSR */
+    std::vector<int> col_ptr, csc_row, csc_id; /* tile CSC */
+    std::vector<int> first;                    /* first entry of each tile (+1) */
+    std::vector<unsigned char> lrow;           /* 16 per tile: local row offsets */
+    std::vector<unsigned char> lcol;
+    std::vector<double> val;
+};
+
+/* csr2tile for one operand (src/csr2tile.h semantics: entries of a tile in
+ * row order, columns ascending; duplicates summed in CSR order). */
+Tiles16 make_tiles16(int m, int n, const int *rp, const int *ci, const double *v, bool by_column)
+{
+    Tiles16 t;
+    t.tm = (m + kLegacyTile - 1) / kLegacyTile;
+    t.tn = (n + kLegacyTile - 1) / kLegacyTile;
+    t.row_ptr.assign(t.tm + 1, 0);
+    t.first.push_back(0);
+    for (int tr = 0; tr < t.tm; ++tr) {
+        const int r0 = tr * kLegacyTile, r1 = std::min(m, r0 + kLegacyTile);
+        /* (tile column, local row, local column, value) of the tile row */
+        struct E { int tc, r, c; double v; };
+        std::vector<E> es;
+        for (int r = r0; r < r1; ++r)
+            for (int e = rp[r]; e < rp[r + 1]; ++e)
+                es.push_back({ci[e] / kLegacyTile, r - r0, ci[e] % kLegacyTile, v ? v[e] : 0.0});
+        std::stable_sort(es.begin(), es.end(), [](const E &a, const E &b) {
+            if (a.tc != b.tc) return a.tc < b.tc;
+            if (a.r != b.r) return a.r < b.r;
+            return a.c < b.c;
+        });
+        size_t i = 0;
+        while (i < es.size()) {
+            const int tc = es[i].tc;
+            t.col_idx.push_back(tc);
+            unsigned char cnt[kLegacyTile] = {};
+            const size_t base = t.lcol.size();
+            while (i < es.size() && es[i].tc == tc) {
+                const int r = es[i].r, c = es[i].c;
+                double sum = 0.0;
+                do {
+                    sum += es[i].v;
+                    ++i;
+                } while (i < es.size() && es[i].tc == tc && es[i].r == r && es[i].c == c);
+                t.lcol.push_back(static_cast<unsigned char>(c));
+                t.val.push_back(sum);
+                ++cnt[r];
+            }
+            unsigned char off = 0;
+            for (int r = 0; r < kLegacyTile; ++r) {
+                t.lrow.push_back(off);
+                off = static_cast<unsigned char>(off + cnt[r]);
+            }
+            t.first.push_back(static_cast<int>(t.lcol.size()));
+            (void)base;
+        }
+        t.row_ptr[tr + 1] = static_cast<int>(t.col_idx.size());
+    }
+    if (by_column) {
+        const int nt = static_cast<int>(t.col_idx.size());
+        t.col_ptr.assign(t.tn + 1, 0);
+        for (int id = 0; id < nt; ++id)
+            ++t.col_ptr[t.col_idx[id] + 1];
+        for (int c = 0; c < t.tn; ++c)
+            t.col_ptr[c + 1] += t.col_ptr[c];
+        t.csc_row.resize(nt);
+        t.csc_id.resize(nt);
+        std::vector<int> fill(t.col_ptr.begin(), t.col_ptr.end() - 1);
+        for (int tr = 0; tr < t.tm; ++tr)
+            for (int id = t.row_ptr[tr]; id < t.row_ptr[tr + 1]; ++id) {
+                const int at = fill[t.col_idx[id]]++;
+                t.csc_row[at] = tr;
+                t.csc_id[at] = id;
+            }
+    }
+    return t;
+}
+
+/* Entries [lo, hi) of local row r of tile id. */
+inline void tile_row_range(const Tiles16 &t, int id, int r, int *lo, int *hi)
+{
+    const int base = t.first[id];
+    *lo = base + t.lrow[static_cast<size_t>(id) * kLegacyTile + r];
+    *hi = r + 1 < kLegacyTile ? base + t.lrow[static_cast<size_t>(id) * kLegacyTile + r + 1]
+                              : t.first[id + 1];
+}
+
+} // namespace
+
+extern "C" {
+
+/* The legacy tile SpGEMM (spgemm_cpu, src/spgemm-cpu.h:369-455): C's tiles
+ * per tile row are the B tile columns whose tile K list meets A's tile row
+ * (step1/step2, a merge of the two sorted K lists per candidate column,
+ * src/spgemm-cpu.h:3-92); each C tile's entries come from a 16x16 flag grid
+ * (step3, :186-263) and its values from a 16x16 dense accumulator (step4,
+ * :266-364), the matched pairs taken in increasing K and each pair's products
+ * in A-row then B-row order.  C is returned as CSR (rows ascending, columns
+ * ascending: tiles of a tile row in column order).  With ci_c == NULL only
+ * rp_c is filled; the return value is nnz(C), or -1 on a shape mismatch. */
+int64_t orc_legacy_tile_spgemm(int m_a, int n_a, const int *rp_a, const int *ci_a,
+                               const double *v_a, int m_b, int n_b, const int *rp_b,
+                               const int *ci_b, const double *v_b, int64_t *rp_c, int *ci_c,
+                               double *v_c)
+{
+    if (n_a != m_b)
+        return -1;
+    const Tiles16 A = make_tiles16(m_a, n_a, rp_a, ci_a, v_a, false);
+    const Tiles16 B = make_tiles16(m_b, n_b, rp_b, ci_b, v_b, true);
+    const bool fill = ci_c != nullptr;
+    /* Per C row: its entries (column, value), assembled tile by tile. */
+    std::vector<std::vector<std::pair<int, double>>> rows(static_cast<size_t>(m_a));
+#pragma omp parallel
+    {
+        double acc[kLegacyTile * kLegacyTile];
+        unsigned char flag[kLegacyTile * kLegacyTile];
+#pragma omp for schedule(dynamic, 4)
+        for (int ti = 0; ti < A.tm; ++ti) {
+            const int ra0 = A.row_ptr[ti], ra1 = A.row_ptr[ti + 1];
+            const int rowlen = std::min(kLegacyTile, m_a - ti * kLegacyTile);
+            for (int tj = 0; tj < B.tn; ++tj) {
+                int pa = ra0, pb = B.col_ptr[tj];
+                const int pb1 = B.col_ptr[tj + 1];
+                bool any = false;
+                while (pa < ra1 && pb < pb1) {
+                    const int ka = A.col_idx[pa], kb = B.csc_row[pb];
+                    if (ka < kb) {
+                        ++pa;
+                    } else if (ka > kb) {
+                        ++pb;
+                    } else {
+                        if (!any) {
+                            std::fill(flag, flag + kLegacyTile * kLegacyTile, 0);
+                            std::fill(acc, acc + kLegacyTile * kLegacyTile, 0.0);
+                        }
+                        any = true;
+                        const int ida = pa, idb = B.csc_id[pb];
+                        for (int r = 0; r < rowlen; ++r) {
+                            int a0, a1;
+                            tile_row_range(A, ida, r, &a0, &a1);
+                            for (int i = a0; i < a1; ++i) {
+                                const int ka_local = A.lcol[i];
+                                int b0, b1;
+                                tile_row_range(B, idb, ka_local, &b0, &b1);
+                                for (int j = b0; j < b1; ++j) {
+                                    const int c = B.lcol[j];
+                                    acc[r * kLegacyTile + c] += A.val[i] * B.val[j];
+                                    flag[r * kLegacyTile + c] = 1;
+                                }
+                            }
+                        }
+                        ++pa;
+                        ++pb;
+                    }
+                }
+                if (!any)
+                    continue;
+                for (int r = 0; r < rowlen; ++r)
+                    for (int c = 0; c < kLegacyTile; ++c)
+                        if (flag[r * kLegacyTile + c])
+                            rows[static_cast<size_t>(ti) * kLegacyTile + r].push_back(
+                                {tj * kLegacyTile + c, acc[r * kLegacyTile + c]});
+            }
+        }
+    }
+    rp_c[0] = 0;
+    for (int i = 0; i < m_a; ++i)
+        rp_c[i + 1] = rp_c[i] + static_cast<int64_t>(rows[i].size());
+    if (fill)
+        for (int i = 0; i < m_a; ++i) {
+            int64_t o = rp_c[i];
+            for (const auto &e : rows[i]) {
+                ci_c[o] = e.first;
+                if (v_c)
+                    v_c[o] = e.second;
+                ++o;
+            }
+        }
+    return rp_c[m_a];
+}
+
+} // extern "C"

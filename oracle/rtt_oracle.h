@@ -55,6 +55,13 @@ int orc_spgemm_fill(int m_a, int n_a, const int *rp_a, const int *ci_a,
                     int *ci_c, double *v_c, double *v_c_hi, double *v_c_lo,
                     double *abs_c);
 
+/* ---- Legacy 16x16 tile SpGEMM (src/spgemm-cpu.h:3-455) -------------------
+ * C as CSR; ci_c == NULL fills rp_c only.  Returns nnz(C) or -1.          */
+int64_t orc_legacy_tile_spgemm(int m_a, int n_a, const int *rp_a, const int *ci_a,
+                               const double *v_a, int m_b, int n_b, const int *rp_b,
+                               const int *ci_b, const double *v_b, int64_t *rp_c, int *ci_c,
+                               double *v_c);
+
 /* ---- Hybrid tile format (src/dmma_tiles.h:163-440) ----------------------
  * counts[0]=num_tiles counts[1]=payload counts[2]=dense counts[3]=sparse
  * Returns 0 on success, -1 on invalid input (validate_csr semantics).     */

@@ -347,3 +347,76 @@ def test_tail_plan_matches_reference_vectors():
             assert got[k] == p[k], (p["settings"], k)
         assert got["heavy_ids"].tolist() == p["heavy_ids"]
         assert got["heavy_chunks"].tolist() == p["heavy_chunks"]
+
+
+# ---- legacy 16x16 tile CPU SpGEMM -------------------------------------------------
+
+def _laplace(g, seed):
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    n = g * g
+    rows, cols, vals = [], [], []
+    for i in range(g):
+        for j in range(g):
+            r = i * g + j
+            rows.append(r); cols.append(r); vals.append(4.0)
+            for di, dj in ((-1, 0), (1, 0), (0, -1), (0, 1)):
+                ii, jj = i + di, j + dj
+                if 0 <= ii < g and 0 <= jj < g:
+                    rows.append(r); cols.append(ii * g + jj); vals.append(-1.0 + rng.uniform(-0.1, 0.1))
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    A.sort_indices()
+    return A
+
+
+def _tup(A):
+    return (A.shape[0], A.shape[1], A.indptr.astype(np.int32), A.indices.astype(np.int32),
+            A.data.astype(np.float64))
+
+
+@pytest.mark.parametrize("case", ["laplace64_aa", "random_rect", "banded_aat", "partial_tiles"])
+def test_legacy_tile_spgemm_equals_the_spa(case):
+    """The legacy 16x16 tile algorithm restated from src/spgemm-cpu.h takes
+    each C entry's products in increasing k, like the row SPA, so on inputs
+    without duplicates both give the same C bit for bit; the 64^2 Laplacian
+    A^2 (the C1 configuration) has 51,972 entries (closed form)."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(3)
+    if case == "laplace64_aa":
+        A = _laplace(64, 20260101); B = A
+    elif case == "random_rect":
+        A = sp.random(150, 90, density=0.06, random_state=4, format="csr")
+        B = sp.random(90, 200, density=0.05, random_state=5, format="csr")
+    elif case == "banded_aat":
+        A = sp.diags([rng.uniform(-1, 1, 300 - abs(o)) for o in range(-7, 8)], list(range(-7, 8)),
+                     format="csr")
+        B = A.T.tocsr()
+    else:
+        A = sp.random(37, 45, density=0.2, random_state=6, format="csr")
+        B = sp.random(45, 23, density=0.2, random_state=7, format="csr")
+    A.sort_indices(); B.sort_indices()
+    rp, col, val = oracle.legacy_tile_spgemm(_tup(A), _tup(B))
+    ref = oracle.spgemm(_tup(A), _tup(B), with_dd=False)
+    assert np.array_equal(rp, ref["rowptr"])
+    assert np.array_equal(col, ref["col"])
+    assert np.array_equal(val.view(np.uint64), ref["val"].view(np.uint64))
+    if case == "laplace64_aa":
+        assert rp[-1] == 51972
+
+
+def test_legacy_tile_spgemm_sums_duplicates_in_the_tile_build():
+    """Duplicates are summed when the tiles are built, so with duplicates the
+    legacy C agrees with the SPA's within rounding, structure exact."""
+    import scipy.sparse as sp
+    A = sp.random(60, 60, density=0.1, random_state=8, format="coo")
+    r = np.concatenate([A.row, A.row[:20]]); c = np.concatenate([A.col, A.col[:20]])
+    v = np.concatenate([A.data, np.linspace(-1, 1, 20)])
+    order = np.lexsort((c, r))
+    rp = np.zeros(61, np.int32)
+    np.add.at(rp, r + 1, 1)
+    a = (60, 60, np.cumsum(rp).astype(np.int32), c[order].astype(np.int32), v[order])
+    lrp, lcol, lval = oracle.legacy_tile_spgemm(a, a)
+    ref = oracle.spgemm(a, a)
+    assert np.array_equal(lrp, ref["rowptr"]) and np.array_equal(lcol, ref["col"])
+    err = oracle.value_errors(lval, ref["hi"], ref["lo"], ref["absprod"])
+    assert err["fwd_max"] <= 1e-14
