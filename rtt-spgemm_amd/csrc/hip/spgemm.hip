@@ -61,20 +61,45 @@ constexpr int kWChunk = kWave;     // A tiles per one-wave enumeration step
 constexpr int kWSeg = 512;         // pairs per one-wave segment window
 constexpr int kWBatch = 6;         // pairs whose loads a lane of that kernel issues together
 
-constexpr int kNumChunk = 64;      // A tiles per numeric chunk
 constexpr int kColRegs = 3;         // values of an A column a numeric lane holds in registers
 constexpr int kExtVals = 3;         // B values past the record a numeric lane prefetches
-static_assert(kThreads == 4 * kNumChunk, "one numeric thread per (pair run, chunk tile)");
 
-// LDS budgets of the two numeric instances: accumulator slots, staged A
-// values, C tiles, hash slots (at least twice the tiles).
+// The numeric instances: threads per workgroup (four pair runs of kChunk A
+// tiles each), pairs per segment-table window, and LDS budgets:
+// accumulator slots, staged A values, C tiles, hash slots (at least twice
+// the tiles).  TinyRow is one wave with 3 KB of LDS against SmallRow's
+// 20 KB, so that a CU holds four times as many of the short rows of sparse
+// operands (prolongators, Galerkin products) in flight.  A second one-wave
+// instance with a 512-value accumulator measured no faster than SmallRow
+// on the rows between the two.
+struct TinyRow {
+    static constexpr int kThreads = kWave, kChunk = kWave / 4, kSeg = 128;
+    static constexpr bool kLeavesTiny = false;
+    static constexpr int kAcc = 128, kAStage = 64, kTiles = 16, kSlots = 32, kSlotBits = 5;
+};
+// SmallRow, launched over every row, leaves the rows within TinyRow's
+// budgets to that instance (launched over their list).
 struct SmallRow {
+    static constexpr int kThreads = 256, kChunk = 64, kSeg = 1024;
+    static constexpr bool kLeavesTiny = true;
     static constexpr int kAcc = 1024, kAStage = 256, kTiles = 128, kSlots = 256, kSlotBits = 8;
 };
 struct LargeRow {
+    static constexpr int kThreads = 256, kChunk = 64, kSeg = 1024;
+    static constexpr bool kLeavesTiny = false;
     static constexpr int kAcc = 4096, kAStage = 2048, kTiles = 512, kSlots = 1024, kSlotBits = 10;
 };
 static_assert(LargeRow::kAcc <= 65536 && LargeRow::kTiles <= 32768, "packed tile slots");
+static_assert(TinyRow::kAcc <= SmallRow::kAcc && TinyRow::kAStage <= SmallRow::kAStage &&
+                  TinyRow::kTiles <= SmallRow::kTiles,
+              "nested row budgets");
+
+// Whether a row of the LDS level with these sizes fits the instance Cfg.
+template <class Cfg>
+__device__ __forceinline__ bool fits_row(int64_t nt, int64_t nnz, int64_t a_len)
+{
+    return nt <= Cfg::kTiles && nnz <= Cfg::kAcc && a_len <= Cfg::kAStage;
+}
 
 constexpr int kWriteBuf = 2048;    // column bytes staged per row in the write pass
 
@@ -184,8 +209,9 @@ __device__ __forceinline__ void write_tile_structure(const COut &c, int64_t t, i
     }
 }
 
-// Exclusive scans of two values over the workgroup with one barrier; s2
-// (2 * kWavesPerBlock ints) must not be reused without a barrier between.
+// Exclusive scans of two values over a workgroup of NW waves with one
+// barrier; s2 (2 * NW ints) must not be reused without a barrier between.
+template <int NW = kWavesPerBlock>
 __device__ __forceinline__ void dual_exclusive_scan(int a, int b, int *s2, int *pa, int *pb,
                                                     int *ta, int *tb)
 {
@@ -194,13 +220,13 @@ __device__ __forceinline__ void dual_exclusive_scan(int a, int b, int *s2, int *
     const int ib = wave_inclusive_scan(b);
     if (lane_id() == kWave - 1) {
         s2[w] = ia;
-        s2[kWavesPerBlock + w] = ib;
+        s2[NW + w] = ib;
     }
     __syncthreads();
     int ba = 0, bb = 0, sa = 0, sb = 0;
 #pragma unroll
-    for (int i = 0; i < kWavesPerBlock; ++i) {
-        const int x = s2[i], y = s2[kWavesPerBlock + i];
+    for (int i = 0; i < NW; ++i) {
+        const int x = s2[i], y = s2[NW + i];
         ba += i < w ? x : 0;
         bb += i < w ? y : 0;
         sa += x;
@@ -212,7 +238,8 @@ __device__ __forceinline__ void dual_exclusive_scan(int a, int b, int *s2, int *
     *tb = sb;
 }
 
-// Exclusive scan over the workgroup; *total receives the sum.
+// Exclusive scan over a workgroup of NW waves; *total receives the sum.
+template <int NW = kWavesPerBlock>
 __device__ __forceinline__ int block_exclusive_scan(int v, int *s_tmp, int *total)
 {
     const int w = threadIdx.x / kWave;
@@ -222,7 +249,7 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *s_tmp, int *tota
     __syncthreads();
     int before = 0, all = 0;
 #pragma unroll
-    for (int i = 0; i < kWavesPerBlock; ++i) {
+    for (int i = 0; i < NW; ++i) {
         const int t = s_tmp[i];
         before += i < w ? t : 0;
         all += t;
@@ -1132,10 +1159,12 @@ __device__ __forceinline__ void trace_mark(const Timeline &tl, bool at_end)
 }
 
 template <class Cfg, bool TRACE = false>
-__global__ __launch_bounds__(kThreads) void numeric_row_kernel(
+__global__ __launch_bounds__(Cfg::kThreads) void numeric_row_kernel(
     Operands g, const int4 *__restrict__ recs, int64_t rec_n, const int *__restrict__ rows,
     const int *__restrict__ row_level, Stage st, COut c, Timeline tl)
 {
+    constexpr int kT = Cfg::kThreads, kCh = Cfg::kChunk, kSg = Cfg::kSeg, kNW = kT / kWave;
+    static_assert(kT == 4 * kCh && kT % kWave == 0, "one thread per (pair run, chunk tile)");
     __shared__ double s_acc[Cfg::kAcc];
     __shared__ double s_acol[Cfg::kAStage + kColRegs];
     __shared__ unsigned long long s_cmask[Cfg::kTiles];   // C tile masks, bit r * 8 + cl
@@ -1143,17 +1172,20 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     __shared__ int4 s_slot[Cfg::kSlots];   // hash of the row's C tiles (make_tile_slot)
     // Per chunk A tile: its staged info and B runs (see stage_tile), and per
     // (run, tile) the first B record minus the run's first pair.
-    __shared__ int4 s_at[kNumChunk];
-    __shared__ int4 s_br[kNumChunk];
-    __shared__ int s_pw[kThreads];
-    __shared__ unsigned char s_seg[kSeg];
-    __shared__ int s_tmp[kWavesPerBlock];
-    __shared__ int s_dual[2 * kWavesPerBlock];
+    __shared__ int4 s_at[kCh];
+    __shared__ int4 s_br[kCh];
+    __shared__ int s_pw[kT];
+    __shared__ unsigned char s_seg[kSg];
+    __shared__ int s_tmp[kNW];
+    __shared__ int s_dual[2 * kNW];
 
     const int tid = threadIdx.x;
     if constexpr (TRACE)
         trace_mark(tl, false);
-    const int row = rows ? rows[blockIdx.x] : xcd_block(blockIdx.x, gridDim.x);
+    // A listed row is read through the same XCD-contiguous mapping, so that
+    // neighbouring list entries (neighbouring rows) meet in one L2.
+    const int bx = xcd_block(blockIdx.x, gridDim.x);
+    const int row = rows ? rows[bx] : bx;
     // The row's inputs come in two rounds of independent loads: its level,
     // C's row extents, A's tile and payload extents (the payload base per
     // tile row is precomputed); then the first staged C tile, A value and A
@@ -1166,8 +1198,11 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
     const int a_base = g.a_row_off[row];
     const int a_len = g.a_row_off[row + 1] - a_base;
-    if ((level != kLevelLds) | (nt > Cfg::kTiles) | (nnz_row > Cfg::kAcc) |
-        (a_len > Cfg::kAStage) | tile_routed(g.tile_fill, a1 - a0, a_len)) {
+    bool skip = (level != kLevelLds) | (nt > Cfg::kTiles) | (nnz_row > Cfg::kAcc) |
+                (a_len > Cfg::kAStage) | tile_routed(g.tile_fill, a1 - a0, a_len);
+    if constexpr (Cfg::kLeavesTiny)
+        skip |= fits_row<TinyRow>(nt, nnz_row, a_len);
+    if (skip) {
         if constexpr (TRACE)
             trace_mark(tl, true);
         return;
@@ -1184,18 +1219,18 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
         av0 = g.a_vals_cm[a_base + tid];
     int4 am0 = make_int4(0, 0, 0, 0), br0 = make_int4(0, 0, 0, 0);
     int2 bl0 = make_int2(0, 0);
-    if (tid < min(kNumChunk, a1 - a0)) {
+    if (tid < min(kCh, a1 - a0)) {
         am0 = g.a_meta[a0 + tid];
         br0 = g.a_brange[a0 + tid];
         bl0 = g.a_blight[a0 + tid];
     }
-    for (int i = tid; i < Cfg::kSlots; i += kThreads)
+    for (int i = tid; i < Cfg::kSlots; i += kT)
         s_slot[i] = make_int4(kEmpty, 0, 0, 0);
-    for (int i = tid; i < nnz_row; i += kThreads)
+    for (int i = tid; i < nnz_row; i += kT)
         s_acc[i] = 0.0;
     if (tid < a_len)
         s_acol[tid] = av0;
-    for (int i = tid + kThreads; i < a_len; i += kThreads)
+    for (int i = tid + kT; i < a_len; i += kT)
         s_acol[i] = g.a_vals_cm[a_base + i];
     if (tid < kColRegs)
         s_acol[a_len + tid] = 0.0;   // the register loads of a column may run past it
@@ -1218,8 +1253,8 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     // The chunk's pairs come in four runs, one per (A class, B class): heavy
     // and light A tiles times the heavy and light runs of their B rows, so
     // that the lanes of a wave take pairs of like cost.  Thread v stands for
-    // run v / kNumChunk of tile v % kNumChunk.
-    const int run = tid / kNumChunk, vt = tid % kNumChunk;
+    // run v / kCh of tile v % kCh.
+    const int run = tid / kCh, vt = tid % kCh;
     auto chunk_runs = [&](int n, int &nb, int &bstart) {
         nb = 0;
         bstart = 0;
@@ -1245,17 +1280,17 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
             h = (h + 1) & (Cfg::kSlots - 1);
         s_slot[h] = make_tile_slot(j, base, q, row_runs(m));
     };
-    if (tid < min(kNumChunk, a1 - a0))
+    if (tid < min(kCh, a1 - a0))
         stage_tile(am0, br0, bl0);
     __syncthreads();   // chunk 0 staged, table cleared
-    // One scan gives the first kThreads tiles' value offsets and chunk 0's
-    // pair offsets.
+    // One scan gives the value offsets of the first kT tiles (one per
+    // thread) and chunk 0's pair offsets.
     int nb = 0, bstart = 0, pre = 0, total = 0, base = 0, carry = 0;
-    chunk_runs(min(kNumChunk, a1 - a0), nb, bstart);
-    dual_exclusive_scan(tid < nt ? __popcll(m0) : 0, nb, s_dual, &base, &pre, &carry, &total);
+    chunk_runs(min(kCh, a1 - a0), nb, bstart);
+    dual_exclusive_scan<kNW>(tid < nt ? __popcll(m0) : 0, nb, s_dual, &base, &pre, &carry, &total);
     if (tid < nt)
         insert_tile(tid, j0, m0, base);
-    for (int q0 = kThreads; q0 < nt; q0 += kThreads) {
+    for (int q0 = kT; q0 < nt; q0 += kT) {
         const int q = q0 + tid;
         int j = 0;
         uint64_t m = 0;
@@ -1264,32 +1299,32 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
             m = st.m[sb + q];
         }
         int tot = 0;
-        const int qb = carry + block_exclusive_scan(q < nt ? __popcll(m) : 0, s_tmp, &tot);
+        const int qb = carry + block_exclusive_scan<kNW>(q < nt ? __popcll(m) : 0, s_tmp, &tot);
         carry += tot;
         if (q < nt)
             insert_tile(q, j, m, qb);
     }
-    for (int c0 = a0; c0 < a1; c0 += kNumChunk) {
-        const int n = min(kNumChunk, a1 - c0);
+    for (int c0 = a0; c0 < a1; c0 += kCh) {
+        const int n = min(kCh, a1 - c0);
         if (c0 != a0) {
             if (tid < n)
                 stage_tile(g.a_meta[c0 + tid], g.a_brange[c0 + tid], g.a_blight[c0 + tid]);
             __syncthreads();
             chunk_runs(n, nb, bstart);
-            pre = block_exclusive_scan(nb, s_tmp, &total);
+            pre = block_exclusive_scan<kNW>(nb, s_tmp, &total);
         }
         s_pw[tid] = bstart - pre;
         {
-            for (int r0 = 0; r0 < total; r0 += kSeg) {
-                const int r1 = min(total, r0 + kSeg);
+            for (int r0 = 0; r0 < total; r0 += kSg) {
+                const int r1 = min(total, r0 + kSg);
                 if (nb > 0)
                     fill_segments(s_seg, pre, nb, r0, r1);
                 __syncthreads();
                 // Consecutive lanes take consecutive pairs, so a wave's record
                 // loads are one contiguous stretch of B's records.
-                for (int p = r0 + tid; p < r1; p += kThreads) {
+                for (int p = r0 + tid; p < r1; p += kT) {
                     const int v = s_seg[p - r0];
-                    const int4 at = s_at[v % kNumChunk];
+                    const int4 at = s_at[v % kCh];
                     // The pair's B record, part by part from the planes.
                     const int64_t rb_idx = s_pw[v] + p;
                     BRecord rec;
@@ -1298,7 +1333,7 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
                     // kHeavyMin entries, one row): their values fit in the
                     // first part.
                     static_assert(kHeavyMin - 1 <= 2, "light values fill one record part");
-                    const bool light_b = v >= 2 * kNumChunk;
+                    const bool light_b = v >= 2 * kCh;
 #pragma unroll
                     for (int i = 0; i < kRecVals / 2; ++i) {
                         double2 d = make_double2(0.0, 0.0);
@@ -1428,7 +1463,7 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
     const int shift = static_cast<int>(zb & 3);
     unsigned char *s_colb = reinterpret_cast<unsigned char *>(s_acol);
     static_assert(sizeof(s_acol) >= Cfg::kAcc + 3, "column byte staging");
-    for (int it = tid; it < nt * 8; it += kThreads) {
+    for (int it = tid; it < nt * 8; it += kT) {
         const int q = it >> 3, r = it & 7;
         const uint64_t m = s_cmask[q];
         uint32_t x = static_cast<uint32_t>(m >> (8 * r)) & 0xFFu;
@@ -1439,14 +1474,14 @@ __global__ __launch_bounds__(kThreads) void numeric_row_kernel(
         }
     }
     __syncthreads();
-    for (int64_t i = tid; i < nnz_row; i += kThreads)
+    for (int64_t i = tid; i < nnz_row; i += kT)
         c.val[zb + i] = s_acc[i];
     // Words [0, nw) of the row's column bytes from the aligned word at zb -
     // shift; the first and last may hold bytes of the neighbouring rows and
     // are written byte by byte.
     unsigned char *colw = c.csr_col + (zb - shift);
     const int nw = static_cast<int>((shift + nnz_row + 3) >> 2);
-    for (int i = tid; i < nw; i += kThreads) {
+    for (int i = tid; i < nw; i += kT) {
         const uint32_t v = reinterpret_cast<const uint32_t *>(s_colb)[i];
         if (i > 0 && i < nw - 1) {
             reinterpret_cast<uint32_t *>(colw)[i] = v;
@@ -1794,10 +1829,11 @@ __global__ void collect_rows_kernel(int n, int mode, const int *__restrict__ lev
                                     const int *__restrict__ a_trp, const int *__restrict__ a_off,
                                     int *__restrict__ out_slab, int *__restrict__ n_slab,
                                     int *__restrict__ out_lds, int *__restrict__ n_lds,
-                                    float tile_fill = INFINITY)
+                                    float tile_fill = INFINITY, int *__restrict__ out_tiny = nullptr,
+                                    int *__restrict__ n_tiny = nullptr)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool to_slab = false, to_lds = false;
+    bool to_slab = false, to_lds = false, to_tiny = false;
     if (i < n) {
         const int lv = level[i];
         if (mode == 0 || mode == 2) {
@@ -1815,11 +1851,15 @@ __global__ void collect_rows_kernel(int n, int mode, const int *__restrict__ lev
                 to_slab = true;
             else if (nt > SmallRow::kTiles || nz > SmallRow::kAcc || alen > SmallRow::kAStage)
                 to_lds = true;
+            else
+                to_tiny = fits_row<TinyRow>(nt, nz, alen);
         }
     }
     wave_append(to_slab, i, out_slab, n_slab);
-    if (mode == 1)
+    if (mode == 1) {
         wave_append(to_lds, i, out_lds, n_lds);
+        wave_append(to_tiny, i, out_tiny, n_tiny);
+    }
 }
 
 // Number of (A tile, B tile) pairs: the sum of the per-A-tile extents, one
@@ -1987,15 +2027,17 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     // Counters read back by the host: [0] pairs, [1] symbolic slab rows,
     // [2] numeric slab rows, [3] numeric large-LDS rows, [4] C tiles,
     // [5] nnz(C), [6] rows for the workgroup symbolic table, [7] slab tiles
-    // the staging cursor handed out.
-    RTT_TRY(w->counters.ensure(8 * sizeof(unsigned long long)));
-    RTT_TRY(w->host_counters.ensure(8 * sizeof(unsigned long long)));
+    // the staging cursor handed out, [8] numeric one-wave (TinyRow) rows.
+    constexpr int kCounters = 9;
+    RTT_TRY(w->counters.ensure(kCounters * sizeof(unsigned long long)));
+    RTT_TRY(w->host_counters.ensure(kCounters * sizeof(unsigned long long)));
     unsigned long long *ctr = w->counters.as<unsigned long long>();
     unsigned long long *hctr = w->host_counters.as<unsigned long long>();
     int *slab_count = reinterpret_cast<int *>(ctr + 1);
     int *num_slab_count = reinterpret_cast<int *>(ctr + 2);
     int *num_lds_count = reinterpret_cast<int *>(ctr + 3);
     int *large_count = reinterpret_cast<int *>(ctr + 6);
+    int *num_tiny_count = reinterpret_cast<int *>(ctr + 8);
 
     auto t_sym = Clock::now();
     // ---- symbolic: per-row tile sets, staged ------------------------------
@@ -2006,12 +2048,13 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     RTT_TRY(w->l3_rows.ensure((R + 1) * sizeof(int)));
     RTT_TRY(w->num_rows.ensure((R + 1) * sizeof(int)));
     RTT_TRY(w->lds_rows.ensure((R + 1) * sizeof(int)));
+    RTT_TRY(w->tiny_rows.ensure((R + 1) * sizeof(int)));
     RTT_TRY(w->stage_j.ensure(static_cast<size_t>(R) * stage_limit * sizeof(int)));
     RTT_TRY(w->stage_m.ensure(static_cast<size_t>(R) * stage_limit * sizeof(uint64_t)));
     RTT_TRY(c->tile_ptr.ensure((R + 1) * sizeof(int64_t)));
     RTT_TRY(w->c_row_nnz_off.ensure((R + 1) * sizeof(int64_t)));
     Stage st{w->stage_j.as<int>(), w->stage_m.as<unsigned long long>(), stage_limit};
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(8), 0, s, ctr, int64_t(8));
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(kCounters), 0, s, ctr, int64_t(kCounters));
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, s,
                        reinterpret_cast<unsigned long long *>(w->row_tiles.as<int64_t>() + R),
                        int64_t(1));
@@ -2114,17 +2157,19 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
                            s, R, 1, w->row_level.as<int>(), c->tile_ptr.as<int64_t>(),
                            w->c_row_nnz_off.as<int64_t>(), g.a_trp, g.a_off,
                            w->num_rows.as<int>(), num_slab_count, w->lds_rows.as<int>(),
-                           num_lds_count, g.tile_fill);
+                           num_lds_count, g.tile_fill, w->tiny_rows.as<int>(), num_tiny_count);
     hipLaunchKernelGGL(set_last_kernel, dim3(1), dim3(1), 0, s,
                        reinterpret_cast<int64_t *>(ctr + 4), c->tile_ptr.as<int64_t>() + R);
     hipLaunchKernelGGL(set_last_kernel, dim3(1), dim3(1), 0, s,
                        reinterpret_cast<int64_t *>(ctr + 5), w->c_row_nnz_off.as<int64_t>() + R);
-    RTT_HIP(hipMemcpyAsync(hctr, ctr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    RTT_HIP(hipMemcpyAsync(hctr, ctr, kCounters * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, s));
     RTT_HIP(hipStreamSynchronize(s));
     w->slab_stage_need = std::max<uint64_t>(w->slab_stage_need, hctr[7]);
     const unsigned long long pairs = hctr[0];
     const int n_num_slab = static_cast<int>(hctr[2] & 0xffffffffu);
     const int n_num_lds = static_cast<int>(hctr[3] & 0xffffffffu);
+    const int n_num_tiny = static_cast<int>(hctr[8] & 0xffffffffu);
     const int64_t total_tiles = static_cast<int64_t>(hctr[4]);
     const int64_t total_nnz = static_cast<int64_t>(hctr[5]);
     const double sym_count_ms = ms_since(t_sym);
@@ -2222,6 +2267,10 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     RTT_HIP(hipGetLastError());
     if (!any_tile)
         RTT_HIP(hipEventRecord(ev->ev[5], s));
+    if (any_vector && n_num_tiny > 0)
+        hipLaunchKernelGGL((numeric_row_kernel<TinyRow>), dim3(n_num_tiny), dim3(TinyRow::kThreads),
+                           0, s, g, b.records.as<int4>(), b.num_tiles, w->tiny_rows.as<int>(),
+                           w->row_level.as<int>(), st, co, Timeline{});
     if (n_num_lds > 0)
         hipLaunchKernelGGL((numeric_row_kernel<LargeRow>), dim3(n_num_lds), dim3(kThreads), 0,
                            s, g, b.records.as<int4>(), b.num_tiles, w->lds_rows.as<int>(),

@@ -197,6 +197,52 @@ def test_dense_tiles_and_threshold(gpu, aat):
         assert t["dense_tiles"] > 0 if thr <= 16 else True
 
 
+# LDS budgets of the VECTOR numeric instances, per C tile row: C tiles, C
+# values, A values (the one-wave TinyRow, the workgroup SmallRow and
+# LargeRow; larger rows go to the slab kernels).
+ROW_TIERS = [("tiny", 16, 128, 64), ("small", 128, 1024, 256), ("large", 512, 4096, 2048)]
+
+
+def row_tiers(a, b):
+    """Tile rows of A*B per numeric instance, from the scipy product."""
+    A, B = a.to_scipy().tocsr(), b.tocsr()
+    C = (A @ B).tocoo()
+    T = (A.shape[0] + 7) // 8
+    w = C.shape[1] // 8 + 1
+    keys = np.unique((C.row // 8).astype(np.int64) * w + C.col // 8)
+    nt = np.bincount(keys // w, minlength=T)
+    nz = np.bincount(C.row // 8, minlength=T)
+    al = np.bincount(A.tocoo().row // 8, minlength=T)
+    out = {}
+    for r in range(T):
+        name = next((t for t, mt, mz, ma in ROW_TIERS
+                     if nt[r] <= mt and nz[r] <= mz and al[r] <= ma), "slab")
+        out[name] = out.get(name, 0) + 1
+    return out
+
+
+@pytest.mark.parametrize("aat", [0, 1])
+def test_vector_row_tiers_in_one_call(gpu, aat):
+    # Four bands of 2048 rows each, with 2, 6, 12 and 25 entries per row
+    # within +-8, 24, 40 and 150 of the diagonal: C tile rows of every
+    # LDS instance (the one-wave TinyRow, SmallRow, LargeRow) in the same
+    # VECTOR call, against the oracle.
+    rng = np.random.default_rng(7)
+    n = 8192
+    bands = [(2, 8), (6, 24), (12, 40), (25, 150)]
+    rows, cols = [], []
+    for i in range(n):
+        k, w = bands[i // 2048]
+        c = np.union1d(np.clip(i + rng.integers(-w, w + 1, k), 0, n - 1), [i])
+        rows += [i] * len(c)
+        cols += list(c)
+    a = csr_from(n, n, rows, cols, rng.uniform(0.5, 1.5, len(rows)))
+    s = a.to_scipy().tocsr()
+    tiers = row_tiers(a, s.T if aat else s)
+    assert all(tiers.get(t[0], 0) > 0 for t in ROW_TIERS), tiers
+    run(gpu, a, aat, positive=True, sched=gpu.schedule(numeric_path="vector"))
+
+
 @pytest.mark.parametrize("aat", [0, 1])
 def test_wide_rows_take_the_slab_paths(gpu, aat):
     # An arrow matrix: dense first rows/columns give C tile rows with more
