@@ -107,7 +107,7 @@ def timed_steps(step, steps, warmup):
     return 1e3 * (time.perf_counter() - t0) / steps
 
 
-def batch_banded(rt, data, steps, warmup, precision):
+def batch_banded(rt, data, steps, warmup, precision, sequential=True):
     """One block-diagonal call (one launch per stage over every problem's
     tiles, schedule from the autotuner) against one call per problem with
     the default schedule."""
@@ -126,6 +126,9 @@ def batch_banded(rt, data, steps, warmup, precision):
 
     ms_b = timed_steps(batched, steps, warmup)
     nnz_c = state["st"]["output_nnz"]
+    tuned = {"chosen": table[chosen], "candidates": table}
+    if not sequential:
+        return {"nnz_cub": nnz_cub, "nnz_c": nnz_c, "ms_batched": ms_b, "tuned": tuned}
     # one product per call, default schedule
     pas = [rt.prepare_identity_a(m) for m in mats]
     pbs = [None] * len(pas)
@@ -141,8 +144,7 @@ def batch_banded(rt, data, steps, warmup, precision):
 
     ms_s = timed_steps(sequential, max(1, steps // 2), 1)
     return {"nnz_cub": nnz_cub, "nnz_c": nnz_c, "ms_batched": ms_b, "ms_sequential": ms_s,
-            "nnz_c_matches_sequential": nnz_c == seq_nnz[0], "tuned": {
-                "chosen": table[chosen], "candidates": table}}
+            "nnz_c_matches_sequential": nnz_c == seq_nnz[0], "tuned": tuned}
 
 
 def batch_amg(rt, data, steps, warmup, precision, sequential=True):
@@ -216,7 +218,8 @@ def batch_main(args):
     rt.init(0)
     data = make_batch(rt, spec)
     if spec["kind"] == "banded":
-        r = batch_banded(rt, data, args.steps, args.warmup, args.precision)
+        r = batch_banded(rt, data, args.steps, args.warmup, args.precision,
+                         sequential=not args.no_sequential)
     else:
         r = batch_amg(rt, data, args.steps, args.warmup, args.precision,
                       sequential=not args.no_sequential)

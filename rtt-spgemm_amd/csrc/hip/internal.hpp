@@ -157,7 +157,7 @@ struct DevC {
 // Scratch of the SpGEMM core, reused across calls.
 struct SpgemmWork {
     DevBuf row_tiles, row_nnz, row_level;   // per A tile row
-    DevBuf scan_tmp, counters, l3_rows, num_rows, lds_rows, tiny_rows, slab;
+    DevBuf scan_tmp, counters, l3_rows, num_rows, lds_rows, tiny_rows, mid_rows, slab;
     PinnedBuf host_counters;
     DevBuf stage_j, stage_m;                // staged sorted tile lists
     DevBuf c_row_nnz_off;                   // per tile row, int64

@@ -84,6 +84,13 @@ struct SmallRow {
     static constexpr bool kLeavesTiny = true;
     static constexpr int kAcc = 1024, kAStage = 256, kTiles = 128, kSlots = 256, kSlotBits = 8;
 };
+// MediumRow takes rows past SmallRow's accumulator with few C tiles and A
+// values (narrow bands squared) at a third of LargeRow's LDS.
+struct MediumRow {
+    static constexpr int kThreads = 256, kChunk = 64, kSeg = 1024;
+    static constexpr bool kLeavesTiny = false;
+    static constexpr int kAcc = 2048, kAStage = 256, kTiles = 64, kSlots = 128, kSlotBits = 7;
+};
 struct LargeRow {
     static constexpr int kThreads = 256, kChunk = 64, kSeg = 1024;
     static constexpr bool kLeavesTiny = false;
@@ -1830,10 +1837,12 @@ __global__ void collect_rows_kernel(int n, int mode, const int *__restrict__ lev
                                     int *__restrict__ out_slab, int *__restrict__ n_slab,
                                     int *__restrict__ out_lds, int *__restrict__ n_lds,
                                     float tile_fill = INFINITY, int *__restrict__ out_tiny = nullptr,
-                                    int *__restrict__ n_tiny = nullptr)
+                                    int *__restrict__ n_tiny = nullptr,
+                                    int *__restrict__ out_mid = nullptr,
+                                    int *__restrict__ n_mid = nullptr)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool to_slab = false, to_lds = false, to_tiny = false;
+    bool to_slab = false, to_lds = false, to_tiny = false, to_mid = false;
     if (i < n) {
         const int lv = level[i];
         if (mode == 0 || mode == 2) {
@@ -1849,16 +1858,19 @@ __global__ void collect_rows_kernel(int n, int mode, const int *__restrict__ lev
             const int64_t alen = a_off[a_trp[i + 1]] - a_off[a_trp[i]];
             if (nt > LargeRow::kTiles || nz > LargeRow::kAcc || alen > LargeRow::kAStage)
                 to_slab = true;
-            else if (nt > SmallRow::kTiles || nz > SmallRow::kAcc || alen > SmallRow::kAStage)
-                to_lds = true;
-            else
+            else if (fits_row<SmallRow>(nt, nz, alen))
                 to_tiny = fits_row<TinyRow>(nt, nz, alen);
+            else if (fits_row<MediumRow>(nt, nz, alen))
+                to_mid = true;
+            else
+                to_lds = true;
         }
     }
     wave_append(to_slab, i, out_slab, n_slab);
     if (mode == 1) {
         wave_append(to_lds, i, out_lds, n_lds);
         wave_append(to_tiny, i, out_tiny, n_tiny);
+        wave_append(to_mid, i, out_mid, n_mid);
     }
 }
 
@@ -2027,8 +2039,9 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     // Counters read back by the host: [0] pairs, [1] symbolic slab rows,
     // [2] numeric slab rows, [3] numeric large-LDS rows, [4] C tiles,
     // [5] nnz(C), [6] rows for the workgroup symbolic table, [7] slab tiles
-    // the staging cursor handed out, [8] numeric one-wave (TinyRow) rows.
-    constexpr int kCounters = 9;
+    // the staging cursor handed out, [8] numeric one-wave (TinyRow) rows,
+    // [9] numeric MediumRow rows.
+    constexpr int kCounters = 10;
     RTT_TRY(w->counters.ensure(kCounters * sizeof(unsigned long long)));
     RTT_TRY(w->host_counters.ensure(kCounters * sizeof(unsigned long long)));
     unsigned long long *ctr = w->counters.as<unsigned long long>();
@@ -2038,6 +2051,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     int *num_lds_count = reinterpret_cast<int *>(ctr + 3);
     int *large_count = reinterpret_cast<int *>(ctr + 6);
     int *num_tiny_count = reinterpret_cast<int *>(ctr + 8);
+    int *num_mid_count = reinterpret_cast<int *>(ctr + 9);
 
     auto t_sym = Clock::now();
     // ---- symbolic: per-row tile sets, staged ------------------------------
@@ -2049,6 +2063,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     RTT_TRY(w->num_rows.ensure((R + 1) * sizeof(int)));
     RTT_TRY(w->lds_rows.ensure((R + 1) * sizeof(int)));
     RTT_TRY(w->tiny_rows.ensure((R + 1) * sizeof(int)));
+    RTT_TRY(w->mid_rows.ensure((R + 1) * sizeof(int)));
     RTT_TRY(w->stage_j.ensure(static_cast<size_t>(R) * stage_limit * sizeof(int)));
     RTT_TRY(w->stage_m.ensure(static_cast<size_t>(R) * stage_limit * sizeof(uint64_t)));
     RTT_TRY(c->tile_ptr.ensure((R + 1) * sizeof(int64_t)));
@@ -2157,7 +2172,8 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
                            s, R, 1, w->row_level.as<int>(), c->tile_ptr.as<int64_t>(),
                            w->c_row_nnz_off.as<int64_t>(), g.a_trp, g.a_off,
                            w->num_rows.as<int>(), num_slab_count, w->lds_rows.as<int>(),
-                           num_lds_count, g.tile_fill, w->tiny_rows.as<int>(), num_tiny_count);
+                           num_lds_count, g.tile_fill, w->tiny_rows.as<int>(), num_tiny_count,
+                           w->mid_rows.as<int>(), num_mid_count);
     hipLaunchKernelGGL(set_last_kernel, dim3(1), dim3(1), 0, s,
                        reinterpret_cast<int64_t *>(ctr + 4), c->tile_ptr.as<int64_t>() + R);
     hipLaunchKernelGGL(set_last_kernel, dim3(1), dim3(1), 0, s,
@@ -2170,6 +2186,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     const int n_num_slab = static_cast<int>(hctr[2] & 0xffffffffu);
     const int n_num_lds = static_cast<int>(hctr[3] & 0xffffffffu);
     const int n_num_tiny = static_cast<int>(hctr[8] & 0xffffffffu);
+    const int n_num_mid = static_cast<int>(hctr[9] & 0xffffffffu);
     const int64_t total_tiles = static_cast<int64_t>(hctr[4]);
     const int64_t total_nnz = static_cast<int64_t>(hctr[5]);
     const double sym_count_ms = ms_since(t_sym);
@@ -2270,6 +2287,10 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     if (any_vector && n_num_tiny > 0)
         hipLaunchKernelGGL((numeric_row_kernel<TinyRow>), dim3(n_num_tiny), dim3(TinyRow::kThreads),
                            0, s, g, b.records.as<int4>(), b.num_tiles, w->tiny_rows.as<int>(),
+                           w->row_level.as<int>(), st, co, Timeline{});
+    if (any_vector && n_num_mid > 0)
+        hipLaunchKernelGGL((numeric_row_kernel<MediumRow>), dim3(n_num_mid), dim3(kThreads), 0,
+                           s, g, b.records.as<int4>(), b.num_tiles, w->mid_rows.as<int>(),
                            w->row_level.as<int>(), st, co, Timeline{});
     if (n_num_lds > 0)
         hipLaunchKernelGGL((numeric_row_kernel<LargeRow>), dim3(n_num_lds), dim3(kThreads), 0,

@@ -198,9 +198,10 @@ def test_dense_tiles_and_threshold(gpu, aat):
 
 
 # LDS budgets of the VECTOR numeric instances, per C tile row: C tiles, C
-# values, A values (the one-wave TinyRow, the workgroup SmallRow and
-# LargeRow; larger rows go to the slab kernels).
-ROW_TIERS = [("tiny", 16, 128, 64), ("small", 128, 1024, 256), ("large", 512, 4096, 2048)]
+# values, A values (the one-wave TinyRow, the workgroup SmallRow, MediumRow
+# and LargeRow, first fit; larger rows go to the slab kernels).
+ROW_TIERS = [("tiny", 16, 128, 64), ("small", 128, 1024, 256), ("medium", 64, 2048, 256),
+             ("large", 512, 4096, 2048)]
 
 
 def row_tiers(a, b):
@@ -223,13 +224,13 @@ def row_tiers(a, b):
 
 @pytest.mark.parametrize("aat", [0, 1])
 def test_vector_row_tiers_in_one_call(gpu, aat):
-    # Four bands of 2048 rows each, with 2, 6, 12 and 25 entries per row
-    # within +-8, 24, 40 and 150 of the diagonal: C tile rows of every
-    # LDS instance (the one-wave TinyRow, SmallRow, LargeRow) in the same
-    # VECTOR call, against the oracle.
+    # Five bands of 2048 rows each, with 2, 6, 12, 16 and 25 entries per
+    # row within +-8, 24, 40, 48 and 150 of the diagonal: C tile rows of
+    # every LDS instance (the one-wave TinyRow, SmallRow, MediumRow,
+    # LargeRow) in the same VECTOR call, against the oracle.
     rng = np.random.default_rng(7)
-    n = 8192
-    bands = [(2, 8), (6, 24), (12, 40), (25, 150)]
+    n = 10240
+    bands = [(2, 8), (6, 24), (12, 40), (16, 48), (25, 150)]
     rows, cols = [], []
     for i in range(n):
         k, w = bands[i // 2048]
