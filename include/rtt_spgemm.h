@@ -404,8 +404,9 @@ typedef struct rtt_stats {       /* DmmaSpGemmStats subset */
     unsigned long long pair_count;     /* (A tile, B tile) products visited */
     unsigned long long output_tiles;   /* 8x8 C tiles */
     unsigned long long output_nnz;     /* nnz(C) */
-    unsigned long long heavy_rows;     /* VECTOR: tile rows beyond the small LDS
-                                          budget (larger LDS kernel or HBM slab) */
+    unsigned long long heavy_rows;     /* VECTOR: tile rows beyond the 1024-value
+                                          LDS budget (larger LDS kernels or HBM
+                                          slab) */
     unsigned long long heavy_chunks;   /* TILE, adaptive-tail: chunks the split heavy
                                           C tiles were cut into (0 when not split) */
     int output_materialized;

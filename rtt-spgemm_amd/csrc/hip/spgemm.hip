@@ -2334,7 +2334,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     times->sym_fill_kernel_ms = event_ms(ev->ev[2], ev->ev[3]);
     times->numeric_kernel_ms = event_ms(ev->ev[4], ev->ev[5]);
     times->pairs = pairs;
-    times->heavy_rows = static_cast<unsigned long long>(n_num_slab + n_num_lds);
+    times->heavy_rows = static_cast<unsigned long long>(n_num_slab + n_num_lds + n_num_mid);
     return RTT_OK;
 }
 
