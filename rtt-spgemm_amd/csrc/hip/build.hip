@@ -369,21 +369,23 @@ __device__ __forceinline__ int4 record_part(int part, const int4 &m, const doubl
     return *reinterpret_cast<const int4 *>(&v);
 }
 
-// B's record planes, one part per thread.
-// Value parts (1 .. kRecParts-1) of B's record planes; part 0 (meta) is
-// left as it is.
+// Value parts (1 .. kRecParts-1) of B's record planes, one tile per thread;
+// part 0 (meta) is left as it is.  A thread reads its tile's first values
+// once (one contiguous stretch, consecutive tiles' stretches adjacent) and
+// writes each plane's entry (consecutive threads, consecutive entries).
 __global__ void record_values_kernel(int64_t n, const int4 *__restrict__ meta,
                                      const double *__restrict__ kmajor,
                                      int4 *__restrict__ planes)
 {
-    const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gid >= n * (kRecParts - 1))
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n)
         return;
-    const int part = 1 + static_cast<int>(gid / n);
-    const int64_t t = gid - (part - 1) * n;
     const int4 m = meta[t];
-    if (part < record_parts_used(m))
-        planes[part * n + t] = record_part(part, m, kmajor);
+    const int used = record_parts_used(m);
+#pragma unroll
+    for (int part = 1; part < kRecParts; ++part)
+        if (part < used)
+            planes[part * n + t] = record_part(part, m, kmajor);
 }
 
 // Values-only update: the first source entry of each slot's run sums the
@@ -763,8 +765,7 @@ int ensure_record_values(DevTiles *b)
     RTT_TRY(build_colmajor_values(b));
     RTT_TRY(b->records.ensure(b->num_tiles * sizeof(BRecord)));
     if (b->num_tiles > 0)
-        hipLaunchKernelGGL(record_values_kernel,
-                           dim3(blocks_for(b->num_tiles * (kRecParts - 1), kThreads)),
+        hipLaunchKernelGGL(record_values_kernel, dim3(blocks_for(b->num_tiles, kThreads)),
                            dim3(kThreads), 0, ctx().stream, b->num_tiles, b->meta.as<int4>(),
                            b->values_cm.as<double>(), b->records.as<int4>());
     RTT_HIP(hipGetLastError());
@@ -883,35 +884,37 @@ int build_b_columns(DevTiles *b, BuildWork *w)
     return RTT_OK;
 }
 
-// Thread per A tile: its payload re-packed column-major.  Entry (r, k) goes
-// to off + (rows of columns < k) + (rows < r in column k).
+// An 8x4 tile's payload re-packed column-major, 8 lanes per tile (lane r
+// for physical slots r * 4 .. r * 4 + 3): entry (r, k) goes to off + (rows
+// of columns < k) + (rows < r in column k).  The lanes of a tile read its
+// payload as one stretch and write within the same span; a sparse tile
+// leaves few lanes idle.
 __global__ void colmajor_values_kernel(int64_t n, const int4 *__restrict__ meta,
                                        const double *__restrict__ vals, double *__restrict__ out)
 {
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t t = gid >> 3;
+    const int r = static_cast<int>(gid & 7);
     if (t >= n)
         return;
     const int4 m = meta[t];
+    const uint32_t mask = static_cast<uint32_t>(m.y);
+    uint32_t bits = (mask >> (4 * r)) & 0xFu;
+    if (!bits)
+        return;
     const uint32_t rb = static_cast<uint32_t>(m.z);
     const uint32_t w = static_cast<uint32_t>(m.w);
     const bool dense = (w & 0x80000000u) != 0;
     const uint32_t off = w & 0x7FFFFFFFu;
-    int start[4];
-    int acc = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        start[k] = acc;
-        acc += __popc((rb >> (8 * k)) & 0xFFu);
-    }
-    uint32_t mm = static_cast<uint32_t>(m.y);
-    int j = 0;
-    while (mm) {
-        const int phys = __builtin_ctz(mm);
-        mm &= mm - 1;
-        const int r = phys >> 2, k = phys & 3;
+    int rank = __popc(mask & ((1u << (4 * r)) - 1u));   // entries of rows < r (r < 8)
+    while (bits) {
+        const int k = __builtin_ctz(bits);
+        bits &= bits - 1;
+        const int before = __popc(rb & ((1u << (8 * k)) - 1u));   // rows of columns < k
         const uint32_t col_rows = (rb >> (8 * k)) & 0xFFu;
-        out[off + start[k] + __popc(col_rows & ((1u << r) - 1u))] = vals[off + (dense ? phys : j)];
-        ++j;
+        out[off + before + __popc(col_rows & ((1u << r) - 1u))] =
+            vals[off + (dense ? 4 * r + k : rank)];
+        ++rank;
     }
 }
 
@@ -946,8 +949,9 @@ int build_colmajor_values(DevTiles *t)
     RTT_TRY(t->values_cm.ensure(static_cast<size_t>(t->payload > 0 ? t->payload : 1) *
                                 sizeof(double)));
     if (t->num_tiles > 0)
-        hipLaunchKernelGGL(colmajor_values_kernel, dim3(blocks_for(t->num_tiles, kThreads)),
-                           dim3(kThreads), 0, ctx().stream, t->num_tiles, t->meta.as<int4>(),
+        hipLaunchKernelGGL(colmajor_values_kernel,
+                           dim3(blocks_for(t->num_tiles * 8, kThreads)), dim3(kThreads), 0,
+                           ctx().stream, t->num_tiles, t->meta.as<int4>(),
                            t->values.as<double>(), t->values_cm.as<double>());
     RTT_HIP(hipGetLastError());
     return RTT_OK;
@@ -1361,8 +1365,7 @@ int update_b_values(const DevCsr &src, bool clear, DevTiles *b)
                            src.values.as<double>(), b->values.as<double>(),
                            kmajor ? b->values_cm.as<double>() : nullptr);
     if (kmajor && b->num_tiles > 0)
-        hipLaunchKernelGGL(record_values_kernel,
-                           dim3(blocks_for(b->num_tiles * (kRecParts - 1), kThreads)),
+        hipLaunchKernelGGL(record_values_kernel, dim3(blocks_for(b->num_tiles, kThreads)),
                            dim3(kThreads), 0, s, b->num_tiles, b->meta.as<int4>(),
                            b->values_cm.as<double>(), b->records.as<int4>());
     RTT_HIP(hipGetLastError());
