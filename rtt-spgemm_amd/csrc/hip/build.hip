@@ -101,8 +101,20 @@ __global__ void validate_csr_kernel(int rows, int cols, int64_t nnz,
 
 // Columns of its 64 rows a wave stages in LDS before the walk (when the rows
 // are contiguous in the CSR and fit), so that the lanes' entry-by-entry walk
-// reads LDS instead of issuing one dependent global load per entry.
+// reads LDS instead of issuing one dependent global load per entry.  The
+// stage is sized per launch (dynamic LDS, at most kBuildStage columns per
+// wave) from the operand's average row length: a fixed 3072-column stage
+// held the kernel to 3 waves per SIMD on every operand.
 constexpr int kBuildStage = 3072;
+
+int build_stage_cap(const DevCsr &src, bool row_map)
+{
+    if (row_map || src.rows <= 0)
+        return 0;   // a row map disables the staging
+    const double per_wave = 1.5 * kWave * static_cast<double>(src.nnz) / src.rows;
+    const int cap = (static_cast<int>(per_wave) + 255) / 256 * 256;
+    return std::min(kBuildStage, std::max(256, cap));
+}
 
 // One group of TR lanes per tile row.  COUNT pass: per tile row the number of
 // tiles, payload slots and dense tiles.  FILL pass: tile columns, masks,
@@ -118,30 +130,28 @@ __global__ __launch_bounds__(kThreads) void tile_build_kernel(
     const int64_t *__restrict__ tile_off, const int64_t *__restrict__ payload_off,
     int *__restrict__ tile_col_idx, uint32_t *__restrict__ masks,
     int *__restrict__ value_offsets, double *__restrict__ values, int4 *__restrict__ meta,
-    int2 *__restrict__ src_slot, int *__restrict__ dup_flag)
+    int2 *__restrict__ src_slot, int *__restrict__ dup_flag, int stage_cap)
 {
     static_assert(TR * TC == 32, "tiles hold 32 elements");
     static_assert(kWave % TR == 0, "a wave holds whole tile rows");
-    __shared__ int s_col[kThreads / kWave][kBuildStage];
+    extern __shared__ int s_build_stage[];   // stage_cap columns per wave
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     // Row gid (= tile row gid / TR, row gid % TR of it).  The wave's rows are
     // 64 consecutive rows; without a row map their entries are one stretch
-    // of the CSR, staged in LDS when it fits.
+    // of the CSR, whose first stage_cap columns are staged in LDS (all of
+    // them when they fit).
     const int64_t wave_row0 = gid - lane_id();
     const int64_t lim = min<int64_t>(static_cast<int64_t>(tile_row_count) * TR, active_rows);
-    int e0 = 0, e1 = 0;
+    int e0 = 0, ns = 0;
     if (!row_map && wave_row0 < lim) {
         e0 = rp[wave_row0];
-        e1 = rp[min<int64_t>(wave_row0 + kWave, lim)];
+        ns = min(rp[min<int64_t>(wave_row0 + kWave, lim)] - e0, stage_cap);
     }
-    const bool staged = !row_map && wave_row0 < lim && e1 - e0 <= kBuildStage;
-    int *stage = s_col[threadIdx.x / kWave];
-    if (staged) {
-        for (int x = e0 + lane_id(); x < e1; x += kWave)
-            stage[x - e0] = ci[x];
-    }
+    int *stage = s_build_stage + (threadIdx.x / kWave) * stage_cap;
+    for (int x = lane_id(); x < ns; x += kWave)
+        stage[x] = ci[e0 + x];
     wave_sync();
-    auto col = [&](int x) { return staged ? stage[x - e0] : ci[x]; };
+    auto col = [&](int x) { return x - e0 < ns ? stage[x - e0] : ci[x]; };
     const int64_t tr = gid / TR;
     const int r = static_cast<int>(gid % TR);
     if (tr >= tile_row_count)
@@ -682,15 +692,17 @@ int run_tile_build(const DevCsr &src, const int *row_map, int active_rows,
     hipLaunchKernelGGL(set_i64_kernel, dim3(1), dim3(1), 0, s, w->row_dense.as<int64_t>() + R,
                        int64_t(1), int64_t(0));
     const int64_t threads_total = static_cast<int64_t>(R) * TR;
+    const int stage_cap = build_stage_cap(src, row_map != nullptr);
+    const size_t stage_bytes = static_cast<size_t>(kThreads / kWave) * stage_cap * sizeof(int);
     if (R > 0)
         hipLaunchKernelGGL((tile_build_kernel<TR, TC, CM, false>),
-                           dim3(blocks_for(threads_total, kThreads)), dim3(kThreads), 0, s, R,
-                           active_rows, active_cols, src.row_ptr.as<int>(),
+                           dim3(blocks_for(threads_total, kThreads)), dim3(kThreads), stage_bytes,
+                           s, R, active_rows, active_cols, src.row_ptr.as<int>(),
                            src.col_idx.as<int>(), src.values.as<double>(), row_map,
                            dense_threshold, w->row_tiles.as<int64_t>(),
                            w->row_payload.as<int64_t>(), w->row_dense.as<int64_t>(),
                            nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                           nullptr, nullptr);
+                           nullptr, nullptr, stage_cap);
     RTT_HIP(hipGetLastError());
     RTT_TRY(exclusive_scan_i64(w->row_tiles.as<int64_t>(), tile_pre, R, &w->scan_tmp, s));
     RTT_TRY(exclusive_scan_i64(w->row_payload.as<int64_t>(), pay_pre, R, &w->scan_tmp, s));
@@ -723,13 +735,13 @@ int run_tile_build(const DevCsr &src, const int *row_map, int active_rows,
                        out->value_offsets.as<int>() + out->num_tiles, int64_t(1));
     if (R > 0 && out->num_tiles > 0)
         hipLaunchKernelGGL((tile_build_kernel<TR, TC, CM, true>),
-                           dim3(blocks_for(threads_total, kThreads)), dim3(kThreads), 0, s, R,
-                           active_rows, active_cols, src.row_ptr.as<int>(),
+                           dim3(blocks_for(threads_total, kThreads)), dim3(kThreads), stage_bytes,
+                           s, R, active_rows, active_cols, src.row_ptr.as<int>(),
                            src.col_idx.as<int>(), src.values.as<double>(), row_map,
                            dense_threshold, nullptr, nullptr, nullptr, tile_pre, pay_pre,
                            out->tile_col_idx.as<int>(), out->masks.as<uint32_t>(),
                            out->value_offsets.as<int>(), out->values.as<double>(),
-                           out->meta.as<int4>(), src_slot, dup_flag);
+                           out->meta.as<int4>(), src_slot, dup_flag, stage_cap);
     RTT_HIP(hipGetLastError());
     return RTT_OK;
 }
