@@ -896,37 +896,40 @@ int build_b_columns(DevTiles *b, BuildWork *w)
     return RTT_OK;
 }
 
-// An 8x4 tile's payload re-packed column-major, 8 lanes per tile (lane r
-// for physical slots r * 4 .. r * 4 + 3): entry (r, k) goes to off + (rows
-// of columns < k) + (rows < r in column k).  The lanes of a tile read its
-// payload as one stretch and write within the same span; a sparse tile
-// leaves few lanes idle.
+// An 8x4 tile's payload re-packed column-major, LANES lanes per tile (lane
+// l for tile rows l, l + LANES, ...; row r holds physical slots r * 4 ..
+// r * 4 + 3): entry (r, k) goes to off + (rows of columns < k) + (rows < r
+// in column k).  Eight lanes split the rows of well-filled tiles; one lane
+// walks a tile of a few entries alone, which leaves no lanes idle.
+template <int LANES>
 __global__ void colmajor_values_kernel(int64_t n, const int4 *__restrict__ meta,
                                        const double *__restrict__ vals, double *__restrict__ out)
 {
+    static_assert(8 % LANES == 0, "lanes split the eight tile rows");
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t t = gid >> 3;
-    const int r = static_cast<int>(gid & 7);
+    const int64_t t = gid / LANES;
+    const int r0 = static_cast<int>(gid % LANES);
     if (t >= n)
         return;
     const int4 m = meta[t];
     const uint32_t mask = static_cast<uint32_t>(m.y);
-    uint32_t bits = (mask >> (4 * r)) & 0xFu;
-    if (!bits)
-        return;
     const uint32_t rb = static_cast<uint32_t>(m.z);
     const uint32_t w = static_cast<uint32_t>(m.w);
     const bool dense = (w & 0x80000000u) != 0;
     const uint32_t off = w & 0x7FFFFFFFu;
-    int rank = __popc(mask & ((1u << (4 * r)) - 1u));   // entries of rows < r (r < 8)
-    while (bits) {
-        const int k = __builtin_ctz(bits);
-        bits &= bits - 1;
-        const int before = __popc(rb & ((1u << (8 * k)) - 1u));   // rows of columns < k
-        const uint32_t col_rows = (rb >> (8 * k)) & 0xFFu;
-        out[off + before + __popc(col_rows & ((1u << r) - 1u))] =
-            vals[off + (dense ? 4 * r + k : rank)];
-        ++rank;
+#pragma unroll
+    for (int r = r0; r < 8; r += LANES) {
+        uint32_t bits = (mask >> (4 * r)) & 0xFu;
+        int rank = __popc(mask & ((1u << (4 * r)) - 1u));   // entries of rows < r (r < 8)
+        while (bits) {
+            const int k = __builtin_ctz(bits);
+            bits &= bits - 1;
+            const int before = __popc(rb & ((1u << (8 * k)) - 1u));   // rows of columns < k
+            const uint32_t col_rows = (rb >> (8 * k)) & 0xFFu;
+            out[off + before + __popc(col_rows & ((1u << r) - 1u))] =
+                vals[off + (dense ? 4 * r + k : rank)];
+            ++rank;
+        }
     }
 }
 
@@ -960,8 +963,14 @@ int build_colmajor_values(DevTiles *t)
 {
     RTT_TRY(t->values_cm.ensure(static_cast<size_t>(t->payload > 0 ? t->payload : 1) *
                                 sizeof(double)));
-    if (t->num_tiles > 0)
-        hipLaunchKernelGGL(colmajor_values_kernel,
+    // Tiles of a few entries on average (prolongators, Galerkin products)
+    // take one lane each, fuller tiles eight.
+    if (t->num_tiles > 0 && t->payload < 6 * t->num_tiles)
+        hipLaunchKernelGGL((colmajor_values_kernel<1>), dim3(blocks_for(t->num_tiles, kThreads)),
+                           dim3(kThreads), 0, ctx().stream, t->num_tiles, t->meta.as<int4>(),
+                           t->values.as<double>(), t->values_cm.as<double>());
+    else if (t->num_tiles > 0)
+        hipLaunchKernelGGL((colmajor_values_kernel<8>),
                            dim3(blocks_for(t->num_tiles * 8, kThreads)), dim3(kThreads), 0,
                            ctx().stream, t->num_tiles, t->meta.as<int4>(),
                            t->values.as<double>(), t->values_cm.as<double>());
