@@ -896,19 +896,18 @@ int build_b_columns(DevTiles *b, BuildWork *w)
     return RTT_OK;
 }
 
-// An 8x4 tile's payload re-packed column-major, LANES lanes per tile (lane
-// l for tile rows l, l + LANES, ...; row r holds physical slots r * 4 ..
-// r * 4 + 3): entry (r, k) goes to off + (rows of columns < k) + (rows < r
-// in column k).  Eight lanes split the rows of well-filled tiles; one lane
-// walks a tile of a few entries alone, which leaves no lanes idle.
+// An 8x4 tile's payload re-packed column-major: entry (r, k) goes to off +
+// (rows of columns < k) + (rows < r in column k).  LANES = 8: lane r takes
+// tile row r (physical slots r * 4 .. r * 4 + 3), so the lanes of a
+// well-filled tile read its payload as one stretch.  LANES = 1: one lane
+// walks the mask of a tile of a few entries, leaving no lanes idle.
 template <int LANES>
 __global__ void colmajor_values_kernel(int64_t n, const int4 *__restrict__ meta,
                                        const double *__restrict__ vals, double *__restrict__ out)
 {
-    static_assert(8 % LANES == 0, "lanes split the eight tile rows");
+    static_assert(LANES == 1 || LANES == 8, "a lane per tile or per tile row");
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const int64_t t = gid / LANES;
-    const int r0 = static_cast<int>(gid % LANES);
     if (t >= n)
         return;
     const int4 m = meta[t];
@@ -917,8 +916,27 @@ __global__ void colmajor_values_kernel(int64_t n, const int4 *__restrict__ meta,
     const uint32_t w = static_cast<uint32_t>(m.w);
     const bool dense = (w & 0x80000000u) != 0;
     const uint32_t off = w & 0x7FFFFFFFu;
+    if constexpr (LANES == 1) {
+        int start[4];
+        int acc = 0;
 #pragma unroll
-    for (int r = r0; r < 8; r += LANES) {
+        for (int k = 0; k < 4; ++k) {
+            start[k] = acc;
+            acc += __popc((rb >> (8 * k)) & 0xFFu);
+        }
+        uint32_t mm = mask;
+        int j = 0;
+        while (mm) {
+            const int phys = __builtin_ctz(mm);
+            mm &= mm - 1;
+            const int r = phys >> 2, k = phys & 3;
+            const uint32_t col_rows = (rb >> (8 * k)) & 0xFFu;
+            out[off + start[k] + __popc(col_rows & ((1u << r) - 1u))] =
+                vals[off + (dense ? phys : j)];
+            ++j;
+        }
+    } else {
+        const int r = static_cast<int>(gid & 7);
         uint32_t bits = (mask >> (4 * r)) & 0xFu;
         int rank = __popc(mask & ((1u << (4 * r)) - 1u));   // entries of rows < r (r < 8)
         while (bits) {
