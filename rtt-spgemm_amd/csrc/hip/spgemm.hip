@@ -318,7 +318,7 @@ __global__ __launch_bounds__(kWave) void symbolic_wave_kernel(
     __shared__ unsigned char s_seg[kWSeg];
     static_assert(2 * kWChunk <= 256, "segment entries are bytes");
 
-    const int row = xcd_block(blockIdx.x, gridDim.x);
+    const int row = static_cast<int>(blockIdx.x);
     const int lane = threadIdx.x;
     const int a0 = g.a_trp[row], a1 = g.a_trp[row + 1];
     for (int i = lane; i < kWSlots; i += kWave) {
@@ -484,7 +484,7 @@ __global__ __launch_bounds__(kThreads) void symbolic_count_kernel(
     __shared__ long long s_tmp64[kWavesPerBlock];
     __shared__ int s_ovf;
 
-    const int row = rows ? rows[blockIdx.x] : xcd_block(blockIdx.x, gridDim.x);
+    const int row = rows ? rows[blockIdx.x] : static_cast<int>(blockIdx.x);
     const int tid = threadIdx.x;
     for (int i = tid; i < kSymSlots; i += kThreads) {
         s_key[i] = kEmpty;
@@ -1189,10 +1189,7 @@ __global__ __launch_bounds__(Cfg::kThreads) void numeric_row_kernel(
     const int tid = threadIdx.x;
     if constexpr (TRACE)
         trace_mark(tl, false);
-    // A listed row is read through the same XCD-contiguous mapping, so that
-    // neighbouring list entries (neighbouring rows) meet in one L2.
-    const int bx = xcd_block(blockIdx.x, gridDim.x);
-    const int row = rows ? rows[bx] : bx;
+    const int row = rows ? rows[blockIdx.x] : static_cast<int>(blockIdx.x);
     // The row's inputs come in two rounds of independent loads: its level,
     // C's row extents, A's tile and payload extents (the payload base per
     // tile row is precomputed); then the first staged C tile, A value and A
