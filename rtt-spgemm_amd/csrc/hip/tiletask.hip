@@ -12,16 +12,16 @@
 // increasing K, so C's values do not depend on scheduling: two calls give the
 // same bits.
 //
-// Work unit: one wave per C tile row, taken from per-XCD row queues (each XCD
-// starts on its own contiguous eighth of the rows, so neighbouring rows, which
-// share B columns, meet in one L2; a wave whose XCD queue is drained takes
-// rows from the next XCD's).  The wave stages A row I's tile columns in LDS,
-// then, for up to 64 C tiles at a time, enumerates the B column entries of
-// all of them (one lane per entry), finds each entry's K among A's columns by
-// binary search and compacts the matches, in (C tile, K) order, into an LDS
-// pair list.  The pair loop then issues the operand loads of eight pairs at
-// once and runs their products into the running sum, storing each C tile's
-// entries when its pairs end.
+// Work unit: one wave per C tile row, taken from per-XCD row queues (queue x
+// hands out chunks x, x + 8, x + 16, ... of 16 consecutive rows, so the XCDs
+// work through one window of the rows together; a wave whose XCD queue is
+// drained takes rows from the next XCD's).  The wave stages A row I's tile
+// columns in LDS, then, for up to 64 C tiles at a time, enumerates the B
+// column entries of all of them (one lane per entry), finds each entry's K
+// among A's columns by binary search and compacts the matches, in (C tile,
+// K) order, into an LDS pair list.  The pair loop then issues the operand
+// loads of four pairs at once and runs their products into the running sum,
+// storing each C tile's entries when its pairs end.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -724,16 +724,19 @@ void tile_numeric_kernel(TileTask g)
     const int home = blockIdx.x % kXcds;
     for (int probe = 0; probe < kXcds; ++probe) {
         const int x = (home + probe) % kXcds;
-        const int lo = static_cast<int>(static_cast<int64_t>(g.rows) * x / kXcds);
-        const int hi = static_cast<int>(static_cast<int64_t>(g.rows) * (x + 1) / kXcds);
+        // Queue x hands out chunks x, x + 8, x + 16, ... of kRowGrab rows, so
+        // that the XCDs work through one window of consecutive rows
+        // together (faster than a contiguous eighth of the rows each).
         for (;;) {
             int r = 0;
             if (lane == 0)
-                r = atomicAdd(&g.queue[x], kRowGrab);
-            const int base = lo + read_lane(r, 0);
-            if (base >= hi)
+                r = atomicAdd(&g.queue[x], 1);
+            const int64_t base64 =
+                (static_cast<int64_t>(read_lane(r, 0)) * kXcds + x) * kRowGrab;
+            if (base64 >= g.rows)
                 break;
-            const int rend = min(base + kRowGrab, hi);
+            const int base = static_cast<int>(base64);
+            const int rend = min(base + kRowGrab, g.rows);
             for (int row = base; row < rend; ++row) {
                 const int64_t c0 = g.c_ptr[row], c1 = g.c_ptr[row + 1];
                 if (c0 == c1)
