@@ -214,27 +214,59 @@ __global__ __launch_bounds__(kThreads) void tile_build_kernel(
                         values[p_out + phys] = 0.0;
                 }
             }
-            int x = p;
-            while (x < q) {
-                const int c = col(x);
-                const int x0 = x;
-                double s = val[x];
-                ++x;
-                while (x < q && col(x) == c) {
-                    s += val[x];
-                    ++x;
-                }
+            // Sum of source entries x0 .. x-1 (duplicates of column c, in
+            // entry order) into its payload slot.
+            auto store = [&](int c, double s, int x0, int x) {
                 const int lc = c % TC;
                 const int phys = COL_MAJOR ? lc * TR + r : r * TC + lc;
                 const int slot = static_cast<int>(p_out) + payload_slot(mask, dense ? 32 : cnt, phys);
                 values[slot] = s;
                 if (src_slot) {
-                    // Source entries x0 .. x-1 (duplicates of one coordinate) feed this slot.
                     const int2 sl = make_int2(slot, static_cast<int>(p_out) + kmajor_slot(mask, phys));
                     for (int y = x0; y < x; ++y)
                         src_slot[y] = sl;
                     if (x - x0 > 1)
                         *dup_flag = 1;
+                }
+            };
+            if (q - p <= TC) {
+                // At most TC entries (always, without duplicates): their
+                // values are loaded together, not one dependent load each.
+                double vb[TC];
+#pragma unroll
+                for (int u = 0; u < TC; ++u)
+                    vb[u] = p + u < q ? val[p + u] : 0.0;
+                int cprev = -1, x0 = p;
+                double s = 0.0;
+#pragma unroll
+                for (int u = 0; u < TC; ++u) {
+                    if (p + u < q) {
+                        const int c = col(p + u);
+                        if (c != cprev) {
+                            if (cprev >= 0)
+                                store(cprev, s, x0, p + u);
+                            cprev = c;
+                            x0 = p + u;
+                            s = vb[u];
+                        } else {
+                            s += vb[u];
+                        }
+                    }
+                }
+                if (cprev >= 0)
+                    store(cprev, s, x0, q);
+            } else {
+                int x = p;
+                while (x < q) {
+                    const int c = col(x);
+                    const int x0 = x;
+                    double s = val[x];
+                    ++x;
+                    while (x < q && col(x) == c) {
+                        s += val[x];
+                        ++x;
+                    }
+                    store(c, s, x0, x);
                 }
             }
             ++t_out;
