@@ -179,7 +179,25 @@ __global__ __launch_bounds__(kThreads) void tile_build_kernel(
             break;
         uint32_t bits = 0;
         int q = p;
-        while (q < end) {
+        {
+            // The first TC columns from p are read together; without
+            // duplicates a row has at most TC entries in a tile.
+            int cb[TC];
+#pragma unroll
+            for (int u = 0; u < TC; ++u)
+                cb[u] = p + u < end ? col(p + u) : -1;
+            bool in_tile = true;
+#pragma unroll
+            for (int u = 0; u < TC; ++u) {
+                in_tile = in_tile && cb[u] >= 0 && cb[u] / TC == head;
+                if (in_tile) {
+                    const int lc = cb[u] % TC;
+                    bits |= 1u << (COL_MAJOR ? lc * TR + r : r * TC + lc);
+                    ++q;
+                }
+            }
+        }
+        while (q < end) {   // duplicates: more than TC entries in this tile
             const int c = col(q);
             if (c / TC != head)
                 break;
