@@ -1807,27 +1807,41 @@ __global__ __launch_bounds__(kThreads) void numeric_slab_kernel(
     }
 }
 
-// Lists rows for a fallback path.  mode 0: symbolic slab rows; mode 1:
-// numeric rows beyond the small LDS instance, split into the large LDS
-// instance's list (out_lds) and the slab list (out_slab).
-// Appends v to out (count holds the length) for the lanes where pred holds:
-// one atomic per wave, positions in lane order within the wave.
-__device__ __forceinline__ void wave_append(bool pred, int v, int *__restrict__ out,
-                                            int *__restrict__ count)
+// Appends v to out (count holds the length) for the threads where pred
+// holds: one atomic per workgroup of NW waves (an atomic per wave on one
+// counter serialised at the L2 on millions of rows), positions in thread
+// order within the workgroup.  s holds NW + 1 ints; every thread calls it.
+template <int NW>
+__device__ __forceinline__ void block_append(bool pred, int v, int *__restrict__ out,
+                                             int *__restrict__ count, int *s)
 {
     const unsigned long long m = __ballot(pred);
-    if (!m)
-        return;
-    const int leader = __ffsll(static_cast<long long>(m)) - 1;
-    int base = 0;
-    if (lane_id() == leader)
-        base = atomicAdd(count, __popcll(m));
-    base = __shfl(base, leader, kWave);
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0)
+        s[w] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int total = 0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i)
+            total += s[i];
+        s[NW] = total ? atomicAdd(count, total) : 0;
+    }
+    __syncthreads();
+    int base = s[NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i)
+        base += i < w ? s[i] : 0;
     if (pred)
         out[base + __popcll(m & ((1ull << lane_id()) - 1ull))] = v;
+    __syncthreads();   // s is reused by the next call
 }
 
-__global__ void collect_rows_kernel(int n, int mode, const int *__restrict__ level,
+// Lists rows for the other passes.  mode 0: symbolic slab rows; mode 2:
+// rows for the workgroup symbolic table; mode 1: numeric rows off
+// SmallRow, into the LargeRow list (out_lds), the slab list (out_slab),
+// the TinyRow list (out_tiny) and the MediumRow list (out_mid).
+__global__ __launch_bounds__(kThreads) void collect_rows_kernel(int n, int mode, const int *__restrict__ level,
                                     const int64_t *__restrict__ tile_ptr,
                                     const int64_t *__restrict__ nnz_off,
                                     const int *__restrict__ a_trp, const int *__restrict__ a_off,
@@ -1863,11 +1877,12 @@ __global__ void collect_rows_kernel(int n, int mode, const int *__restrict__ lev
                 to_lds = true;
         }
     }
-    wave_append(to_slab, i, out_slab, n_slab);
+    __shared__ int s_app[kWavesPerBlock + 1];
+    block_append<kWavesPerBlock>(to_slab, i, out_slab, n_slab, s_app);
     if (mode == 1) {
-        wave_append(to_lds, i, out_lds, n_lds);
-        wave_append(to_tiny, i, out_tiny, n_tiny);
-        wave_append(to_mid, i, out_mid, n_mid);
+        block_append<kWavesPerBlock>(to_lds, i, out_lds, n_lds, s_app);
+        block_append<kWavesPerBlock>(to_tiny, i, out_tiny, n_tiny, s_app);
+        block_append<kWavesPerBlock>(to_mid, i, out_mid, n_mid, s_app);
     }
 }
 

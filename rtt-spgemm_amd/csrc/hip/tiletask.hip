@@ -1111,24 +1111,43 @@ TileColumns b_columns(const DevTiles &b)
 }
 
 // count[0]: tile rows routed to the tile-pair kernel at `fill`; count[1]:
-// tile rows holding any tile.  One atomic pair per wave.
-__global__ void count_tile_rows_kernel(int rows, const int *__restrict__ trp,
-                                       const int *__restrict__ row_off, float fill,
-                                       unsigned long long *__restrict__ count)
+// tile rows holding any tile.  A grid-stride pass of at most kCountBlocks
+// workgroups with one atomic pair per workgroup: an atomic pair per wave on
+// the same two counters serialised at the L2 (0.3 ms for 2.6 M rows).
+constexpr int kCountThreads = 256;
+constexpr int kCountBlocks = 1024;
+__global__ __launch_bounds__(kCountThreads) void count_tile_rows_kernel(
+    int rows, const int *__restrict__ trp, const int *__restrict__ row_off, float fill,
+    unsigned long long *__restrict__ count)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    bool take = false, any = false;
-    if (r < rows) {
+    __shared__ unsigned int s_m[kCountThreads / kWave], s_n[kCountThreads / kWave];
+    unsigned int m = 0, n = 0;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
         const int t = trp[r + 1] - trp[r];
-        any = t > 0;
-        take = any && static_cast<float>(row_off[r + 1] - row_off[r]) >= fill * static_cast<float>(t);
+        const bool any = t > 0;
+        const bool take =
+            any && static_cast<float>(row_off[r + 1] - row_off[r]) >= fill * static_cast<float>(t);
+        m += __popcll(__ballot(take));
+        n += __popcll(__ballot(any));
     }
-    const unsigned long long m = __ballot(take), n = __ballot(any);
+    // Lane 0 holds the wave's smallest row, so it takes part in every ballot
+    // of its wave and its sums are the wave's totals.
     if (lane_id() == 0) {
-        if (m)
-            atomicAdd(count, static_cast<unsigned long long>(__popcll(m)));
-        if (n)
-            atomicAdd(count + 1, static_cast<unsigned long long>(__popcll(n)));
+        s_m[threadIdx.x / kWave] = m;
+        s_n[threadIdx.x / kWave] = n;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long bm = 0, bn = 0;
+#pragma unroll
+        for (int w = 0; w < kCountThreads / kWave; ++w) {
+            bm += s_m[w];
+            bn += s_n[w];
+        }
+        if (bm)
+            atomicAdd(count, bm);
+        if (bn)
+            atomicAdd(count + 1, bn);
     }
 }
 
@@ -1143,8 +1162,10 @@ int count_tile_rows(const DevTiles &a, float fill, DevBuf *scratch, int64_t *til
                        d + 1);
     const int R = a.tile_row_count;
     if (R > 0)
-        hipLaunchKernelGGL(count_tile_rows_kernel, dim3(blocks_for(R, 256)), dim3(256), 0, s, R,
-                           a.tile_row_ptr.as<int>(), a.row_payload_off.as<int>(), fill, d);
+        hipLaunchKernelGGL(count_tile_rows_kernel,
+                           dim3(std::min<unsigned int>(kCountBlocks, blocks_for(R, kCountThreads))),
+                           dim3(kCountThreads), 0, s, R, a.tile_row_ptr.as<int>(),
+                           a.row_payload_off.as<int>(), fill, d);
     RTT_HIP(hipGetLastError());
     unsigned long long h[2] = {0, 0};
     RTT_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s));
