@@ -179,9 +179,11 @@ __global__ __launch_bounds__(kThreads) void tile_build_kernel(
             break;
         uint32_t bits = 0;
         int q = p;
-        {
-            // The first TC columns from p are read together; without
-            // duplicates a row has at most TC entries in a tile.
+        if (p + TC - e0 <= ns) {
+            // The first TC columns from p, staged in LDS, are read together;
+            // without duplicates a row has at most TC entries in a tile.
+            // (Unstaged, the extra reads would be global loads of columns
+            // past a sparse tile's few entries.)
             int cb[TC];
 #pragma unroll
             for (int u = 0; u < TC; ++u)
@@ -197,7 +199,7 @@ __global__ __launch_bounds__(kThreads) void tile_build_kernel(
                 }
             }
         }
-        while (q < end) {   // duplicates: more than TC entries in this tile
+        while (q < end) {   // the rest (all of it when unstaged)
             const int c = col(q);
             if (c / TC != head)
                 break;
@@ -247,9 +249,12 @@ __global__ __launch_bounds__(kThreads) void tile_build_kernel(
                         *dup_flag = 1;
                 }
             };
-            if (q - p <= TC) {
+            if (q - p == 1) {
+                store(col(p), val[p], p, q);   // the common case of sparse operands
+            } else if (!row_map && q - p <= TC) {
                 // At most TC entries (always, without duplicates): their
                 // values are loaded together, not one dependent load each.
+                // (Measured slower for rows gathered through a row map.)
                 double vb[TC];
 #pragma unroll
                 for (int u = 0; u < TC; ++u)
