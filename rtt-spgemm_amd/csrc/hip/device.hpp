@@ -21,11 +21,24 @@ __device__ __forceinline__ void wave_sync()
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on
-// XCD b % 8).  The row kernels take row b in workgroup b, so the rows in
-// flight on the chip are one window of consecutive rows spread over every
-// XCD: measured against giving each XCD a contiguous eighth of the rows, the
-// stencil A*A^T numeric pass ran 3 % and the banded batch 6 % faster.
+// XCD b % 8).  The row kernels keep the rows in flight on the chip one
+// window of consecutive rows spread over every XCD: measured against giving
+// each XCD a contiguous eighth of the rows, the stencil A*A^T numeric pass
+// ran 3 % and the banded batch 6 % faster.
 constexpr int kXcds = 8;
+
+// Row of workgroup b of n: chunks of C consecutive rows dealt to the XCDs
+// round-robin (XCD x takes chunks x, x + 8, ...), so that the rows one L2
+// holds come in runs of C; rows past the last whole round of 8 C rows map
+// to themselves.  A bijection of [0, n).
+template <int C>
+__device__ __forceinline__ int xcd_chunk_row(int b, int n)
+{
+    if (b >= n / (kXcds * C) * (kXcds * C))
+        return b;
+    const int x = b % kXcds, i = b / kXcds;
+    return ((i / C) * kXcds + x) * C + i % C;
+}
 
 // Gathers bits k, k+4, ..., k+28 of a 32-bit tile mask into 8 bits.  For an
 // A tile (bit r*4+k) this is the set of rows present in column k; for a B
