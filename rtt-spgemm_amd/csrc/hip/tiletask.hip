@@ -13,7 +13,7 @@
 // same bits.
 //
 // Work unit: one wave per C tile row, taken from per-XCD row queues (queue x
-// hands out chunks x, x + 8, x + 16, ... of 16 consecutive rows, so the XCDs
+// hands out chunks x, x + 8, x + 16, ... of 8 consecutive rows, so the XCDs
 // work through one window of the rows together; a wave whose XCD queue is
 // drained takes rows from the next XCD's).  The wave stages A row I's tile
 // columns in LDS, then, for up to 64 C tiles at a time, enumerates the B
@@ -45,8 +45,9 @@ constexpr int kBatch = 4;       // pairs whose operand loads are issued together
                                 // block-FEM product 5 % faster than 8: fewer registers, no
                                 // spills, and the next batch's loads are issued sooner)
 constexpr int kEnumRounds = 4;  // rounds of 64 B column entries whose K are loaded together
-constexpr int kRowGrab = 16;    // tile rows a wave takes from a row queue at once (one atomic:
-                                // per-row atomics on 8 heads serialise at the L2)
+constexpr int kRowGrab = 8;     // tile rows a wave takes from a row queue at once (one atomic:
+                                // per-row atomics on 8 heads serialise at the L2); with the
+                                // queues interleaved, 8 ran 6 % faster than 16 and 4 or 32
 static_assert(kPairCap % kBatch == 0 && kPairCap >= 2 * kWave, "pair buffer");
 
 struct TileTask {
