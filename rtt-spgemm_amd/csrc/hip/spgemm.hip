@@ -1189,14 +1189,13 @@ __global__ __launch_bounds__(Cfg::kThreads) void numeric_row_kernel(
     const int tid = threadIdx.x;
     if constexpr (TRACE)
         trace_mark(tl, false);
-    // Unlisted rows in 64-row chunks per XCD: the B records that
+    // Rows (or list entries) in 64-row chunks per XCD: the B records that
     // neighbouring rows share stay in one L2 for a run of 64 rows, and the
     // window of rows in flight still spans the chip.  Against row b in
-    // workgroup b: the same speed or 1 % better on the stencil step, and a
-    // third less L2 fetch traffic.
-    const int row = rows ? rows[blockIdx.x]
-                         : xcd_chunk_row<64>(static_cast<int>(blockIdx.x),
-                                             static_cast<int>(gridDim.x));
+    // workgroup b: the same speed or 1 % better on the stencil step, a
+    // third less L2 fetch traffic, and 0.5 % on the batched forms' lists.
+    const int bx = xcd_chunk_row<64>(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+    const int row = rows ? rows[bx] : bx;
     // The row's inputs come in two rounds of independent loads: its level,
     // C's row extents, A's tile and payload extents (the payload base per
     // tile row is precomputed); then the first staged C tile, A value and A
