@@ -405,7 +405,7 @@ int rtt_prepare_external_a(const rtt_smatrix *host, const int *row_new_to_old,
     RTT_TRY(rtt::build_colmajor_values(&a->tiles));
     RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
     auto t_tr = rtt::Clock::now();
-    RTT_TRY(rtt::build_transpose_source(&a->tiles));
+    RTT_TRY(rtt::build_transpose_source(&a->tiles, &a->build));
     RTT_HIP(hipStreamSynchronize(rtt::ctx().stream));
     st.transpose_src_ms = rtt::ms_since(t_tr);
     st.tile_build_ms = rtt::ms_since(t_tiles);

@@ -529,6 +529,113 @@ __device__ __forceinline__ int b_tile_class(uint32_t mask, uint32_t bytes)
     return 1 + __builtin_ctz(rows);
 }
 
+// Counting transpose of A's tiles into B = A^T rows, for operands whose tile
+// columns hold at most kShortCol tiles: a count per B row, a scan, a scatter
+// in arbitrary order within each row, then one workgroup per kOrderRows rows
+// ranks every entry within its row by (class, A tile id).  The result is the
+// stable sort by (row, class) that the radix path produces, entry for entry.
+constexpr int kShortCol = 64;     // most tiles in an A tile column it takes
+constexpr int kOrderRows = 64;    // B rows per ordering workgroup
+constexpr int kOrderIdBits = 28;  // A tile ids below 2^28; the class above them
+
+// B class of the A^T tile made from each A tile, and its arrival rank in its
+// B row (the tile's column of A).
+__global__ void column_rank_kernel(int64_t n, const int *__restrict__ col,
+                                   const uint32_t *__restrict__ masks, int *__restrict__ counts,
+                                   int *__restrict__ rank_cls)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    const uint32_t mask = masks[t];
+    uint32_t bytes = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        bytes |= stride4_bits(mask, k) << (8 * k);
+    const int r = atomicAdd(&counts[col[t]], 1);
+    rank_cls[t] = (r << 3) | b_tile_class(mask, bytes);
+}
+
+__global__ void column_scatter_kernel(int64_t n, const int *__restrict__ col,
+                                      const int *__restrict__ rank_cls,
+                                      const int *__restrict__ rp, uint32_t *__restrict__ ent)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    const int rc = rank_cls[t];
+    ent[rp[col[t]] + (rc >> 3)] = (static_cast<uint32_t>(rc & 7) << kOrderIdBits) |
+                                  static_cast<uint32_t>(t);
+}
+
+// Rows r0 .. r0 + kOrderRows - 1 of B: each entry's place is its row's start
+// plus the number of the row's entries below it by (class, id); each
+// (row, class) run starts after the row's entries of lower classes.
+__global__ void __launch_bounds__(kThreads)
+    order_rows_kernel(int R, int64_t n, const int *__restrict__ rp,
+                      const uint32_t *__restrict__ ent, int *__restrict__ sorted_ids,
+                      int *__restrict__ groups)
+{
+    __shared__ int s_rp[kOrderRows + 1];
+    __shared__ uint32_t s_ent[kOrderRows * kShortCol];
+    const int r0 = blockIdx.x * kOrderRows;
+    const int nr = min(kOrderRows, R - r0);
+    for (int i = threadIdx.x; i <= nr; i += blockDim.x)
+        s_rp[i] = rp[r0 + i];
+    __syncthreads();
+    const int base = s_rp[0];
+    const int total = s_rp[nr] - base;   // at most nr * kShortCol
+    for (int e = threadIdx.x; e < total; e += blockDim.x)
+        s_ent[e] = ent[base + e];
+    __syncthreads();
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        int lo = 0, hi = nr - 1;   // last row starting at or before e
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_rp[mid] - base <= e)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int b = s_rp[lo] - base, end = s_rp[lo + 1] - base;
+        const uint32_t v = s_ent[e];
+        int rank = 0;
+        for (int x = b; x < end; ++x)
+            rank += s_ent[x] < v ? 1 : 0;
+        sorted_ids[base + b + rank] = static_cast<int>(v & ((1u << kOrderIdBits) - 1u));
+    }
+    for (int i = threadIdx.x; i < nr * kClassSlots; i += blockDim.x) {
+        const int r = i / kClassSlots;
+        const uint32_t c = static_cast<uint32_t>(i % kClassSlots);
+        const int b = s_rp[r] - base, end = s_rp[r + 1] - base;
+        int below = 0;
+        for (int x = b; x < end; ++x)
+            below += (s_ent[x] >> kOrderIdBits) < c ? 1 : 0;
+        groups[static_cast<int64_t>(r0 + r) * kClassSlots + c] = base + b + below;
+    }
+    if (r0 + nr == R && threadIdx.x == 0)
+        groups[static_cast<int64_t>(R) * kClassSlots] = static_cast<int>(n);
+}
+
+// Tiles per tile column, and their maximum in counts[cols].
+__global__ void column_count_kernel(int64_t n, const int *__restrict__ col, int *__restrict__ counts)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < n)
+        atomicAdd(&counts[col[t]], 1);
+}
+
+__global__ void column_max_kernel(int cols, int *__restrict__ counts)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int m = j < cols ? counts[j] : 0;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1)
+        m = max(m, __shfl_xor(m, o, kWave));
+    if (lane_id() == 0 && m > 0)
+        atomicMax(&counts[cols], m);
+}
+
 // Per A tile t in column K: {first B tile of row K, heavy tiles, all tiles}
 // and the light run its pairs need: for an A tile whose entries all lie in
 // one column k, only the light tiles of row k can meet it.
@@ -1019,8 +1126,26 @@ __global__ void row_payload_off_kernel(int rows, const int *__restrict__ trp,
         out[r] = off[trp[r]];
 }
 
-int build_transpose_source(DevTiles *t)
+int build_transpose_source(DevTiles *t, BuildWork *w)
 {
+    // The longest tile column decides how the B = A^T update orders B's rows.
+    const int C = t->tile_col_count;
+    RTT_TRY(w->counters.ensure((static_cast<size_t>(C) + 1) * sizeof(int)));
+    int *counts = w->counters.as<int>();
+    hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(C + 1, kThreads)), dim3(kThreads), 0,
+                       ctx().stream, counts, static_cast<int64_t>(C) + 1, 0);
+    if (t->num_tiles > 0) {
+        hipLaunchKernelGGL(column_count_kernel, dim3(blocks_for(t->num_tiles, kThreads)),
+                           dim3(kThreads), 0, ctx().stream, t->num_tiles,
+                           t->tile_col_idx.as<int>(), counts);
+        hipLaunchKernelGGL(column_max_kernel, dim3(blocks_for(C, kThreads)), dim3(kThreads), 0,
+                           ctx().stream, C, counts);
+    }
+    int longest = 0;
+    RTT_HIP(hipMemcpyAsync(&longest, counts + C, sizeof(int), hipMemcpyDeviceToHost,
+                           ctx().stream));
+    RTT_HIP(hipStreamSynchronize(ctx().stream));
+    t->max_col_tiles = longest;
     RTT_TRY(t->row_payload_off.ensure((static_cast<size_t>(t->tile_row_count) + 1) * sizeof(int)));
     hipLaunchKernelGGL(row_payload_off_kernel, dim3(blocks_for(t->tile_row_count + 1, kThreads)),
                        dim3(kThreads), 0, ctx().stream, t->tile_row_count,
@@ -1242,6 +1367,34 @@ int transpose_tiles(const DevTiles &a, int b_cols, DevTiles *b, BuildWork *w)
     RTT_TRY(w->ids_in.ensure(n * sizeof(int)));
     RTT_TRY(w->keys_in.ensure(n * sizeof(int)));
     RTT_TRY(w->keys_out.ensure(n * sizeof(int)));
+    if (n > 0 && a.max_col_tiles >= 0 && a.max_col_tiles <= kShortCol &&
+        n < (int64_t(1) << kOrderIdBits)) {
+        // The same order by counting: B's rows are short.
+        RTT_TRY(w->keys_out.ensure((std::max<int64_t>(n, R + 1)) * sizeof(int)));
+        int *counts = w->keys_out.as<int>();
+        int *rp = b->tile_row_ptr.as<int>();
+        hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(R + 1, kThreads)), dim3(kThreads), 0,
+                           s, counts, static_cast<int64_t>(R) + 1, 0);
+        hipLaunchKernelGGL(column_rank_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0,
+                           s, n, a.tile_col_idx.as<int>(), a.masks.as<uint32_t>(), counts,
+                           w->keys_in.as<int>());
+        size_t bytes = 0;
+        RTT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, counts, rp, R + 1, s));
+        RTT_TRY(w->scan_tmp.ensure(bytes));
+        RTT_HIP(hipcub::DeviceScan::ExclusiveSum(w->scan_tmp.ptr, bytes, counts, rp, R + 1, s));
+        hipLaunchKernelGGL(column_scatter_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads),
+                           0, s, n, a.tile_col_idx.as<int>(), w->keys_in.as<int>(), rp,
+                           w->ids_in.as<uint32_t>());
+        hipLaunchKernelGGL(order_rows_kernel, dim3(blocks_for(R, kOrderRows)), dim3(kThreads), 0,
+                           s, R, n, rp, w->ids_in.as<uint32_t>(), b->sorted_ids.as<int>(),
+                           b->row_groups.as<int>());
+        RTT_TRY(b->records.ensure(n * sizeof(BRecord)));
+        hipLaunchKernelGGL(transposed_records_kernel, dim3(blocks_for(n, kThreads)),
+                           dim3(kThreads), 0, s, n, b->sorted_ids.as<int>(),
+                           a.transpose_src.as<BRecord>(), b->records.as<int4>());
+        RTT_HIP(hipGetLastError());
+        return RTT_OK;
+    }
     if (n > 0) {
         // Stable radix sort of A's tiles by (tile column, class): B's row
         // order with every row's heavy tiles first, then its light tiles by

@@ -123,6 +123,7 @@ struct DevTiles {
                           // A^T this is A's array (payload_kmajor()).
 
     DevBuf transpose_src; // A only: per tile, the BRecord of the B = A^T tile it becomes
+    int max_col_tiles = -1;   // A only: most tiles in one tile column (-1: not counted)
     DevBuf row_payload_off;   // A only: payload offset of each tile row (tile_row_count + 1)
     // B built as A^T at tile level: meta offsets index A's payload, and
     // sorted_ids[b] is the A tile behind B tile b.
@@ -236,7 +237,7 @@ int fill_tile_row_of(DevTiles *t);
 int build_colmajor_values(DevTiles *t);
 // Fills t->transpose_src (A tiles, offline; needs tile_row_of) and
 // t->row_payload_off.
-int build_transpose_source(DevTiles *t);
+int build_transpose_source(DevTiles *t, BuildWork *w);
 // b->a_brange[t] = {B.tile_row_ptr[K], heavy tiles, count} and b->a_blight[t]
 // for every A tile t in column K.
 int build_pair_extents(const DevTiles &a, DevTiles *b);
