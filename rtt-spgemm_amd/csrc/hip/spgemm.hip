@@ -1910,6 +1910,30 @@ __global__ void fill_u64_kernel(unsigned long long *p, int64_t n)
 
 __global__ void set_last_kernel(int64_t *p, const int64_t *v) { *p = *v; }
 
+__global__ void set_last2_kernel(int64_t *p, const int64_t *v, int64_t *q, const int64_t *u)
+{
+    *p = *v;
+    *q = *u;
+}
+
+// Zeroes the host-read counters except [0], which takes the pair total, the
+// trailing entries of the per-row size arrays and the queue heads.
+__global__ void symbolic_init_kernel(unsigned long long *ctr, int n_ctr,
+                                     const unsigned long long *pair_total,
+                                     unsigned long long *row_tiles_end,
+                                     unsigned long long *row_nnz_end, int *queue, int n_queue)
+{
+    const int i = threadIdx.x;
+    if (i < n_ctr)
+        ctr[i] = i == 0 ? *pair_total : 0ull;
+    if (i < n_queue)
+        queue[i] = 0;
+    if (i == 0) {
+        *row_tiles_end = 0ull;
+        *row_nnz_end = 0ull;
+    }
+}
+
 double event_ms(hipEvent_t a, hipEvent_t b)
 {
     float ms = 0.f;
@@ -2087,16 +2111,15 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     RTT_TRY(c->tile_ptr.ensure((R + 1) * sizeof(int64_t)));
     RTT_TRY(w->c_row_nnz_off.ensure((R + 1) * sizeof(int64_t)));
     Stage st{w->stage_j.as<int>(), w->stage_m.as<unsigned long long>(), stage_limit};
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(kCounters), 0, s, ctr, int64_t(kCounters));
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, s,
+    // Queue heads: [0] symbolic slab count, [1] symbolic slab fill, [2] numeric slab.
+    RTT_TRY(w->queue.ensure(4 * sizeof(int)));
+    int *queue = w->queue.as<int>();
+    // Counter [0] is the pair total, summed by the B update's pair-extent pass.
+    hipLaunchKernelGGL(symbolic_init_kernel, dim3(1), dim3(kCounters), 0, s, ctr, kCounters,
+                       b.pair_total.as<unsigned long long>(),
                        reinterpret_cast<unsigned long long *>(w->row_tiles.as<int64_t>() + R),
-                       int64_t(1));
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, s,
                        reinterpret_cast<unsigned long long *>(w->row_nnz.as<int64_t>() + R),
-                       int64_t(1));
-    // The pair total, summed by the B update's pair-extent pass.
-    hipLaunchKernelGGL(set_last_kernel, dim3(1), dim3(1), 0, s, reinterpret_cast<int64_t *>(ctr),
-                       b.pair_total.as<int64_t>());
+                       queue, 4);
     // One wave per row first; rows that outgrow its table go to the
     // workgroup table, and rows that outgrow that to the slab.
     RTT_HIP(hipEventRecord(ev->ev[0], s));
@@ -2133,11 +2156,7 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     RTT_HIP(hipEventRecord(ev->ev[1], s));
     int slab_blocks = 0;
     const int *slab_rows = nullptr;
-    // Queue heads: [0] symbolic slab count, [1] symbolic slab fill, [2] numeric slab.
-    RTT_TRY(w->queue.ensure(4 * sizeof(int)));
-    int *queue = w->queue.as<int>();
     const int static_rows = lb_mode == RTT_LB_STATIC ? 1 : 0;
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(4), 0, s, queue, int64_t(4), 0);
     SlabStage ss;
     if (n_slab > 0) {
         // Staging of the slab rows' tile lists: grown to what the previous
@@ -2192,9 +2211,8 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
                            w->num_rows.as<int>(), num_slab_count, w->lds_rows.as<int>(),
                            num_lds_count, g.tile_fill, w->tiny_rows.as<int>(), num_tiny_count,
                            w->mid_rows.as<int>(), num_mid_count);
-    hipLaunchKernelGGL(set_last_kernel, dim3(1), dim3(1), 0, s,
-                       reinterpret_cast<int64_t *>(ctr + 4), c->tile_ptr.as<int64_t>() + R);
-    hipLaunchKernelGGL(set_last_kernel, dim3(1), dim3(1), 0, s,
+    hipLaunchKernelGGL(set_last2_kernel, dim3(1), dim3(1), 0, s,
+                       reinterpret_cast<int64_t *>(ctr + 4), c->tile_ptr.as<int64_t>() + R,
                        reinterpret_cast<int64_t *>(ctr + 5), w->c_row_nnz_off.as<int64_t>() + R);
     RTT_HIP(hipMemcpyAsync(hctr, ctr, kCounters * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, s));

@@ -1158,8 +1158,7 @@ int count_tile_rows(const DevTiles &a, float fill, DevBuf *scratch, int64_t *til
     hipStream_t s = ctx().stream;
     RTT_TRY(scratch->ensure(2 * sizeof(unsigned long long)));
     unsigned long long *d = scratch->as<unsigned long long>();
-    hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(2), 0, s, reinterpret_cast<int *>(d), 0, d);
-    hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(2), 0, s, reinterpret_cast<int *>(d), 0,
+    hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(4), 0, s, reinterpret_cast<int *>(d), 4,
                        d + 1);
     const int R = a.tile_row_count;
     if (R > 0)
