@@ -2130,15 +2130,46 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
     RTT_HIP(hipGetLastError());
     int n_slab = 0;
     COut none{};
+    // Sizes of C from the per-row counts: row offsets, the numeric routing
+    // lists, and every counter read back with one copy.  Rows of the
+    // tile-pair kernel are not listed.
+    auto finish_sizes = [&]() -> int {
+        RTT_TRY(exclusive_scan_i64(w->row_tiles.as<int64_t>(), c->tile_ptr.as<int64_t>(), R,
+                                   &w->scan_tmp, s));
+        RTT_TRY(exclusive_scan_i64(w->row_nnz.as<int64_t>(), w->c_row_nnz_off.as<int64_t>(), R,
+                                   &w->scan_tmp, s));
+        if (R > 0 && any_vector)
+            hipLaunchKernelGGL(collect_rows_kernel, dim3(blocks_for(R, kThreads)),
+                               dim3(kThreads), 0, s, R, 1, w->row_level.as<int>(),
+                               c->tile_ptr.as<int64_t>(), w->c_row_nnz_off.as<int64_t>(),
+                               g.a_trp, g.a_off, w->num_rows.as<int>(), num_slab_count,
+                               w->lds_rows.as<int>(), num_lds_count, g.tile_fill,
+                               w->tiny_rows.as<int>(), num_tiny_count, w->mid_rows.as<int>(),
+                               num_mid_count);
+        hipLaunchKernelGGL(set_last2_kernel, dim3(1), dim3(1), 0, s,
+                           reinterpret_cast<int64_t *>(ctr + 4), c->tile_ptr.as<int64_t>() + R,
+                           reinterpret_cast<int64_t *>(ctr + 5),
+                           w->c_row_nnz_off.as<int64_t>() + R);
+        RTT_HIP(hipMemcpyAsync(hctr, ctr, kCounters * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, s));
+        RTT_HIP(hipStreamSynchronize(s));
+        return RTT_OK;
+    };
+    bool sized = false;
     if (R > 0) {
         hipLaunchKernelGGL(collect_rows_kernel, dim3(blocks_for(R, kThreads)), dim3(kThreads), 0,
                            s, R, 2, w->row_level.as<int>(), nullptr, nullptr, nullptr, nullptr,
                            w->l3_rows.as<int>(), large_count, nullptr, nullptr);
-        RTT_HIP(hipMemcpyAsync(hctr, ctr, 7 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                               s));
-        RTT_HIP(hipStreamSynchronize(s));
+        RTT_HIP(hipEventRecord(ev->ev[1], s));
+        // Sized at once, as if every row fit the wave table: one readback
+        // when none outgrew it (the usual case), redone below when some did.
+        RTT_TRY(finish_sizes());
+        sized = true;
         const int n_large = static_cast<int>(hctr[6] & 0xffffffffu);
         if (n_large > 0) {
+            sized = false;
+            hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(2), 0, s, ctr + 2, int64_t(2));
+            hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(2), 0, s, ctr + 8, int64_t(2));
             hipLaunchKernelGGL(symbolic_count_kernel, dim3(n_large), dim3(kThreads), 0, s, g,
                                stage_limit, st, w->l3_rows.as<int>(),
                                w->row_tiles.as<int64_t>(), w->row_nnz.as<int64_t>(),
@@ -2151,9 +2182,11 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
                                    hipMemcpyDeviceToHost, s));
             RTT_HIP(hipStreamSynchronize(s));
             n_slab = static_cast<int>(hctr[1] & 0xffffffffu);
+            RTT_HIP(hipEventRecord(ev->ev[1], s));
         }
+    } else {
+        RTT_HIP(hipEventRecord(ev->ev[1], s));
     }
-    RTT_HIP(hipEventRecord(ev->ev[1], s));
     int slab_blocks = 0;
     const int *slab_rows = nullptr;
     const int static_rows = lb_mode == RTT_LB_STATIC ? 1 : 0;
@@ -2197,26 +2230,8 @@ int spgemm_core(const DevTiles &a, const DevTiles &b, DevC *c, SpgemmWork *w, Ev
                            w->row_nnz.as<int64_t>(), ss, none);
         RTT_HIP(hipGetLastError());
     }
-    RTT_TRY(exclusive_scan_i64(w->row_tiles.as<int64_t>(), c->tile_ptr.as<int64_t>(), R,
-                               &w->scan_tmp, s));
-    RTT_TRY(exclusive_scan_i64(w->row_nnz.as<int64_t>(), w->c_row_nnz_off.as<int64_t>(), R,
-                               &w->scan_tmp, s));
-    // Numeric row routing needs only the per-row sizes: list the fallback
-    // rows now and read every count back with one copy.  Rows of the
-    // tile-pair kernel are not listed.
-    if (R > 0 && any_vector)
-        hipLaunchKernelGGL(collect_rows_kernel, dim3(blocks_for(R, kThreads)), dim3(kThreads), 0,
-                           s, R, 1, w->row_level.as<int>(), c->tile_ptr.as<int64_t>(),
-                           w->c_row_nnz_off.as<int64_t>(), g.a_trp, g.a_off,
-                           w->num_rows.as<int>(), num_slab_count, w->lds_rows.as<int>(),
-                           num_lds_count, g.tile_fill, w->tiny_rows.as<int>(), num_tiny_count,
-                           w->mid_rows.as<int>(), num_mid_count);
-    hipLaunchKernelGGL(set_last2_kernel, dim3(1), dim3(1), 0, s,
-                       reinterpret_cast<int64_t *>(ctr + 4), c->tile_ptr.as<int64_t>() + R,
-                       reinterpret_cast<int64_t *>(ctr + 5), w->c_row_nnz_off.as<int64_t>() + R);
-    RTT_HIP(hipMemcpyAsync(hctr, ctr, kCounters * sizeof(unsigned long long),
-                           hipMemcpyDeviceToHost, s));
-    RTT_HIP(hipStreamSynchronize(s));
+    if (!sized)
+        RTT_TRY(finish_sizes());
     w->slab_stage_need = std::max<uint64_t>(w->slab_stage_need, hctr[7]);
     const unsigned long long pairs = hctr[0];
     const int n_num_slab = static_cast<int>(hctr[2] & 0xffffffffu);
