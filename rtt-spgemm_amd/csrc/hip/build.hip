@@ -689,6 +689,121 @@ __global__ __launch_bounds__(kThreads) void pair_extent_kernel(
     }
 }
 
+// Stable partition of each B row's tiles by class (tiles arrive in column
+// order): the order a stable sort by (row, class) gives.  Rows of up to
+// kPartRow tiles take one thread, which packs its per-class counts into the
+// bytes of a 64-bit word; longer rows are listed for a wave each.
+constexpr int kPartRow = 64;
+constexpr int kSlabRunsB = 5;   // classes in use: heavy, light k = 0..3
+
+__device__ __forceinline__ int meta_class(const int4 &m)
+{
+    return b_tile_class(static_cast<uint32_t>(m.y), static_cast<uint32_t>(m.z));
+}
+
+__global__ void class_partition_kernel(int R, const int *__restrict__ trp,
+                                       const int4 *__restrict__ meta, int4 *__restrict__ out,
+                                       int4 *__restrict__ plane0, int *__restrict__ groups,
+                                       int *__restrict__ long_rows, int *__restrict__ n_long)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > R)
+        return;
+    const int s = trp[r];
+    if (r == R) {
+        groups[static_cast<int64_t>(kClassSlots) * R] = s;
+        return;
+    }
+    const int e = trp[r + 1];
+    if (e - s > kPartRow) {
+        long_rows[atomicAdd(n_long, 1)] = r;
+        return;
+    }
+    unsigned long long cnt = 0;   // byte c: tiles of class c
+    for (int x = s; x < e; ++x)
+        cnt += 1ull << (8 * meta_class(meta[x]));
+    // Byte c: tiles of classes below c (at most kPartRow, so no carries).
+    unsigned long long pos = (cnt << 8) * 0x0101010101010101ull;
+    int *g = groups + static_cast<int64_t>(kClassSlots) * r;
+#pragma unroll
+    for (int c = 0; c < kClassSlots; ++c)
+        g[c] = s + static_cast<int>((pos >> (8 * c)) & 0xFFu);
+    for (int x = s; x < e; ++x) {
+        const int4 m = meta[x];
+        const int c = meta_class(m);
+        const int p = s + static_cast<int>((pos >> (8 * c)) & 0xFFu);
+        pos += 1ull << (8 * c);
+        out[p] = m;
+        plane0[p] = m;
+    }
+}
+
+// The listed long rows, a wave each: class counts by ballot, then every
+// tile placed after the row's tiles of lower classes and the earlier tiles
+// of its own.
+__global__ void __launch_bounds__(kThreads)
+    class_partition_long_kernel(const int *__restrict__ trp, const int4 *__restrict__ meta,
+                                int4 *__restrict__ out, int4 *__restrict__ plane0,
+                                int *__restrict__ groups, const int *__restrict__ long_rows,
+                                const int *__restrict__ n_long)
+{
+    const int lane = lane_id();
+    const int waves = gridDim.x * (blockDim.x / kWave);
+    const int count = *n_long;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < count; i += waves) {
+        const int r = long_rows[i];
+        const int s = trp[r], e = trp[r + 1];
+        int run[kSlabRunsB];
+#pragma unroll
+        for (int c = 0; c < kSlabRunsB; ++c)
+            run[c] = 0;
+        for (int x0 = s; x0 < e; x0 += kWave) {
+            const int x = x0 + lane;
+            const int c = x < e ? meta_class(meta[x]) : -1;
+#pragma unroll
+            for (int k = 0; k < kSlabRunsB; ++k)
+                run[k] += __popcll(__ballot(c == k));
+        }
+        int start = s;
+#pragma unroll
+        for (int k = 0; k < kSlabRunsB; ++k) {
+            const int nk = run[k];
+            run[k] = start;
+            start += nk;
+        }
+        if (lane < kClassSlots) {
+            int v = e;
+#pragma unroll
+            for (int k = 0; k < kSlabRunsB; ++k)
+                if (lane == k)
+                    v = run[k];
+            groups[static_cast<int64_t>(kClassSlots) * r + lane] = v;
+        }
+        for (int x0 = s; x0 < e; x0 += kWave) {
+            const int x = x0 + lane;
+            int4 m = make_int4(0, 0, 0, 0);
+            int c = -1;
+            if (x < e) {
+                m = meta[x];
+                c = meta_class(m);
+            }
+            int p = 0;
+#pragma unroll
+            for (int k = 0; k < kSlabRunsB; ++k) {
+                const uint64_t b = __ballot(c == k);
+                if (c == k)
+                    p = run[k] + __popcll(b & below);
+                run[k] += __popcll(b);
+            }
+            if (x < e) {
+                out[p] = m;
+                plane0[p] = m;
+            }
+        }
+    }
+}
+
 // Sort key of a B tile: its row times kClassSlots plus its class.
 __global__ void class_keys_kernel(int64_t n, const int *__restrict__ row,
                                   const int4 *__restrict__ meta, int *__restrict__ keys)
@@ -988,32 +1103,29 @@ int order_by_class(DevTiles *b, BuildWork *w)
     const int R = b->tile_row_count;
     const int64_t slots = static_cast<int64_t>(kClassSlots) * R;
     RTT_TRY(b->row_groups.ensure((slots + 1) * sizeof(int)));
-    RTT_TRY(w->keys_in.ensure((n > 0 ? n : 1) * sizeof(int)));
-    RTT_TRY(w->keys_out.ensure((n > 0 ? n : 1) * sizeof(int)));
-    if (n > 0) {
-        RTT_TRY(fill_tile_row_of(b));
-        RTT_TRY(w->ids_in.ensure(n * sizeof(int)));
-        RTT_TRY(w->ids_out.ensure(n * sizeof(int)));
-        RTT_TRY(w->meta2.ensure(n * sizeof(int4)));
-        RTT_TRY(b->records.ensure(n * sizeof(BRecord)));
-        hipLaunchKernelGGL(class_keys_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
-                           n, b->tile_row_of.as<int>(), b->meta.as<int4>(), w->keys_in.as<int>());
-        hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s,
-                           w->ids_in.as<int>(), n);
-        RTT_TRY(sort_pairs(w->keys_in.as<int>(), w->keys_out.as<int>(), w->ids_in.as<int>(),
-                           w->ids_out.as<int>(), n, bits_for(slots), &w->scan_tmp, s));
-        // Meta in class order, also as record plane 0; the value planes
-        // follow from it in ensure_record_values.
-        hipLaunchKernelGGL(gather_meta_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0,
-                           s, n, w->ids_out.as<int>(), b->meta.as<int4>(), w->meta2.as<int4>(),
-                           b->records.as<int4>());
+    RTT_TRY(w->keys_in.ensure((static_cast<size_t>(R) + 1) * sizeof(int)));   // long rows
+    RTT_TRY(w->keys_out.ensure(sizeof(int)));                                 // their count
+    RTT_TRY(w->meta2.ensure((n > 0 ? n : 1) * sizeof(int4)));
+    RTT_TRY(b->records.ensure((n > 0 ? n : 1) * sizeof(BRecord)));
+    int *n_long = w->keys_out.as<int>();
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(1), 0, s, n_long, int64_t(1), 0);
+    // Meta in class order, also as record plane 0; the value planes follow
+    // from it in ensure_record_values.
+    hipLaunchKernelGGL(class_partition_kernel, dim3(blocks_for(static_cast<int64_t>(R) + 1, kThreads)),
+                       dim3(kThreads), 0, s, R, b->tile_row_ptr.as<int>(), b->meta.as<int4>(),
+                       w->meta2.as<int4>(), b->records.as<int4>(), b->row_groups.as<int>(),
+                       w->keys_in.as<int>(), n_long);
+    if (n > kPartRow) {
+        const int cus = ctx().num_cus > 0 ? ctx().num_cus : 256;
+        hipLaunchKernelGGL(class_partition_long_kernel, dim3(static_cast<unsigned>(2 * cus)),
+                           dim3(kThreads), 0, s, b->tile_row_ptr.as<int>(), b->meta.as<int4>(),
+                           w->meta2.as<int4>(), b->records.as<int4>(), b->row_groups.as<int>(),
+                           w->keys_in.as<int>(), n_long);
+    }
+    if (n > 0)
         RTT_HIP(hipMemcpyAsync(b->meta.ptr, w->meta2.ptr, n * sizeof(int4),
                                hipMemcpyDeviceToDevice, s));
-    }
     b->rec_values_ready = false;
-    hipLaunchKernelGGL(row_ptr_from_sorted_kernel, dim3(blocks_for(n + 1, kThreads)),
-                       dim3(kThreads), 0, s, w->keys_out.as<int>(), n, static_cast<int>(slots),
-                       b->row_groups.as<int>());
     RTT_HIP(hipGetLastError());
     return RTT_OK;
 }
