@@ -584,7 +584,12 @@ __global__ void __launch_bounds__(kThreads)
         s_rp[i] = rp[r0 + i];
     __syncthreads();
     const int base = s_rp[0];
-    const int total = s_rp[nr] - base;   // at most nr * kShortCol
+    const int total = s_rp[nr] - base;
+    // At most nr * kShortCol when every column holds at most kShortCol
+    // tiles, which the host checks before taking this path; the bound keeps
+    // the stage in its LDS whatever the input.
+    if (total > kOrderRows * kShortCol)
+        return;
     for (int e = threadIdx.x; e < total; e += blockDim.x)
         s_ent[e] = ent[base + e];
     __syncthreads();
