@@ -699,6 +699,7 @@ __global__ __launch_bounds__(kThreads) void pair_extent_kernel(
 // kPartRow tiles take one thread, which packs its per-class counts into the
 // bytes of a 64-bit word; longer rows are listed for a wave each.
 constexpr int kPartRow = 64;
+constexpr int kPartBatch = 8;    // metas a thread loads at once
 constexpr int kSlabRunsB = 5;   // classes in use: heavy, light k = 0..3
 
 __device__ __forceinline__ int meta_class(const int4 &m)
@@ -724,22 +725,47 @@ __global__ void class_partition_kernel(int R, const int *__restrict__ trp,
         long_rows[atomicAdd(n_long, 1)] = r;
         return;
     }
+    // The row's metas come kPartBatch at a time, loads issued together; a
+    // row of at most kPartBatch tiles keeps them for the placing pass.
+    int4 m[kPartBatch];
+#pragma unroll
+    for (int u = 0; u < kPartBatch; ++u)
+        m[u] = make_int4(0, 0, 0, 0);
     unsigned long long cnt = 0;   // byte c: tiles of class c
-    for (int x = s; x < e; ++x)
-        cnt += 1ull << (8 * meta_class(meta[x]));
+    for (int x0 = s; x0 < e; x0 += kPartBatch) {
+#pragma unroll
+        for (int u = 0; u < kPartBatch; ++u)
+            if (x0 + u < e)
+                m[u] = meta[x0 + u];
+#pragma unroll
+        for (int u = 0; u < kPartBatch; ++u)
+            if (x0 + u < e)
+                cnt += 1ull << (8 * meta_class(m[u]));
+    }
     // Byte c: tiles of classes below c (at most kPartRow, so no carries).
     unsigned long long pos = (cnt << 8) * 0x0101010101010101ull;
     int *g = groups + static_cast<int64_t>(kClassSlots) * r;
 #pragma unroll
     for (int c = 0; c < kClassSlots; ++c)
         g[c] = s + static_cast<int>((pos >> (8 * c)) & 0xFFu);
-    for (int x = s; x < e; ++x) {
-        const int4 m = meta[x];
-        const int c = meta_class(m);
-        const int p = s + static_cast<int>((pos >> (8 * c)) & 0xFFu);
-        pos += 1ull << (8 * c);
-        out[p] = m;
-        plane0[p] = m;
+    const bool held = e - s <= kPartBatch;
+    for (int x0 = s; x0 < e; x0 += kPartBatch) {
+        if (!held) {
+#pragma unroll
+            for (int u = 0; u < kPartBatch; ++u)
+                if (x0 + u < e)
+                    m[u] = meta[x0 + u];
+        }
+#pragma unroll
+        for (int u = 0; u < kPartBatch; ++u) {
+            if (x0 + u < e) {
+                const int c = meta_class(m[u]);
+                const int p = s + static_cast<int>((pos >> (8 * c)) & 0xFFu);
+                pos += 1ull << (8 * c);
+                out[p] = m[u];
+                plane0[p] = m[u];
+            }
+        }
     }
 }
 
