@@ -622,7 +622,8 @@ __global__ void __launch_bounds__(kThreads)
         groups[static_cast<int64_t>(R) * kClassSlots] = static_cast<int>(n);
 }
 
-// Tiles per tile column, and their maximum in counts[cols].
+// Tiles per tile column; column_max_kernel then puts the largest count in
+// counts[cols].
 __global__ void column_count_kernel(int64_t n, const int *__restrict__ col, int *__restrict__ counts)
 {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -700,7 +701,7 @@ __global__ __launch_bounds__(kThreads) void pair_extent_kernel(
 // bytes of a 64-bit word; longer rows are listed for a wave each.
 constexpr int kPartRow = 64;
 constexpr int kPartBatch = 8;    // metas a thread loads at once
-constexpr int kSlabRunsB = 5;   // classes in use: heavy, light k = 0..3
+constexpr int kClassesUsed = 5;   // heavy, light k = 0..3
 
 __device__ __forceinline__ int meta_class(const int4 &m)
 {
@@ -785,20 +786,20 @@ __global__ void __launch_bounds__(kThreads)
     for (int i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < count; i += waves) {
         const int r = long_rows[i];
         const int s = trp[r], e = trp[r + 1];
-        int run[kSlabRunsB];
+        int run[kClassesUsed];
 #pragma unroll
-        for (int c = 0; c < kSlabRunsB; ++c)
+        for (int c = 0; c < kClassesUsed; ++c)
             run[c] = 0;
         for (int x0 = s; x0 < e; x0 += kWave) {
             const int x = x0 + lane;
             const int c = x < e ? meta_class(meta[x]) : -1;
 #pragma unroll
-            for (int k = 0; k < kSlabRunsB; ++k)
+            for (int k = 0; k < kClassesUsed; ++k)
                 run[k] += __popcll(__ballot(c == k));
         }
         int start = s;
 #pragma unroll
-        for (int k = 0; k < kSlabRunsB; ++k) {
+        for (int k = 0; k < kClassesUsed; ++k) {
             const int nk = run[k];
             run[k] = start;
             start += nk;
@@ -806,7 +807,7 @@ __global__ void __launch_bounds__(kThreads)
         if (lane < kClassSlots) {
             int v = e;
 #pragma unroll
-            for (int k = 0; k < kSlabRunsB; ++k)
+            for (int k = 0; k < kClassesUsed; ++k)
                 if (lane == k)
                     v = run[k];
             groups[static_cast<int64_t>(kClassSlots) * r + lane] = v;
@@ -821,7 +822,7 @@ __global__ void __launch_bounds__(kThreads)
             }
             int p = 0;
 #pragma unroll
-            for (int k = 0; k < kSlabRunsB; ++k) {
+            for (int k = 0; k < kClassesUsed; ++k) {
                 const uint64_t b = __ballot(c == k);
                 if (c == k)
                     p = run[k] + __popcll(b & below);
