@@ -447,10 +447,18 @@ __global__ void record_values_kernel(int64_t n, const int4 *__restrict__ meta,
         return;
     const int4 m = meta[t];
     const int used = record_parts_used(m);
+    // Every value the record holds is loaded before any plane is written.
+    const uint32_t off = static_cast<uint32_t>(m.w) & 0x7FFFFFFFu;
+    const int nv = min(kRecVals, __popc(static_cast<uint32_t>(m.y)));
+    double v[kRecVals];
+#pragma unroll
+    for (int j = 0; j < kRecVals; ++j)
+        v[j] = j < nv ? kmajor[off + j] : 0.0;
 #pragma unroll
     for (int part = 1; part < kRecParts; ++part)
         if (part < used)
-            planes[part * n + t] = record_part(part, m, kmajor);
+            planes[part * n + t] = *reinterpret_cast<const int4 *>(
+                &reinterpret_cast<const double2 *>(v)[part - 1]);
 }
 
 // Values-only update: the first source entry of each slot's run sums the
